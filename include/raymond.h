@@ -1,0 +1,130 @@
+/*
+ * libraymond_hip — the C ABI of Raymond's rendering path on one MI355X.
+ *
+ * This replaces the reference's WebGPU device boundary: the buffer uploads
+ * and compute dispatches that src/Renderer.mjs issues, and the readback in
+ * readPixels(). Plain pointers and sizes only; every host buffer is copied
+ * during the call and never retained. Calls return RM_OK (0) or a negative
+ * RM_ERR_* code; rm_last_error() then says why, in a sentence.
+ *
+ * Packed formats are the reference's (src/Scene.mjs:15-60, src/BVH.mjs:6-15):
+ * 64-byte primitives, 32-byte BVH nodes, 96-byte materials, attribute
+ * vec4s, the 208-byte parameter block (src/shaders.mjs:9-24).
+ */
+#ifndef RAYMOND_H
+#define RAYMOND_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RM_OK 0
+#define RM_ERR_INVALID (-1)   /* a bad argument */
+#define RM_ERR_DEVICE (-2)    /* a HIP runtime error */
+#define RM_ERR_MEMORY (-3)    /* device memory exhausted */
+#define RM_ERR_STATE (-4)     /* a call before what it needs (e.g. rm_frame before rm_resize) */
+
+#define RM_PARAMS_BYTES 208
+
+/* Buffers rm_read copies back, W·H·4 floats each. */
+#define RM_BUFFER_ACCUM 0        /* per pixel Σ colour, Σ luminance² in w */
+#define RM_BUFFER_GUIDE_ALBEDO 1 /* Σ first non-delta albedo */
+#define RM_BUFFER_GUIDE_NORMAL 2 /* Σ first non-delta normal, Σ distance in w */
+#define RM_BUFFER_DENOISED 3     /* the denoiser's output (an average) */
+
+/* Stages timed by rm_stage_times, in this order. */
+#define RM_STAGE_TRACE 0
+#define RM_STAGE_LIGHT_TRACE 1
+#define RM_STAGE_RESOLVE 2
+#define RM_STAGE_GATHER 3
+#define RM_STAGE_PREPARE 4
+#define RM_STAGE_ATROUS 5
+#define RM_STAGE_COMPOSE 6
+#define RM_STAGE_CLEAR 7
+#define RM_STAGE_RAYCAST 8
+#define RM_STAGE_COUNT 9
+
+typedef struct rm_context rm_context;
+typedef struct rm_scene rm_scene;
+
+/* Opens HIP device `device` with its own stream.
+ * Replaces Renderer.create's adapter/device request (src/Renderer.mjs:247-279). */
+int rm_open(int device, rm_context** out);
+
+/* Frees everything the context owns (its scenes included).
+ * Replaces Renderer.dispose (src/Renderer.mjs:531-547). */
+int rm_close(rm_context* ctx);
+
+/* The message for the calling thread's last failure ("" if none). */
+const char* rm_last_error(void);
+
+/* Copies a packed scene to device memory.
+ * primitive_slots counts every 64-byte record (the BVH's primitive_count,
+ * then volumes); lights is the u32 table of light primitives, volume
+ * primitives and the lights' cumulative chances as f32 bits.
+ * Replaces Renderer.#upload's storage buffers (src/Renderer.mjs:550-626). */
+int rm_scene_upload(rm_context* ctx, const void* primitives, uint32_t primitive_slots, uint32_t primitive_count,
+                    const void* nodes, uint32_t node_count, const void* materials, uint32_t material_count,
+                    const float* attributes, uint32_t attribute_vec4s, const uint32_t* lights, uint32_t lights_len,
+                    rm_scene** out);
+
+/* Releases a scene's device memory. */
+int rm_scene_release(rm_context* ctx, rm_scene* scene);
+
+/* RGBA8 texture layers (sRGB colour when normal_maps is 0, plain normal
+ * maps when 1), width × height × layers × 4 bytes, for a scene's materials.
+ * Replaces Renderer.#uploadTextures (src/Renderer.mjs:637-661). */
+int rm_scene_textures(rm_context* ctx, rm_scene* scene, int normal_maps, const uint8_t* rgba, uint32_t width,
+                      uint32_t height, uint32_t layers);
+
+/* The environment map (RGBA half floats, width × height) and its importance
+ * table (table_width × (table_height + 1) floats).
+ * Replaces Renderer.#uploadEnvironment (src/Renderer.mjs:723-760). */
+int rm_environment(rm_context* ctx, const uint16_t* rgba_half, uint32_t width, uint32_t height, const float* table,
+                   uint32_t table_width, uint32_t table_height);
+
+/* Sizes the per-pixel buffers for width × height (reused while they fit).
+ * Replaces Renderer.#resize (src/Renderer.mjs:766-799). */
+int rm_resize(rm_context* ctx, uint32_t width, uint32_t height);
+
+/* Enqueues one frame: the path-tracing pass, then light tracing, splat
+ * resolve and photon gather when params.caustics is set (emitters must then
+ * be given, emitter_vec4s of them), then the denoiser when params.denoise is
+ * set. Returns once the work is queued.
+ * Replaces the compute pass of Renderer.render (src/Renderer.mjs:441-489). */
+int rm_frame(rm_context* ctx, const rm_scene* scene, const void* params, const float* emitters,
+             uint32_t emitter_vec4s);
+
+/* Waits for the queued work and copies `floats` floats of a per-pixel buffer.
+ * Replaces readPixels' copy and map (src/Renderer.mjs:502-528). */
+int rm_read(rm_context* ctx, int buffer, float* dst, size_t floats);
+
+/* Waits for all queued work (device.queue.onSubmittedWorkDone()). */
+int rm_synchronize(rm_context* ctx);
+
+/* Traces n rays (8 floats each: origin, t_max, direction, unused) and
+ * writes 4 words per ray (t f32, primitive u32 or 0xffffffff, u f32, v f32);
+ * any_hit stops at the first hit found. counters, if not NULL, receive 2
+ * words per ray: node records and primitive records tested.
+ * Replaces trace() as Scene.raycast mirrors it (src/shaders.mjs:389-447,
+ * src/Scene.mjs:202-283). */
+int rm_raycast(rm_context* ctx, const rm_scene* scene, const float* rays, uint32_t n, int any_hit, uint32_t* hits,
+               uint32_t* counters);
+
+/* Per-stage HIP event timing: enable or disable, read (waits; fills ms and
+ * launch counts for RM_STAGE_COUNT stages, accumulated since the last
+ * reset), and reset. No counterpart in the reference, which has no GPU timers. */
+int rm_timing(rm_context* ctx, int enabled);
+int rm_stage_times(rm_context* ctx, double* ms, uint32_t* launches, uint32_t n);
+int rm_reset_stage_times(rm_context* ctx);
+
+/* The device's name and compute-unit count. */
+int rm_device_info(rm_context* ctx, char* name, size_t name_len, int* compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAYMOND_H */

@@ -1,0 +1,350 @@
+/*
+ * raymond.node — the thin N-API addon between the JS Renderer
+ * (raymond_amd/js/renderer.mjs) and libraymond_hip.so. Each export maps one
+ * C-ABI call; failures throw a JS Error with the library's message. Reads
+ * and waits run on a worker thread and resolve a Promise, as WebGPU's
+ * mapAsync and onSubmittedWorkDone do for the reference's Renderer.
+ */
+#include <node_api.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "raymond.h"
+
+#define CHECK(call)                                                  \
+    do {                                                             \
+        if ((call) != napi_ok) {                                     \
+            napi_throw_error(env, NULL, "N-API call failed: " #call); \
+            return NULL;                                             \
+        }                                                            \
+    } while (0)
+
+static napi_value throw_rm(napi_env env, int code) {
+    char msg[512];
+    snprintf(msg, sizeof msg, "%s (code %d)", rm_last_error(), code);
+    napi_throw_error(env, NULL, msg);
+    return NULL;
+}
+
+static int args(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
+    size_t argc = want;
+    if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok) return 0;
+    if (argc < want) {
+        napi_throw_type_error(env, NULL, "Too few arguments.");
+        return 0;
+    }
+    return 1;
+}
+
+static int get_u32(napi_env env, napi_value v, uint32_t* out) {
+    if (napi_get_value_uint32(env, v, out) != napi_ok) {
+        napi_throw_type_error(env, NULL, "Expected a number.");
+        return 0;
+    }
+    return 1;
+}
+
+static int get_ptr(napi_env env, napi_value v, void** out) {
+    if (napi_get_value_external(env, v, out) != napi_ok || !*out) {
+        napi_throw_type_error(env, NULL, "Expected a Raymond handle.");
+        return 0;
+    }
+    return 1;
+}
+
+/* Data and byte length of an ArrayBuffer or typed array; null gives NULL. */
+static int get_bytes(napi_env env, napi_value v, void** data, size_t* bytes) {
+    napi_valuetype t;
+    *data = NULL;
+    *bytes = 0;
+    if (napi_typeof(env, v, &t) != napi_ok) return 0;
+    if (t == napi_null || t == napi_undefined) return 1;
+    bool is_ab = false, is_ta = false;
+    napi_is_arraybuffer(env, v, &is_ab);
+    if (is_ab) return napi_get_arraybuffer_info(env, v, data, bytes) == napi_ok;
+    napi_is_typedarray(env, v, &is_ta);
+    if (is_ta) {
+        napi_typedarray_type type;
+        size_t length, offset;
+        napi_value ab;
+        void* base;
+        if (napi_get_typedarray_info(env, v, &type, &length, &base, &ab, &offset) != napi_ok) return 0;
+        size_t elem = 1;
+        switch (type) {
+        case napi_uint16_array: case napi_int16_array: elem = 2; break;
+        case napi_uint32_array: case napi_int32_array: case napi_float32_array: elem = 4; break;
+        case napi_float64_array: elem = 8; break;
+        default: elem = 1;
+        }
+        *data = base;
+        *bytes = length * elem;
+        return 1;
+    }
+    napi_throw_type_error(env, NULL, "Expected an ArrayBuffer or typed array.");
+    return 0;
+}
+
+static napi_value make_external(napi_env env, void* p) {
+    napi_value out;
+    CHECK(napi_create_external(env, p, NULL, NULL, &out));
+    return out;
+}
+
+static napi_value undefined(napi_env env) {
+    napi_value u;
+    napi_get_undefined(env, &u);
+    return u;
+}
+
+static napi_value js_open(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    uint32_t device = 0;
+    size_t argc = 1;
+    CHECK(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc >= 1 && !get_u32(env, argv[0], &device)) return NULL;
+    rm_context* ctx = NULL;
+    int r = rm_open((int)device, &ctx);
+    if (r != RM_OK) return throw_rm(env, r);
+    return make_external(env, ctx);
+}
+
+static napi_value js_close(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    void* ctx;
+    if (!args(env, info, 1, argv) || !get_ptr(env, argv[0], &ctx)) return NULL;
+    int r = rm_close((rm_context*)ctx);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+
+static napi_value js_upload_scene(napi_env env, napi_callback_info info) {
+    napi_value argv[9];
+    void *ctx, *prims, *nodes, *mats, *attr, *lights;
+    size_t pb, nb, mb, ab, lb;
+    uint32_t prim_count, node_count, mat_count;
+    if (!args(env, info, 9, argv) || !get_ptr(env, argv[0], &ctx)) return NULL;
+    if (!get_bytes(env, argv[1], &prims, &pb) || !get_u32(env, argv[2], &prim_count) ||
+        !get_bytes(env, argv[3], &nodes, &nb) || !get_u32(env, argv[4], &node_count) ||
+        !get_bytes(env, argv[5], &mats, &mb) || !get_u32(env, argv[6], &mat_count) ||
+        !get_bytes(env, argv[7], &attr, &ab) || !get_bytes(env, argv[8], &lights, &lb))
+        return NULL;
+    if (node_count * 32ull > nb || mat_count * 96ull > mb) {
+        napi_throw_range_error(env, NULL, "A scene buffer is shorter than its count.");
+        return NULL;
+    }
+    rm_scene* scene = NULL;
+    int r = rm_scene_upload((rm_context*)ctx, prims, (uint32_t)(pb / 64), prim_count, nodes, node_count, mats,
+                            mat_count, (const float*)attr, (uint32_t)(ab / 16), (const uint32_t*)lights,
+                            (uint32_t)(lb / 4), &scene);
+    if (r != RM_OK) return throw_rm(env, r);
+    return make_external(env, scene);
+}
+
+static napi_value js_release_scene(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    void *ctx, *scene;
+    if (!args(env, info, 2, argv) || !get_ptr(env, argv[0], &ctx) || !get_ptr(env, argv[1], &scene)) return NULL;
+    int r = rm_scene_release((rm_context*)ctx, (rm_scene*)scene);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+
+static napi_value js_upload_textures(napi_env env, napi_callback_info info) {
+    napi_value argv[7];
+    void *ctx, *scene, *data;
+    size_t bytes;
+    bool normal = false;
+    uint32_t w, h, layers;
+    if (!args(env, info, 7, argv) || !get_ptr(env, argv[0], &ctx) || !get_ptr(env, argv[1], &scene)) return NULL;
+    napi_get_value_bool(env, argv[2], &normal);
+    if (!get_bytes(env, argv[3], &data, &bytes) || !get_u32(env, argv[4], &w) || !get_u32(env, argv[5], &h) ||
+        !get_u32(env, argv[6], &layers))
+        return NULL;
+    if ((uint64_t)w * h * layers * 4 > bytes) {
+        napi_throw_range_error(env, NULL, "The texture data is shorter than its size.");
+        return NULL;
+    }
+    int r = rm_scene_textures((rm_context*)ctx, (rm_scene*)scene, normal ? 1 : 0, (const uint8_t*)data, w, h, layers);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+
+static napi_value js_upload_environment(napi_env env, napi_callback_info info) {
+    napi_value argv[7];
+    void *ctx, *rgba, *table;
+    size_t rb, tb;
+    uint32_t w, h, tw, th;
+    if (!args(env, info, 7, argv) || !get_ptr(env, argv[0], &ctx)) return NULL;
+    if (!get_bytes(env, argv[1], &rgba, &rb) || !get_u32(env, argv[2], &w) || !get_u32(env, argv[3], &h) ||
+        !get_bytes(env, argv[4], &table, &tb) || !get_u32(env, argv[5], &tw) || !get_u32(env, argv[6], &th))
+        return NULL;
+    if ((uint64_t)w * h * 8 > rb || (uint64_t)tw * (th + 1) * 4 > tb) {
+        napi_throw_range_error(env, NULL, "The environment data is shorter than its size.");
+        return NULL;
+    }
+    int r = rm_environment((rm_context*)ctx, (const uint16_t*)rgba, w, h, (const float*)table, tw, th);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+
+static napi_value js_resize(napi_env env, napi_callback_info info) {
+    napi_value argv[3];
+    void* ctx;
+    uint32_t w, h;
+    if (!args(env, info, 3, argv) || !get_ptr(env, argv[0], &ctx) || !get_u32(env, argv[1], &w) ||
+        !get_u32(env, argv[2], &h))
+        return NULL;
+    int r = rm_resize((rm_context*)ctx, w, h);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+
+static napi_value js_frame(napi_env env, napi_callback_info info) {
+    napi_value argv[4];
+    void *ctx, *scene, *params, *emitters;
+    size_t pb, eb;
+    if (!args(env, info, 4, argv) || !get_ptr(env, argv[0], &ctx) || !get_ptr(env, argv[1], &scene)) return NULL;
+    if (!get_bytes(env, argv[2], &params, &pb) || !get_bytes(env, argv[3], &emitters, &eb)) return NULL;
+    if (pb < RM_PARAMS_BYTES) {
+        napi_throw_range_error(env, NULL, "The parameter block is 208 bytes.");
+        return NULL;
+    }
+    int r = rm_frame((rm_context*)ctx, (const rm_scene*)scene, params, (const float*)emitters, (uint32_t)(eb / 16));
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+
+/* ---- asynchronous reads and waits ---- */
+
+typedef struct {
+    napi_async_work work;
+    napi_deferred deferred;
+    napi_ref keep;          /* the destination array, alive until done */
+    rm_context* ctx;
+    int buffer;             /* -1: synchronize only */
+    float* dst;
+    size_t floats;
+    int result;
+    char message[512];
+} AsyncJob;
+
+static void job_execute(napi_env env, void* data) {
+    (void)env;
+    AsyncJob* j = (AsyncJob*)data;
+    j->result = j->buffer < 0 ? rm_synchronize(j->ctx) : rm_read(j->ctx, j->buffer, j->dst, j->floats);
+    if (j->result != RM_OK) snprintf(j->message, sizeof j->message, "%s (code %d)", rm_last_error(), j->result);
+}
+
+static void job_complete(napi_env env, napi_status status, void* data) {
+    AsyncJob* j = (AsyncJob*)data;
+    if (status == napi_ok && j->result == RM_OK) {
+        napi_resolve_deferred(env, j->deferred, undefined(env));
+    } else {
+        napi_value msg, err;
+        napi_create_string_utf8(env, j->result != RM_OK ? j->message : "The GPU wait was cancelled.", NAPI_AUTO_LENGTH,
+                                &msg);
+        napi_create_error(env, NULL, msg, &err);
+        napi_reject_deferred(env, j->deferred, err);
+    }
+    if (j->keep) napi_delete_reference(env, j->keep);
+    napi_delete_async_work(env, j->work);
+    free(j);
+}
+
+static napi_value start_job(napi_env env, AsyncJob* j, napi_value keep) {
+    napi_value promise, name;
+    if (napi_create_promise(env, &j->deferred, &promise) != napi_ok) {
+        free(j);
+        napi_throw_error(env, NULL, "Couldn't create a promise.");
+        return NULL;
+    }
+    if (keep) napi_create_reference(env, keep, 1, &j->keep);
+    napi_create_string_utf8(env, "raymond", NAPI_AUTO_LENGTH, &name);
+    napi_create_async_work(env, NULL, name, job_execute, job_complete, j, &j->work);
+    napi_queue_async_work(env, j->work);
+    return promise;
+}
+
+static napi_value js_read(napi_env env, napi_callback_info info) {
+    napi_value argv[3];
+    void *ctx, *dst;
+    size_t bytes;
+    uint32_t which;
+    if (!args(env, info, 3, argv) || !get_ptr(env, argv[0], &ctx) || !get_u32(env, argv[1], &which) ||
+        !get_bytes(env, argv[2], &dst, &bytes))
+        return NULL;
+    AsyncJob* j = (AsyncJob*)calloc(1, sizeof *j);
+    j->ctx = (rm_context*)ctx;
+    j->buffer = (int)which;
+    j->dst = (float*)dst;
+    j->floats = bytes / 4;
+    return start_job(env, j, argv[2]);
+}
+
+static napi_value js_synchronize(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    void* ctx;
+    if (!args(env, info, 1, argv) || !get_ptr(env, argv[0], &ctx)) return NULL;
+    AsyncJob* j = (AsyncJob*)calloc(1, sizeof *j);
+    j->ctx = (rm_context*)ctx;
+    j->buffer = -1;
+    return start_job(env, j, NULL);
+}
+
+static napi_value js_raycast(napi_env env, napi_callback_info info) {
+    napi_value argv[4];
+    void *ctx, *scene, *rays;
+    size_t bytes;
+    bool any = false;
+    if (!args(env, info, 4, argv) || !get_ptr(env, argv[0], &ctx) || !get_ptr(env, argv[1], &scene) ||
+        !get_bytes(env, argv[2], &rays, &bytes))
+        return NULL;
+    napi_get_value_bool(env, argv[3], &any);
+    uint32_t n = (uint32_t)(bytes / 32);
+    void* out_data;
+    napi_value ab, hits;
+    CHECK(napi_create_arraybuffer(env, (size_t)n * 16 + 16, &out_data, &ab));
+    CHECK(napi_create_typedarray(env, napi_uint32_array, (size_t)n * 4, ab, 0, &hits));
+    int r = rm_raycast((rm_context*)ctx, (const rm_scene*)scene, (const float*)rays, n, any ? 1 : 0,
+                       (uint32_t*)out_data, NULL);
+    if (r != RM_OK) return throw_rm(env, r);
+    return hits;
+}
+
+static napi_value js_device_info(napi_env env, napi_callback_info info) {
+    napi_value argv[1], obj, name, cus;
+    void* ctx;
+    char buf[256];
+    int n = 0;
+    if (!args(env, info, 1, argv) || !get_ptr(env, argv[0], &ctx)) return NULL;
+    int r = rm_device_info((rm_context*)ctx, buf, sizeof buf, &n);
+    if (r != RM_OK) return throw_rm(env, r);
+    CHECK(napi_create_object(env, &obj));
+    CHECK(napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &name));
+    CHECK(napi_create_int32(env, n, &cus));
+    CHECK(napi_set_named_property(env, obj, "name", name));
+    CHECK(napi_set_named_property(env, obj, "computeUnits", cus));
+    return obj;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+    napi_property_descriptor props[] = {
+        {"open", NULL, js_open, NULL, NULL, NULL, napi_default, NULL},
+        {"close", NULL, js_close, NULL, NULL, NULL, napi_default, NULL},
+        {"uploadScene", NULL, js_upload_scene, NULL, NULL, NULL, napi_default, NULL},
+        {"releaseScene", NULL, js_release_scene, NULL, NULL, NULL, napi_default, NULL},
+        {"uploadTextures", NULL, js_upload_textures, NULL, NULL, NULL, napi_default, NULL},
+        {"uploadEnvironment", NULL, js_upload_environment, NULL, NULL, NULL, napi_default, NULL},
+        {"resize", NULL, js_resize, NULL, NULL, NULL, napi_default, NULL},
+        {"frame", NULL, js_frame, NULL, NULL, NULL, napi_default, NULL},
+        {"read", NULL, js_read, NULL, NULL, NULL, napi_default, NULL},
+        {"synchronize", NULL, js_synchronize, NULL, NULL, NULL, napi_default, NULL},
+        {"raycast", NULL, js_raycast, NULL, NULL, NULL, napi_default, NULL},
+        {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_default, NULL},
+    };
+    if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return NULL;
+    return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

@@ -1,0 +1,174 @@
+"""ctypes binding of libraymond_hip.so (include/raymond.h) for Python harnesses.
+
+The JS Renderer reaches the same library through raymond.node; this module
+is how the Python tests and bench drive it. There is no fallback: if the
+library is missing or no GPU is present, opening a context raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libraymond_hip.so"
+
+RM_OK = 0
+BUFFER_ACCUM, BUFFER_GUIDE_ALBEDO, BUFFER_GUIDE_NORMAL, BUFFER_DENOISED = 0, 1, 2, 3
+STAGES = ("trace", "light_trace", "resolve", "gather", "prepare", "atrous", "compose", "clear", "raycast")
+
+# Every function include/raymond.h declares, with its ctypes signature.
+_P = C.c_void_p
+_U32 = C.c_uint32
+SIGNATURES = {
+    "rm_open": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "rm_close": (C.c_int, [_P]),
+    "rm_last_error": (C.c_char_p, []),
+    "rm_scene_upload": (C.c_int, [_P, _P, _U32, _U32, _P, _U32, _P, _U32, _P, _U32, _P, _U32, C.POINTER(_P)]),
+    "rm_scene_release": (C.c_int, [_P, _P]),
+    "rm_scene_textures": (C.c_int, [_P, _P, C.c_int, _P, _U32, _U32, _U32]),
+    "rm_environment": (C.c_int, [_P, _P, _U32, _U32, _P, _U32, _U32]),
+    "rm_resize": (C.c_int, [_P, _U32, _U32]),
+    "rm_frame": (C.c_int, [_P, _P, _P, _P, _U32]),
+    "rm_read": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    "rm_synchronize": (C.c_int, [_P]),
+    "rm_raycast": (C.c_int, [_P, _P, _P, _U32, C.c_int, _P, _P]),
+    "rm_timing": (C.c_int, [_P, C.c_int]),
+    "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
+    "rm_reset_stage_times": (C.c_int, [_P]),
+    "rm_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
+}
+
+_lib = None
+
+
+class RaymondError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Loads the library, failing loudly when it isn't built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RaymondError(f"{LIB_PATH} is not built: run python -m raymond_amd.build")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, argt) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = argt
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(code: int, what: str) -> None:
+    if code != RM_OK:
+        raise RaymondError(f"{what}: {load().rm_last_error().decode()} (code {code})")
+
+
+class Scene:
+    def __init__(self, ctx: "Context", handle: C.c_void_p):
+        self.ctx = ctx
+        self.handle = handle
+
+    def release(self) -> None:
+        if self.handle:
+            _check(load().rm_scene_release(self.ctx.handle, self.handle), "rm_scene_release")
+            self.handle = None
+
+
+class Context:
+    """One device context: a stream, per-pixel buffers and uploaded scenes."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.rm_open(device, C.byref(h)), "rm_open")
+        self.handle = h
+        self.width = self.height = 0
+
+    def close(self) -> None:
+        if self.handle:
+            _check(load().rm_close(self.handle), "rm_close")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload_scene(self, packed) -> Scene:
+        """packed: a raymond_amd.scenefile.Packed (or anything with the same arrays)."""
+        lib = load()
+        h = C.c_void_p()
+        prims = np.ascontiguousarray(packed.primitives, dtype=np.uint32)
+        nodes = np.ascontiguousarray(packed.nodes, dtype=np.uint32)
+        mats = np.ascontiguousarray(packed.materials, dtype=np.uint32)
+        attr = np.ascontiguousarray(packed.attributes, dtype=np.float32)
+        lights = np.ascontiguousarray(packed.lights, dtype=np.uint32)
+        _check(lib.rm_scene_upload(self.handle, _ptr(prims), prims.size // 16, packed.primitive_count,
+                                   _ptr(nodes), nodes.size // 8, _ptr(mats), mats.size // 24,
+                                   _ptr(attr), attr.size // 4, _ptr(lights), lights.size, C.byref(h)),
+               "rm_scene_upload")
+        return Scene(self, h)
+
+    def environment(self, rgba_half: np.ndarray, width: int, height: int, table: np.ndarray, tw: int, th: int) -> None:
+        rgba = np.ascontiguousarray(rgba_half, dtype=np.uint16)
+        tab = np.ascontiguousarray(table, dtype=np.float32)
+        _check(load().rm_environment(self.handle, _ptr(rgba), width, height, _ptr(tab), tw, th), "rm_environment")
+
+    def textures(self, scene: Scene, normal_maps: bool, rgba: np.ndarray, width: int, height: int, layers: int):
+        data = np.ascontiguousarray(rgba, dtype=np.uint8)
+        _check(load().rm_scene_textures(self.handle, scene.handle, int(normal_maps), _ptr(data), width, height, layers),
+               "rm_scene_textures")
+
+    def resize(self, width: int, height: int) -> None:
+        _check(load().rm_resize(self.handle, width, height), "rm_resize")
+        self.width, self.height = width, height
+
+    def frame(self, scene: Scene, params: np.ndarray, emitters: np.ndarray | None = None) -> None:
+        p = np.ascontiguousarray(params, dtype=np.uint8)
+        e = None if emitters is None else np.ascontiguousarray(emitters, dtype=np.float32)
+        _check(load().rm_frame(self.handle, scene.handle, _ptr(p), _ptr(e), 0 if e is None else e.size // 4),
+               "rm_frame")
+
+    def read(self, buffer: int = BUFFER_ACCUM) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), dtype=np.float32)
+        _check(load().rm_read(self.handle, buffer, _ptr(out), out.size), "rm_read")
+        return out
+
+    def synchronize(self) -> None:
+        _check(load().rm_synchronize(self.handle), "rm_synchronize")
+
+    def raycast(self, scene: Scene, rays: np.ndarray, any_hit: bool = False, counters: bool = False):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        hits = np.empty((n, 4), dtype=np.uint32)
+        cnt = np.empty((n, 2), dtype=np.uint32) if counters else None
+        _check(load().rm_raycast(self.handle, scene.handle, _ptr(r), n, int(any_hit), _ptr(hits), _ptr(cnt)),
+               "rm_raycast")
+        return (hits, cnt) if counters else hits
+
+    def timing(self, enabled: bool) -> None:
+        _check(load().rm_timing(self.handle, int(enabled)), "rm_timing")
+
+    def stage_times(self) -> dict:
+        n = len(STAGES)
+        ms = (C.c_double * n)()
+        launches = (C.c_uint32 * n)()
+        _check(load().rm_stage_times(self.handle, ms, launches, n), "rm_stage_times")
+        return {s: (ms[i], launches[i]) for i, s in enumerate(STAGES)}
+
+    def reset_stage_times(self) -> None:
+        _check(load().rm_reset_stage_times(self.handle), "rm_reset_stage_times")
+
+    def device_info(self) -> tuple[str, int]:
+        buf = C.create_string_buffer(256)
+        cus = C.c_int()
+        _check(load().rm_device_info(self.handle, buf, 256, C.byref(cus)), "rm_device_info")
+        return buf.value.decode(), cus.value

@@ -1,0 +1,537 @@
+// The C ABI (include/raymond.h) over the kernels: device memory, the frame's
+// dispatch order (src/Renderer.mjs:441-489), readback and stage timing.
+#include <raymond.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& message) {
+    g_error = message;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    int code = (e == hipErrorOutOfMemory) ? RM_ERR_MEMORY : RM_ERR_DEVICE;
+    return fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define RM_HIP(call, what)                              \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+// A growable device allocation.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, need);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct StageTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[RM_STAGE_COUNT];
+    std::vector<hipEvent_t> pool;
+    double ms[RM_STAGE_COUNT] = {};
+    uint32_t launches[RM_STAGE_COUNT] = {};
+};
+
+}  // namespace
+
+struct rm_scene {
+    DevBuf prims, nodes, mats, attr, lights, textures, nmaps;
+    uint32_t primitive_slots = 0, primitive_count = 0, node_count = 0, material_count = 0;
+    uint32_t tex_w = 0, tex_h = 0, tex_layers = 0, nmap_w = 0, nmap_h = 0, nmap_layers = 0;
+    bool dispersive = false;
+    std::vector<uint32_t> lights_host;   // the lights table, for checking parameters against
+};
+
+struct rm_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t width = 0, height = 0, capacity = 0;
+    DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats;
+    DevBuf heads, photons, emitters, spill;
+    DevBuf env, env_table;
+    uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
+    DevBuf rays, hits, counters;
+    bool timing = false;
+    StageTimer timer;
+    std::vector<rm_scene*> scenes;
+};
+
+namespace {
+
+hipEvent_t timer_event(rm_context* c) {
+    if (!c->timer.pool.empty()) {
+        hipEvent_t e = c->timer.pool.back();
+        c->timer.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one launch with events on the context's stream when timing.
+struct Timed {
+    rm_context* c;
+    int stage;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(rm_context* ctx, int s) : c(ctx), stage(s) {
+        if (c->timing) {
+            a = timer_event(c);
+            b = timer_event(c);
+            if (a && b) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->timing && a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->timer.pending[stage].push_back({a, b});
+        }
+    }
+};
+
+int collect_times(rm_context* c) {
+    RM_HIP(hipStreamSynchronize(c->stream), "waiting for the stream");
+    for (int s = 0; s < RM_STAGE_COUNT; ++s) {
+        for (auto& p : c->timer.pending[s]) {
+            float ms = 0.0f;
+            RM_HIP(hipEventElapsedTime(&ms, p.first, p.second), "reading a stage timer");
+            c->timer.ms[s] += ms;
+            c->timer.launches[s] += 1;
+            c->timer.pool.push_back(p.first);
+            c->timer.pool.push_back(p.second);
+        }
+        c->timer.pending[s].clear();
+    }
+    return RM_OK;
+}
+
+rm::DevScene dev_scene(const rm_context* c, const rm_scene* s) {
+    rm::DevScene S;
+    S.prims = s->prims.as<const float4>();
+    S.nodes = s->nodes.as<const float4>();
+    S.mats = s->mats.as<const float4>();
+    S.attr = s->attr.as<const float4>();
+    S.lights = s->lights.as<const uint32_t>();
+    S.env = c->env.as<const ushort4>();
+    S.env_w = c->env_w;
+    S.env_h = c->env_h;
+    S.env_table = c->env_table.as<const float>();
+    S.table_w = c->table_w;
+    S.table_h = c->table_h;
+    S.textures = s->textures.as<const uchar4>();
+    S.tex_w = s->tex_w;
+    S.tex_h = s->tex_h;
+    S.tex_layers = s->tex_layers;
+    S.nmaps = s->nmaps.as<const uchar4>();
+    S.nmap_w = s->nmap_w;
+    S.nmap_h = s->nmap_h;
+    S.nmap_layers = s->nmap_layers;
+    return S;
+}
+
+int upload(rm_context* c, DevBuf& b, const void* src, size_t bytes, const char* what) {
+    size_t alloc = bytes < 16 ? 16 : bytes;
+    RM_HIP(b.ensure(alloc), what);
+    if (bytes) RM_HIP(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, c->stream), what);
+    return RM_OK;
+}
+
+int ensure_spill(rm_context* c, uint32_t slots) {
+    RM_HIP(c->spill.ensure((size_t)slots * rm::SPILL_ENTRIES * sizeof(uint32_t)), "allocating the traversal spill");
+    return RM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rm_last_error(void) { return g_error.c_str(); }
+
+int rm_open(int device, rm_context** out) {
+    if (!out) return fail(RM_ERR_INVALID, "rm_open needs somewhere to put the context.");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(RM_ERR_DEVICE, "No HIP device is available.");
+    if (device < 0 || device >= count)
+        return fail(RM_ERR_INVALID, "Device " + std::to_string(device) + " doesn't exist (" + std::to_string(count) +
+                                        " available).");
+    RM_HIP(hipSetDevice(device), "selecting the device");
+    rm_context* c = new rm_context();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "creating a stream");
+    }
+    // A 1×1 black environment and an empty table stand in until one is given.
+    uint16_t black[4] = {0, 0, 0, 0x3c00};
+    float table[1] = {0.0f};
+    int r = rm_environment(c, black, 1, 1, table, 1, 0);
+    if (r != RM_OK) {
+        rm_close(c);
+        return r;
+    }
+    *out = c;
+    return RM_OK;
+}
+
+int rm_close(rm_context* c) {
+    if (!c) return fail(RM_ERR_INVALID, "rm_close needs a context.");
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (rm_scene* s : c->scenes) {
+        for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights, &s->textures, &s->nmaps}) b->release();
+        delete s;
+    }
+    for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->heads,
+                      &c->photons, &c->emitters, &c->spill, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters})
+        b->release();
+    for (int s = 0; s < RM_STAGE_COUNT; ++s)
+        for (auto& p : c->timer.pending[s]) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RM_OK;
+}
+
+int rm_scene_upload(rm_context* c, const void* primitives, uint32_t primitive_slots, uint32_t primitive_count,
+                    const void* nodes, uint32_t node_count, const void* materials, uint32_t material_count,
+                    const float* attributes, uint32_t attribute_vec4s, const uint32_t* lights, uint32_t lights_len,
+                    rm_scene** out) {
+    if (!c || !out) return fail(RM_ERR_INVALID, "rm_scene_upload needs a context and an output.");
+    if (!primitives || primitive_slots == 0 || primitive_count > primitive_slots)
+        return fail(RM_ERR_INVALID, "A scene needs at least one primitive slot, and no more in the BVH than slots.");
+    if (!nodes || node_count == 0) return fail(RM_ERR_INVALID, "A scene needs at least one BVH node.");
+    if (!materials && material_count) return fail(RM_ERR_INVALID, "Materials are missing.");
+    // Every node's children and leaf ranges must stay inside the buffers the
+    // kernels index.
+    const uint32_t* nw = static_cast<const uint32_t*>(nodes);
+    for (uint32_t i = 0; i < node_count; ++i) {
+        uint32_t first = nw[i * 8 + 3], count = nw[i * 8 + 7];
+        bool ok = count > 0 ? (uint64_t)first + count <= primitive_slots : (uint64_t)first + 1 < node_count;
+        if (!ok) return fail(RM_ERR_INVALID, "BVH node " + std::to_string(i) + " points outside the scene.");
+    }
+    const uint32_t* pw = static_cast<const uint32_t*>(primitives);
+    for (uint32_t i = 0; i < primitive_slots; ++i) {
+        uint32_t kind = pw[i * 16], mat = pw[i * 16 + 1], attr = pw[i * 16 + 3];
+        if (kind <= 3 && mat >= material_count)
+            return fail(RM_ERR_INVALID, "Primitive " + std::to_string(i) + " has no material.");
+        if (kind == 3 && attr != 0xffffffffu && (uint64_t)attr + 4 > attribute_vec4s)
+            return fail(RM_ERR_INVALID, "Triangle " + std::to_string(i) + " has no attribute block.");
+    }
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    rm_scene* s = new rm_scene();
+    s->primitive_slots = primitive_slots;
+    s->primitive_count = primitive_count;
+    s->node_count = node_count;
+    s->material_count = material_count;
+    const float* mf = static_cast<const float*>(materials);
+    for (uint32_t i = 0; i < material_count; ++i)
+        if (mf[i * 24 + 19] > 0.0f) s->dispersive = true;
+    int r = RM_OK;
+    if (r == RM_OK) r = upload(c, s->prims, primitives, (size_t)primitive_slots * 64, "uploading primitives");
+    if (r == RM_OK) r = upload(c, s->nodes, nodes, (size_t)node_count * 32, "uploading BVH nodes");
+    if (r == RM_OK) r = upload(c, s->mats, materials, (size_t)material_count * 96, "uploading materials");
+    if (r == RM_OK) r = upload(c, s->attr, attributes, (size_t)attribute_vec4s * 16, "uploading attributes");
+    if (r == RM_OK) r = upload(c, s->lights, lights, (size_t)lights_len * 4, "uploading the lights table");
+    if (r == RM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(RM_ERR_DEVICE, "uploading a scene");
+    if (r == RM_OK && lights) s->lights_host.assign(lights, lights + lights_len);
+    if (r != RM_OK) {
+        for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights}) b->release();
+        delete s;
+        return r;
+    }
+    c->scenes.push_back(s);
+    *out = s;
+    return RM_OK;
+}
+
+int rm_scene_release(rm_context* c, rm_scene* s) {
+    if (!c || !s) return fail(RM_ERR_INVALID, "rm_scene_release needs a context and a scene.");
+    for (size_t i = 0; i < c->scenes.size(); ++i)
+        if (c->scenes[i] == s) {
+            (void)hipSetDevice(c->device);
+            (void)hipStreamSynchronize(c->stream);
+            for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights, &s->textures, &s->nmaps})
+                b->release();
+            delete s;
+            c->scenes.erase(c->scenes.begin() + i);
+            return RM_OK;
+        }
+    return fail(RM_ERR_INVALID, "That scene doesn't belong to this context.");
+}
+
+int rm_scene_textures(rm_context* c, rm_scene* s, int normal_maps, const uint8_t* rgba, uint32_t width,
+                      uint32_t height, uint32_t layers) {
+    if (!c || !s || (!rgba && layers)) return fail(RM_ERR_INVALID, "rm_scene_textures needs a context, scene and data.");
+    if (layers && (width == 0 || height == 0)) return fail(RM_ERR_INVALID, "Texture layers need a size.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    DevBuf& b = normal_maps ? s->nmaps : s->textures;
+    int r = upload(c, b, rgba, (size_t)width * height * layers * 4, "uploading textures");
+    if (r != RM_OK) return r;
+    RM_HIP(hipStreamSynchronize(c->stream), "uploading textures");
+    if (normal_maps) { s->nmap_w = width; s->nmap_h = height; s->nmap_layers = layers; }
+    else { s->tex_w = width; s->tex_h = height; s->tex_layers = layers; }
+    return RM_OK;
+}
+
+int rm_environment(rm_context* c, const uint16_t* rgba_half, uint32_t width, uint32_t height, const float* table,
+                   uint32_t table_width, uint32_t table_height) {
+    if (!c || !rgba_half || !table || width == 0 || height == 0 || table_width == 0)
+        return fail(RM_ERR_INVALID, "An environment needs texels and a table.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    int r = upload(c, c->env, rgba_half, (size_t)width * height * 8, "uploading the environment");
+    if (r == RM_OK)
+        r = upload(c, c->env_table, table, (size_t)table_width * (table_height + 1) * 4, "uploading the environment table");
+    if (r != RM_OK) return r;
+    RM_HIP(hipStreamSynchronize(c->stream), "uploading the environment");
+    c->env_w = width;
+    c->env_h = height;
+    c->table_w = table_width;
+    c->table_h = table_height;
+    return RM_OK;
+}
+
+int rm_resize(rm_context* c, uint32_t width, uint32_t height) {
+    if (!c || width == 0 || height == 0) return fail(RM_ERR_INVALID, "An image needs a width and height.");
+    if ((uint64_t)width * height > (1ull << 28)) return fail(RM_ERR_INVALID, "That image is too large.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    uint32_t px = width * height;
+    if (px > c->capacity) {
+        size_t bytes = (size_t)px * 16;
+        for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats}) {
+            b->release();
+            RM_HIP(b->ensure(bytes), "allocating per-pixel buffers");
+        }
+        RM_HIP(hipMemsetAsync(c->accum.ptr, 0, bytes, c->stream), "clearing the accumulation");
+        RM_HIP(hipMemsetAsync(c->galb.ptr, 0, bytes, c->stream), "clearing the guides");
+        RM_HIP(hipMemsetAsync(c->gnorm.ptr, 0, bytes, c->stream), "clearing the guides");
+        RM_HIP(hipMemsetAsync(c->splats.ptr, 0, bytes, c->stream), "clearing the splats");
+        c->capacity = px;
+    }
+    int r = ensure_spill(c, px);
+    if (r != RM_OK) return r;
+    c->width = width;
+    c->height = height;
+    return RM_OK;
+}
+
+int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* emitters, uint32_t emitter_vec4s) {
+    if (!c || !s || !params) return fail(RM_ERR_INVALID, "rm_frame needs a context, a scene and parameters.");
+    rm::Params P;
+    std::memcpy(&P, params, sizeof P);
+    if (c->width == 0) return fail(RM_ERR_STATE, "rm_frame before rm_resize.");
+    if (P.width != c->width || P.height != c->height)
+        return fail(RM_ERR_INVALID, "The parameters' size differs from rm_resize's.");
+    if (P.samples_per_frame == 0) return fail(RM_ERR_INVALID, "A frame needs at least one sample per pixel.");
+    if ((uint64_t)P.light_count * 2 + P.volume_count > s->lights_host.size())
+        return fail(RM_ERR_INVALID, "The lights table is shorter than the parameters' light and volume counts.");
+    for (uint32_t i = 0; i < P.light_count + P.volume_count; ++i)
+        if (s->lights_host[i] >= s->primitive_slots)
+            return fail(RM_ERR_INVALID, "The lights table names a primitive the scene doesn't have.");
+    const bool caustics = P.caustics != 0u;
+    if (caustics && (!emitters || emitter_vec4s < 2))
+        return fail(RM_ERR_INVALID, "Light tracing needs the emitters.");
+    if (caustics) {
+        const uint32_t* ew = reinterpret_cast<const uint32_t*>(emitters);
+        uint64_t need = 1ull + ew[0] + ew[1];
+        if (ew[0] == 0 || need > emitter_vec4s)
+            return fail(RM_ERR_INVALID, "The emitters buffer is shorter than its header says.");
+        for (uint32_t k = 0; k < ew[0]; ++k) {
+            uint32_t prim = ew[(1 + k) * 4 + 1];
+            if (prim != 0xffffffffu && prim >= s->primitive_slots)
+                return fail(RM_ERR_INVALID, "An emitter names a primitive the scene doesn't have.");
+        }
+    }
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+
+    rm::DevScene S = dev_scene(c, s);
+    rm::Features f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
+                      s->dispersive};
+    const uint32_t slots = c->width * c->height;
+    {
+        Timed t(c, RM_STAGE_TRACE);
+        rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(),
+                         c->spill.as<uint32_t>(), slots};
+        rm::launch_main(L, f, c->stream);
+    }
+    RM_HIP(hipGetLastError(), "launching the trace");
+
+    if (caustics) {
+        RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
+        RM_HIP(c->photons.ensure((size_t)rm::PHOTON_CAPACITY * 48), "allocating photons");
+        int r = upload(c, c->emitters, emitters, (size_t)emitter_vec4s * 16, "uploading emitters");
+        if (r != RM_OK) return r;
+        uint32_t count = P.width * P.height * P.samples_per_frame;
+        if (count > rm::PHOTON_CAPACITY || (uint64_t)P.width * P.height * P.samples_per_frame > rm::PHOTON_CAPACITY)
+            count = rm::PHOTON_CAPACITY;
+        r = ensure_spill(c, count > slots ? count : slots);
+        if (r != RM_OK) return r;
+        {
+            Timed t(c, RM_STAGE_CLEAR);
+            rm::launch_fill_u32(c->heads.as<uint32_t>(), 0u, rm::PHOTON_TABLE + 1, c->stream);
+        }
+        {
+            Timed t(c, RM_STAGE_LIGHT_TRACE);
+            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats.as<uint32_t>(),
+                                   c->heads.as<uint32_t>(), c->photons.as<float4>(), c->spill.as<uint32_t>(),
+                                   count > slots ? count : slots, count, f, c->stream);
+        }
+        {
+            Timed t(c, RM_STAGE_RESOLVE);
+            rm::launch_resolve(P, c->splats.as<uint4>(), c->accum.as<float4>(), c->stream);
+        }
+        {
+            Timed t(c, RM_STAGE_GATHER);
+            rm::launch_photon_gather(S, P, c->heads.as<const uint32_t>(), c->photons.as<const float4>(),
+                                     c->accum.as<float4>(), c->spill.as<uint32_t>(), slots, f, c->stream);
+        }
+        RM_HIP(hipGetLastError(), "launching light tracing");
+    }
+
+    if (P.denoise != 0u) {
+        float4* a = c->light_a.as<float4>();
+        float4* b = c->light_b.as<float4>();
+        {
+            Timed t(c, RM_STAGE_PREPARE);
+            rm::launch_prepare(P, c->accum.as<const float4>(), c->galb.as<const float4>(), c->gnorm.as<const float4>(),
+                               a, c->stream);
+        }
+        const int steps[5] = {1, 2, 4, 8, 16};
+        float4* in = a;
+        float4* out = b;
+        for (int k = 0; k < 5; ++k) {
+            Timed t(c, RM_STAGE_ATROUS);
+            rm::launch_atrous(P, steps[k], c->gnorm.as<const float4>(), in, out, c->stream);
+            std::swap(in, out);
+        }
+        {
+            Timed t(c, RM_STAGE_COMPOSE);
+            rm::launch_compose(P, c->galb.as<const float4>(), in, c->denoised.as<float4>(), c->stream);
+        }
+        RM_HIP(hipGetLastError(), "launching the denoiser");
+    }
+    return RM_OK;
+}
+
+int rm_read(rm_context* c, int buffer, float* dst, size_t floats) {
+    if (!c || !dst) return fail(RM_ERR_INVALID, "rm_read needs a context and a destination.");
+    if (c->width == 0) return fail(RM_ERR_STATE, "Nothing has been rendered yet.");
+    DevBuf* b = nullptr;
+    switch (buffer) {
+    case RM_BUFFER_ACCUM: b = &c->accum; break;
+    case RM_BUFFER_GUIDE_ALBEDO: b = &c->galb; break;
+    case RM_BUFFER_GUIDE_NORMAL: b = &c->gnorm; break;
+    case RM_BUFFER_DENOISED: b = &c->denoised; break;
+    default: return fail(RM_ERR_INVALID, "Unknown buffer " + std::to_string(buffer) + ".");
+    }
+    size_t have = (size_t)c->width * c->height * 4;
+    if (floats > have) return fail(RM_ERR_INVALID, "rm_read asked for more than the image holds.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    RM_HIP(hipMemcpyAsync(dst, b->ptr, floats * 4, hipMemcpyDeviceToHost, c->stream), "reading back pixels");
+    RM_HIP(hipStreamSynchronize(c->stream), "reading back pixels");
+    return RM_OK;
+}
+
+int rm_synchronize(rm_context* c) {
+    if (!c) return fail(RM_ERR_INVALID, "rm_synchronize needs a context.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    RM_HIP(hipStreamSynchronize(c->stream), "waiting for the GPU");
+    return RM_OK;
+}
+
+int rm_raycast(rm_context* c, const rm_scene* s, const float* rays, uint32_t n, int any_hit, uint32_t* hits,
+               uint32_t* counters) {
+    if (!c || !s || (!rays && n) || (!hits && n)) return fail(RM_ERR_INVALID, "rm_raycast needs rays and room for hits.");
+    if (n == 0) return RM_OK;
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    RM_HIP(c->rays.ensure((size_t)n * 32), "allocating rays");
+    RM_HIP(c->hits.ensure((size_t)n * 16), "allocating hits");
+    if (counters) RM_HIP(c->counters.ensure((size_t)n * 8), "allocating counters");
+    int r = ensure_spill(c, n > c->width * c->height ? n : c->width * c->height);
+    if (r != RM_OK) return r;
+    RM_HIP(hipMemcpyAsync(c->rays.ptr, rays, (size_t)n * 32, hipMemcpyHostToDevice, c->stream), "uploading rays");
+    {
+        Timed t(c, RM_STAGE_RAYCAST);
+        rm::launch_raycast(dev_scene(c, s), c->rays.as<const float4>(), n, c->hits.as<uint4>(), any_hit != 0,
+                           c->spill.as<uint32_t>(), n, counters ? c->counters.as<uint2>() : nullptr, c->stream);
+    }
+    RM_HIP(hipGetLastError(), "launching the raycast");
+    RM_HIP(hipMemcpyAsync(hits, c->hits.ptr, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream), "reading hits");
+    if (counters)
+        RM_HIP(hipMemcpyAsync(counters, c->counters.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream),
+               "reading counters");
+    RM_HIP(hipStreamSynchronize(c->stream), "tracing rays");
+    return RM_OK;
+}
+
+int rm_timing(rm_context* c, int enabled) {
+    if (!c) return fail(RM_ERR_INVALID, "rm_timing needs a context.");
+    c->timing = enabled != 0;
+    return RM_OK;
+}
+
+int rm_stage_times(rm_context* c, double* ms, uint32_t* launches, uint32_t n) {
+    if (!c) return fail(RM_ERR_INVALID, "rm_stage_times needs a context.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    int r = collect_times(c);
+    if (r != RM_OK) return r;
+    for (uint32_t s = 0; s < n && s < RM_STAGE_COUNT; ++s) {
+        if (ms) ms[s] = c->timer.ms[s];
+        if (launches) launches[s] = c->timer.launches[s];
+    }
+    return RM_OK;
+}
+
+int rm_reset_stage_times(rm_context* c) {
+    if (!c) return fail(RM_ERR_INVALID, "rm_reset_stage_times needs a context.");
+    int r = collect_times(c);
+    if (r != RM_OK) return r;
+    for (int s = 0; s < RM_STAGE_COUNT; ++s) {
+        c->timer.ms[s] = 0.0;
+        c->timer.launches[s] = 0;
+    }
+    return RM_OK;
+}
+
+int rm_device_info(rm_context* c, char* name, size_t name_len, int* compute_units) {
+    if (!c) return fail(RM_ERR_INVALID, "rm_device_info needs a context.");
+    hipDeviceProp_t prop;
+    RM_HIP(hipGetDeviceProperties(&prop, c->device), "reading device properties");
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    return RM_OK;
+}
+
+}  // extern "C"

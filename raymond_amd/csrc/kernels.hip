@@ -1,0 +1,808 @@
+// Raymond's frame kernels for gfx950 (one wave64 per 8×8 pixel tile):
+//   k_raycast        closest-hit / any-hit hit records for a batch of rays
+//   k_main           samples_per_frame paths per pixel: NEE+MIS path tracing
+//                    with volumes, dispersion, Russian roulette; guides
+//   k_light_trace    photons from the lights through mirrors and glass,
+//                    splatted to the camera and deposited in the hash grid
+//   k_resolve        fixed-point splats into the accumulation
+//   k_photon_gather  camera paths through delta surfaces gathering photons
+//   k_prepare / k_atrous / k_compose   the à-trous denoiser
+// Each kernel restates the reference entry point cited beside it
+// (src/shaders.mjs). Traversal stacks live in LDS with a global spill.
+#include "kernels.hpp"
+
+namespace rm {
+
+// ------------------------------------------------------------------ thread mapping
+// A 256-thread block covers a 16×16 pixel tile as four 8×8 wave tiles, so a
+// wave's primary rays are spatially coherent.
+RM_D void tile_pixel(uint32_t& x, uint32_t& y) {
+    uint32_t t = threadIdx.x, w = t >> 6, l = t & 63u;
+    x = blockIdx.x * 16u + (w & 1u) * 8u + (l & 7u);
+    y = blockIdx.y * 16u + (w >> 1) * 8u + (l >> 3);
+}
+
+template <int LDS_DEPTH>
+struct BlockStack {
+    using S = Stack<LDS_DEPTH, BLOCK_THREADS>;
+    RM_D static S make(uint32_t* lds, uint32_t* spill_base, uint32_t slot, uint32_t slots) {
+        S s;
+        s.lds = lds + threadIdx.x;
+        s.spill = spill_base + slot;
+        s.stride = slots;
+        return s;
+    }
+};
+
+// ------------------------------------------------------------------ raycast
+// trace() over a ray batch: rays are 8 floats (o, t_max, d, pad); hits are
+// (t, prim, u, v) as 4 words.
+template <bool ANY>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_raycast(DevScene S, const float4* __restrict__ rays, uint32_t n,
+                                                           uint4* __restrict__ hits, uint32_t* spill, uint32_t slots,
+                                                           uint2* counters) {
+    __shared__ uint32_t lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
+    uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    if (i >= n) return;
+    auto stack = BlockStack<TRACE_LDS_DEPTH>::make(lds, spill, i, slots);
+    float4 a = rays[(size_t)i * 2], b = rays[(size_t)i * 2 + 1];
+    Hit h;
+    if (counters) {
+        Count c;
+        h = trace<ANY>(S, xyz(a), xyz(b), a.w, stack, c);
+        counters[i] = make_uint2(c.n, c.p);
+    } else {
+        NoCount c;
+        h = trace<ANY>(S, xyz(a), xyz(b), a.w, stack, c);
+    }
+    hits[i] = make_uint4(__float_as_uint(h.t), h.prim, __float_as_uint(h.u), __float_as_uint(h.v));
+}
+
+// ------------------------------------------------------------------ radiance
+struct Guides { V3 albedo; float4 normal; };
+
+// One camera path through pixel position px (src/shaders.mjs:1179-1359).
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, class STACK>
+RM_D V3 radiance(const Ctx& C, Rng& g, STACK& stack, V2 px, Guides& gd, float spectrum_u) {
+    const Params& P = *C.P;
+    const DevScene& S = C.S;
+    NoCount nc;
+    gd.albedo = {1.0f, 1.0f, 1.0f};
+    gd.normal = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool guided = false;
+    float travelled = 0.0f;
+    Ray ray = camera_ray(P, g, px, true);
+    V3 o = ray.o, d = ray.d;
+    V3 throughput = {1.0f, 1.0f, 1.0f};
+    V3 result = {0.0f, 0.0f, 0.0f};
+    float last_pdf = 0.0f;
+    bool last_delta = true;
+    V3 last_p = o;
+    const uint32_t choices = light_choices<ENV>(C);
+    float wavelength = 0.0f;
+    bool caustic = false, diffuse_seen = false;
+
+    for (uint32_t depth = 0u; depth <= P.max_bounces; depth++) {
+        g.dim = (depth + 1u) << 16u;
+        Hit hit = trace<false>(S, o, d, INF, stack, nc);
+
+        if (VOLUMES && P.volume_count > 0u) {
+            Scatter sc = scatter_in_volumes(C, g, o, d, hit.t);
+            if (sc.volume != NONE) {
+                Mat vm = load_mat(S, __float_as_uint(S.prims[(size_t)sc.volume * 4].y));
+                V3 p = o + d * sc.t;
+                travelled += sc.t;
+                if (!guided) {
+                    guided = true;
+                    gd.albedo = throughput * vm.color;
+                    gd.normal = make_float4(-d.x, -d.y, -d.z, travelled);
+                }
+                if (depth == P.max_bounces) break;
+                throughput = throughput * vm.color;
+                if (choices > 0u) {
+                    LightSample ls = sample_light<ENV, TEXTURES>(C, g, p);
+                    if (ls.pdf > 0.0f) {
+                        float phase = henyey_greenstein(dot(d, ls.wi), vm.anisotropy);
+                        float reach = ls.dist * (float)(1.0 - 1e-3);
+                        if (trace<true>(S, p, ls.wi, reach, stack, nc).prim == NONE)
+                            result = result + throughput * ls.radiance *
+                                                  (phase * transmittance(C, p, ls.wi, reach) *
+                                                   power_heuristic(ls.pdf, phase) / ls.pdf);
+                    }
+                }
+                V3 wi = sample_henyey_greenstein(g, d, vm.anisotropy);
+                last_pdf = henyey_greenstein(dot(d, wi), vm.anisotropy);
+                last_delta = false;
+                last_p = p;
+                o = p;
+                d = wi;
+                caustic = false;
+                diffuse_seen = true;
+                if (!roulette(g, throughput, depth)) break;
+                continue;
+            }
+        }
+
+        if (hit.prim == NONE) {
+            if (!guided) gd.albedo = throughput;
+            float sky_weight = 1.0f;
+            if (has_env<ENV>(C) && !last_delta)
+                sky_weight = power_heuristic(last_pdf, env_pdf(C, d) / (float)choices);
+            result = result + throughput * sky<ENV>(C, d) * sky_weight;
+            if (in_sun(P, d) && !(caustic && last_delta)) {
+                float w = 1.0f;
+                if (!last_delta) w = power_heuristic(last_pdf, 1.0f / ((float)choices * 2.0f * PI * P.sun_omc));
+                result = result + throughput * pv3(P.sun_radiance) * w;
+            }
+            break;
+        }
+
+        Surface s = surface<TEXTURES>(S, o, d, hit);
+        Mat m = load_mat(S, s.material);
+        travelled += hit.t;
+
+        if (!guided && (m.kind == LIGHT || !is_delta(m) || depth == P.max_bounces)) {
+            guided = true;
+            gd.albedo = throughput * (m.kind == LIGHT ? v3(1.0f, 1.0f, 1.0f) : albedo<TEXTURES>(S, m, s.p, s.uv));
+            gd.normal = make_float4(s.n.x, s.n.y, s.n.z, travelled);
+        }
+
+        bool glows = m.emission.x > 0.0f || m.emission.y > 0.0f || m.emission.z > 0.0f;
+        if (s.front && glows && !(caustic && last_delta && s.light != NONE)) {
+            float w = 1.0f;
+            if (!last_delta && s.light != NONE) w = power_heuristic(last_pdf, light_pdf<ENV>(C, hit.prim, last_p, d, hit.t));
+            result = result + throughput * emitted<TEXTURES>(S, m, s.uv) * w;
+        }
+
+        if (m.kind == LIGHT) break;
+        bool delta = is_delta(m);
+        if (!delta) {
+            caustic = P.caustics != 0u && !diffuse_seen;
+            diffuse_seen = true;
+        }
+        if (depth == P.max_bounces) break;
+
+        V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
+        V3 wo = -d;
+
+        if (!delta && choices > 0u) {
+            LightSample ls = sample_light<ENV, TEXTURES>(C, g, s.p);
+            if (ls.pdf > 0.0f && dot(ls.wi, s.ng) > 0.0f) {
+                Eval e = eval_bsdf(m, base, s.n, wo, ls.wi);
+                V3 origin = offset(s, ls.wi);
+                float reach = (ls.dist - dot(origin - s.p, ls.wi)) * (float)(1.0 - 1e-3);
+                if (e.pdf > 0.0f && trace<true>(S, origin, ls.wi, reach, stack, nc).prim == NONE) {
+                    float tr = VOLUMES ? transmittance(C, origin, ls.wi, reach) : 1.0f;
+                    result = result + throughput * e.f * ls.radiance * (tr * power_heuristic(ls.pdf, e.pdf) / ls.pdf);
+                }
+            }
+        }
+
+        Mat dm = dispersed<DISPERSION>(m, wavelength, throughput, spectrum_u);
+        BsdfSample bs = sample_bsdf(g, dm, base, s, wo);
+        if (!bs.ok) break;
+        throughput = throughput * bs.weight;
+        last_pdf = bs.pdf;
+        last_delta = bs.delta;
+        last_p = s.p;
+        o = offset(s, bs.wi);
+        d = bs.wi;
+        if (!roulette(g, throughput, depth)) break;
+    }
+    return result;
+}
+
+// The albedo, normal or depth seen through a pixel centre (src/shaders.mjs:1377-1407).
+template <bool TEXTURES, bool ENV, class STACK>
+RM_D V3 inspect(const Ctx& C, Rng& g, STACK& stack, V2 px) {
+    const Params& P = *C.P;
+    NoCount nc;
+    Ray ray = camera_ray(P, g, px, false);
+    Hit hit = trace<false>(C.S, ray.o, ray.d, INF, stack, nc);
+    if (hit.prim == NONE) return P.view == VIEW_ALBEDO ? sky<ENV>(C, ray.d) : v3(0.0f, 0.0f, 0.0f);
+    Surface s = surface<TEXTURES>(C.S, ray.o, ray.d, hit);
+    Mat m = load_mat(C.S, s.material);
+    if (P.view == VIEW_ALBEDO) {
+        if (m.kind == LIGHT) {
+            V3 e = emitted<TEXTURES>(C.S, m, s.uv);
+            return e / fmaxf(fmaxf(e.x, e.y), fmaxf(e.z, 1.0f));
+        }
+        return albedo<TEXTURES>(C.S, m, s.p, s.uv);
+    }
+    if (P.view == VIEW_NORMAL) return s.n;
+    return s3(hit.t);
+}
+
+// ------------------------------------------------------------------ main
+// src/shaders.mjs:1838-1895. Each pixel's sum over the frame's samples
+// replaces (first frame) or adds to its accumulation and guides.
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_main(DevScene S, Params P, float4* __restrict__ accum,
+                                                        float4* __restrict__ guide_albedo,
+                                                        float4* __restrict__ guide_normal, uint32_t* spill,
+                                                        uint32_t slots) {
+    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= P.width || y >= P.height) return;
+    const Ctx C{S, &P};
+    const uint32_t index = y * P.width + x;
+    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, index, slots);
+
+    V3 sum = {0, 0, 0};
+    float sum_w = 0.0f;
+    V3 sum_albedo = {0, 0, 0};
+    float4 sum_normal = make_float4(0, 0, 0, 0);
+    for (uint32_t k = 0; k < P.samples_per_frame; k++) {
+        Rng g;
+        g.state = pcg(index ^ pcg(P.sample_offset + k));
+        g.qmc = true;
+        g.seed = pcg(index + 0x9e3779b9u);
+        g.index = P.sample_offset + k;
+        g.dim = 0u;
+        float spectrum_u = (float)((pcg(index * 7919u + 1u) + (P.sample_offset + k) * 2654435769u) >> 8u) *
+                           (1.0f / 16777216.0f);
+        Guides gd;
+        gd.albedo = {0, 0, 0};
+        gd.normal = make_float4(0, 0, 0, 0);
+        V3 c;
+        if (P.view == VIEW_COLOR) {
+            V2 j = g.rand2();
+            c = radiance<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, stack, {(float)x + j.x, (float)y + j.y}, gd,
+                                                            spectrum_u);
+            float peak = fmaxf(c.x, fmaxf(c.y, c.z));
+            if (P.firefly_clamp > 0.0f && peak > P.firefly_clamp) c = c * (P.firefly_clamp / peak);
+        } else {
+            c = inspect<TEXTURES, ENV>(C, g, stack, {(float)x + 0.5f, (float)y + 0.5f});
+        }
+        if (finite3(c) && finite3(gd.albedo) && finite3(v3(gd.normal.x, gd.normal.y, gd.normal.z)) &&
+            finite3(s3(gd.normal.w))) {
+            float l = luminance(c);
+            sum = sum + c;
+            sum_w += l * l;
+            sum_albedo = sum_albedo + gd.albedo;
+            sum_normal = make_float4(sum_normal.x + gd.normal.x, sum_normal.y + gd.normal.y,
+                                     sum_normal.z + gd.normal.z, sum_normal.w + gd.normal.w);
+        }
+    }
+    if (P.sample_offset == 0u) {
+        accum[index] = make_float4(sum.x, sum.y, sum.z, sum_w);
+        guide_albedo[index] = make_float4(sum_albedo.x, sum_albedo.y, sum_albedo.z, 0.0f);
+        guide_normal[index] = sum_normal;
+    } else {
+        float4 a = accum[index], ga = guide_albedo[index], gn = guide_normal[index];
+        accum[index] = make_float4(a.x + sum.x, a.y + sum.y, a.z + sum.z, a.w + sum_w);
+        guide_albedo[index] = make_float4(ga.x + sum_albedo.x, ga.y + sum_albedo.y, ga.z + sum_albedo.z, ga.w + 0.0f);
+        guide_normal[index] = make_float4(gn.x + sum_normal.x, gn.y + sum_normal.y, gn.z + sum_normal.z,
+                                          gn.w + sum_normal.w);
+    }
+}
+
+// ------------------------------------------------------------------ light tracing
+RM_D uint32_t photon_count(const Params& P) {
+    return min(P.width * P.height * P.samples_per_frame, PHOTON_CAPACITY);
+}
+RM_D uint32_t photon_cell_hash(int32_t cx, int32_t cy, int32_t cz) {
+    return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & (PHOTON_TABLE - 1u);
+}
+
+// Adds a photon's light to the pixel that sees surface s (src/shaders.mjs:1444-1491).
+template <class STACK>
+RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surface& s, const Mat& m, V3 base, V3 d,
+                V3 beta) {
+    const Params& P = *C.P;
+    NoCount nc;
+    V3 to_camera = pv3(P.cam_pos) - s.p;
+    float dist = length(to_camera);
+    V3 wo = to_camera / dist;
+    float cos_film = -dot(wo, pv3(P.cam_forward));
+    if (cos_film <= 1e-6f || dot(wo, s.ng) <= 0.0f) return;
+    float x = -dot(wo, pv3(P.cam_right)) / (cos_film * P.tan_half_fov * P.aspect);
+    float y = -dot(wo, pv3(P.cam_up)) / (cos_film * P.tan_half_fov);
+    float px = (x + 1.0f) * 0.5f * (float)P.width;
+    float py = (1.0f - y) * 0.5f * (float)P.height;
+    if (px < 0.0f || py < 0.0f || px >= (float)P.width || py >= (float)P.height) return;
+    V3 wi = -d;
+    Eval e = eval_bsdf(m, base, s.n, wo, wi);
+    float cos_in = dot(s.n, wi);
+    if (e.pdf <= 0.0f || cos_in <= 0.0f) return;
+    V3 origin = offset(s, wo);
+    if (trace<true>(C.S, origin, wo, dist - dot(origin - s.p, wo), stack, nc).prim != NONE) return;
+    float pixel_area = (4.0f * P.tan_half_fov * P.tan_half_fov * P.aspect) / (float)(P.width * P.height);
+    V3 value = beta * (e.f / cos_in) * dot(wo, s.ng) / (dist * dist * pixel_area * cos_film * cos_film * cos_film);
+    if (!finite3(value)) return;
+    uint32_t index = (to_u32(py) * P.width + to_u32(px)) * 4u;
+    float r0 = g.rand(), r1 = g.rand(), r2 = g.rand();
+    V3 fx = vmin(value * SPLAT_SCALE + v3(r0, r1, r2), s3(4e9f));
+    atomicAdd(&splats[index], to_u32(fx.x));
+    atomicAdd(&splats[index + 1u], to_u32(fx.y));
+    atomicAdd(&splats[index + 2u], to_u32(fx.z));
+}
+
+// One photon per thread (src/shaders.mjs:1630-1814). Deposits happen after
+// the bounce loop with one counter atomic per wave (ballot + prefix count):
+// photon slots are claimed in a different order than one atomic per lane
+// would give, which the reference leaves unspecified too.
+template <bool TEXTURES, bool DISPERSION>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Params P, const float4* __restrict__ E,
+                                                               uint32_t* splats, uint32_t* heads,
+                                                               float4* __restrict__ photons, uint32_t* spill,
+                                                               uint32_t slots) {
+    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    const Ctx C{S, &P};
+    const uint32_t id = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    const uint32_t photons_n = photon_count(P);
+    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, id, slots);
+    bool pending = false;
+    V3 dep_p = {0, 0, 0}, dep_d = {0, 0, 0}, dep_beta = {0, 0, 0};
+
+    if (id < photons_n) {
+        Rng g;
+        g.qmc = false;
+        g.state = pcg(id ^ pcg(P.sample_offset * 2654435769u + 977u));
+        const float4 header = E[0];
+        const uint32_t sources = __float_as_uint(header.x), targets = __float_as_uint(header.y);
+        float xi = g.rand();
+        uint32_t k = 0u;
+        while (!(k + 1u >= sources || E[1u + k].x > xi)) k++;
+        const float4 entry = E[1u + k];
+        const float chance = entry.z;
+        const uint32_t prim_index = __float_as_uint(entry.y);
+        const uint32_t first_target = 1u + sources;
+        bool alive = chance > 0.0f;
+        V3 o = {0, 0, 0}, d = {0, 0, 0}, beta = {0, 0, 0};
+        if (alive && prim_index == NONE) {
+            d = -sample_cone(g, pv3(P.sun_dir), P.sun_omc);
+            float pick = g.rand() * header.w / PI;
+            uint32_t j = 0u;
+            for (;;) {
+                float r = E[first_target + j].w;
+                if (j + 1u >= targets || pick < r * r) break;
+                pick -= r * r;
+                j++;
+            }
+            float4 circle = E[first_target + j];
+            Frame f = frame(d);
+            V2 disk = concentric_disk(g);
+            disk = {disk.x * circle.w, disk.y * circle.w};
+            o = xyz(circle) - d * header.z + f.c0 * disk.x + f.c1 * disk.y;
+            float through = 1.0f;
+            for (uint32_t i = 0u; i < targets; i++) {
+                float4 c = E[first_target + i];
+                V3 q = o - xyz(c);
+                V3 across = q - d * dot(q, d);
+                through += (i != j && dot(across, across) < c.w * c.w) ? 1.0f : 0.0f;
+            }
+            V3 irradiance = pv3(P.sun_radiance) * (TWO_PI * P.sun_omc);
+            beta = irradiance * header.w / (through * chance * (float)photons_n);
+        } else if (alive) {
+            const float4* pr = S.prims + (size_t)prim_index * 4;
+            float4 hd = pr[0], p0 = pr[1], p1 = pr[2], p2 = pr[3];
+            Mat m = load_mat(S, __float_as_uint(hd.y));
+            V3 n;
+            float area;
+            V2 uv;
+            if (__float_as_uint(hd.x) == QUAD) {
+                float a = g.rand();
+                float b = g.rand();
+                o = xyz(p0) + xyz(p1) * a + xyz(p2) * b;
+                V3 c = cross(xyz(p1), xyz(p2));
+                area = length(c);
+                n = c / area;
+                uv = {a, 1.0f - b};
+            } else {
+                float z = 1.0f - 2.0f * g.rand();
+                float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                float phi = TWO_PI * g.rand();
+                n = {r * cosf(phi), z, r * sinf(phi)};
+                o = xyz(p0) + n * p0.w;
+                area = FOUR_PI * p0.w * p0.w;
+                uv = sphere_uv(n, p1, p2);
+            }
+            float cones = 0.0f;
+            bool inside = false;
+            for (uint32_t i = 0u; i < targets; i++) {
+                float4 aim = E[first_target + i];
+                V3 to = xyz(aim) - o;
+                float d2 = dot(to, to);
+                inside = inside || d2 <= aim.w * aim.w;
+                cones += sphere_omc(aim.w, fmaxf(d2, aim.w * aim.w));
+            }
+            if (!inside) {
+                float pick = g.rand() * cones;
+                uint32_t j = 0u;
+                for (;;) {
+                    float4 aim = E[first_target + j];
+                    V3 to = xyz(aim) - o;
+                    float omc = sphere_omc(aim.w, dot(to, to));
+                    if (j + 1u >= targets || pick < omc) {
+                        d = sample_cone(g, to / length(to), omc);
+                        break;
+                    }
+                    pick -= omc;
+                    j++;
+                }
+                float within = 1.0f;
+                for (uint32_t i = 0u; i < targets; i++) {
+                    float4 aim = E[first_target + i];
+                    V3 to = xyz(aim) - o;
+                    float omc = sphere_omc(aim.w, dot(to, to));
+                    within += (i != j && 1.0f - dot(d, to / length(to)) <= omc) ? 1.0f : 0.0f;
+                }
+                float cos_out = dot(d, n);
+                if (cos_out <= 0.0f) alive = false;
+                else beta = emitted<TEXTURES>(S, m, uv) * (cos_out * area * 2.0f * PI * cones) /
+                            (within * chance * (float)photons_n);
+            } else {
+                d = cosine_hemisphere(g, n);
+                beta = emitted<TEXTURES>(S, m, uv) * (PI * area) / (chance * (float)photons_n);
+            }
+            o = o + n * (1e-4f * fmaxf(1.0f, fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)))));
+        }
+
+        float wavelength = 0.0f;
+        uint32_t bounces = 0u;
+        V3 carried = {1.0f, 1.0f, 1.0f};
+        NoCount nc;
+        for (uint32_t depth = 0u; alive && depth < P.max_bounces; depth++) {
+            Hit hit = trace<false>(S, o, d, INF, stack, nc);
+            if (hit.prim == NONE) break;
+            Surface s = surface<TEXTURES>(S, o, d, hit);
+            Mat m = load_mat(S, s.material);
+            if (m.kind == LIGHT) break;
+            V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
+            if (!is_delta(m)) {
+                if (bounces > 0u) {
+                    splat(C, g, stack, splats, s, m, base, d, beta * carried);
+                    pending = true;
+                    dep_p = s.p;
+                    dep_d = d;
+                    dep_beta = beta * carried;
+                }
+                break;
+            }
+            float u = g.rand();
+            Mat dm = dispersed<DISPERSION>(m, wavelength, carried, u);
+            BsdfSample bs = sample_bsdf(g, dm, base, s, -d);
+            if (!bs.ok) break;
+            carried = carried * bs.weight;
+            bounces++;
+            o = offset(s, bs.wi);
+            d = bs.wi;
+            if (!roulette(g, carried, depth)) break;
+        }
+    }
+
+    // deposit() (src/shaders.mjs:1518-1528), one counter atomic per wave.
+    uint64_t mask = __ballot(pending);
+    if (mask == 0) return;
+    uint32_t lane = __lane_id();
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&heads[PHOTON_TABLE], (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    if (!pending) return;
+    uint32_t i = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (i >= PHOTON_CAPACITY) return;
+    V3 q = dep_p / P.photon_cell;
+    uint32_t h = photon_cell_hash(to_i32(floorf(q.x)), to_i32(floorf(q.y)), to_i32(floorf(q.z)));
+    uint32_t next = atomicExch(&heads[h], i + 1u);
+    photons[(size_t)i * 3] = make_float4(dep_p.x, dep_p.y, dep_p.z, __uint_as_float(next));
+    photons[(size_t)i * 3 + 1] = make_float4(dep_beta.x, dep_beta.y, dep_beta.z, 0.0f);
+    photons[(size_t)i * 3 + 2] = make_float4(-dep_d.x, -dep_d.y, -dep_d.z, 0.0f);
+}
+
+// src/shaders.mjs:1819-1831
+__global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* splats, float4* accum) {
+    uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    if (i >= P.width * P.height) return;
+    uint4 s = splats[i];
+    splats[i] = make_uint4(0u, 0u, 0u, s.w);
+    V3 light = v3((float)s.x, (float)s.y, (float)s.z) / SPLAT_SCALE;
+    float l = luminance(light);
+    float spp = (float)P.samples_per_frame;
+    float4 a = accum[i];
+    accum[i] = make_float4(a.x + light.x * spp, a.y + light.y * spp, a.z + light.z * spp, a.w + l * l * spp);
+}
+
+// The photons within the radius of a surface, times the BSDF, over the
+// disc's area (src/shaders.mjs:1534-1570).
+RM_D V3 gather(const Ctx& C, const uint32_t* heads, const float4* photons, const Surface& s, const Mat& m, V3 base,
+               V3 wo) {
+    const Params& P = *C.P;
+    float r = P.photon_radius, r2 = r * r;
+    V3 here = s.p / P.photon_cell;
+    int32_t cx = to_i32(floorf(here.x)), cy = to_i32(floorf(here.y)), cz = to_i32(floorf(here.z));
+    int32_t sx = (here.x - floorf(here.x)) >= 0.5f ? 1 : -1;
+    int32_t sy = (here.y - floorf(here.y)) >= 0.5f ? 1 : -1;
+    int32_t sz = (here.z - floorf(here.z)) >= 0.5f ? 1 : -1;
+    V3 sum = {0, 0, 0};
+    uint32_t visited = 0u;
+    for (uint32_t k = 0u; k < 8u; k++) {
+        uint32_t index = heads[photon_cell_hash(cx + (int32_t)(k & 1u) * sx, cy + (int32_t)((k >> 1u) & 1u) * sy,
+                                                cz + (int32_t)((k >> 2u) & 1u) * sz)];
+        while (index != 0u && visited < 4096u) {
+            visited++;
+            float4 at = photons[(size_t)(index - 1u) * 3];
+            V3 off = xyz(at) - s.p;
+            if (dot(off, off) < r2 && fabsf(dot(off, s.ng)) < 0.25f * r) {
+                V3 wi = xyz(photons[(size_t)(index - 1u) * 3 + 2]);
+                float cos_in = dot(s.n, wi);
+                if (cos_in > 0.0f && dot(s.ng, wi) > 0.0f) {
+                    Eval e = eval_bsdf(m, base, s.n, wo, wi);
+                    sum = sum + xyz(photons[(size_t)(index - 1u) * 3 + 1]) * (e.f / cos_in);
+                }
+            }
+            index = __float_as_uint(at.w);
+        }
+    }
+    return sum / (PI * r2);
+}
+
+// src/shaders.mjs:1577-1626
+template <bool TEXTURES>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Params P, const uint32_t* heads,
+                                                                 const float4* __restrict__ photons, float4* accum,
+                                                                 uint32_t* spill, uint32_t slots) {
+    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= P.width || y >= P.height) return;
+    const Ctx C{S, &P};
+    const uint32_t index = y * P.width + x;
+    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, index, slots);
+    NoCount nc;
+    Rng g;
+    g.qmc = false;
+    g.state = pcg(index ^ pcg(P.sample_offset * 2654435769u + 4099u));
+    float jx = g.rand();
+    float jy = g.rand();
+    Ray ray = camera_ray(P, g, {(float)x + jx, (float)y + jy}, true);
+    V3 o = ray.o, d = ray.d;
+    V3 throughput = {1, 1, 1};
+    for (uint32_t depth = 0u; depth < P.max_bounces; depth++) {
+        Hit hit = trace<false>(S, o, d, INF, stack, nc);
+        if (hit.prim == NONE) return;
+        Surface s = surface<TEXTURES>(S, o, d, hit);
+        Mat m = load_mat(S, s.material);
+        if (m.kind == LIGHT) return;
+        V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
+        if (!is_delta(m)) {
+            if (depth > 0u) {
+                V3 light = throughput * gather(C, heads, photons, s, m, base, -d);
+                if (finite3(light)) {
+                    float l = luminance(light);
+                    float spp = (float)P.samples_per_frame;
+                    float4 a = accum[index];
+                    accum[index] = make_float4(a.x + light.x * spp, a.y + light.y * spp, a.z + light.z * spp,
+                                               a.w + l * l * spp);
+                }
+            }
+            return;
+        }
+        BsdfSample bs = sample_bsdf(g, m, base, s, -d);
+        if (!bs.ok) return;
+        throughput = throughput * bs.weight;
+        o = offset(s, bs.wi);
+        d = bs.wi;
+    }
+}
+
+// ------------------------------------------------------------------ denoiser
+// src/shaders.mjs:1916-2098
+constexpr float KERNEL0 = 0.375f, KERNEL1 = 0.25f, KERNEL2 = 0.0625f;
+RM_D float kernel_w(int k) { return k == 0 ? KERNEL0 : (k == 1 ? KERNEL1 : KERNEL2); }
+
+struct DenoiseView {
+    const Params* P;
+    RM_D float samples() const { return (float)max(P->total_samples, 1u); }
+};
+RM_D V3 albedo_at(const float4* galb, uint32_t i, float n) {
+    float4 g = galb[i];
+    return vmax(v3(g.x, g.y, g.z) / n, s3(0.01f));
+}
+RM_D float4 normal_at(const float4* gnorm, uint32_t i, float n) {
+    float4 g = gnorm[i];
+    V3 v = {g.x, g.y, g.z};
+    V3 nn = dot(v, v) > 0.0f ? normalize(v) : v3(0, 0, 0);
+    return make_float4(nn.x, nn.y, nn.z, g.w / n);
+}
+RM_D float similarity(float4 p, float4 q, float step) {
+    if (p.w <= 0.0f || q.w <= 0.0f) return (p.w <= 0.0f && q.w <= 0.0f) ? 1.0f : 0.0f;
+    if (p.x * p.x + p.y * p.y + p.z * p.z == 0.0f) return 0.0f;
+    float wn = powf(fmaxf(dot(v3(p.x, p.y, p.z), v3(q.x, q.y, q.z)), 0.0f), 64.0f);
+    float wz = expf(-fabsf(p.w - q.w) / (p.w * 0.02f * step + 1e-6f));
+    return wn * wz;
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float4* __restrict__ accum,
+                                                           const float4* __restrict__ galb,
+                                                           const float4* __restrict__ gnorm, float4* light_out) {
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= P.width || y >= P.height) return;
+    const uint32_t i = y * P.width + x;
+    const float n = (float)max(P.total_samples, 1u);
+    float4 ac = accum[i];
+    V3 light = (v3(ac.x, ac.y, ac.z) / n) / albedo_at(galb, i, n);
+    float4 gp = normal_at(gnorm, i, n);
+    float a = luminance(albedo_at(galb, i, n));
+    float variance;
+    if (n >= 4.0f) {
+        float mean = luminance(v3(ac.x, ac.y, ac.z)) / n;
+        float spread = fmaxf(ac.w / n - mean * mean, 0.0f) * n / (n - 1.0f);
+        variance = spread / n / (a * a);
+    } else {
+        float sum = 0.0f, sum2 = 0.0f, weight = 0.0f;
+        for (int dy = -2; dy <= 2; dy++)
+            for (int dx = -2; dx <= 2; dx++) {
+                int xx = (int)x + dx, yy = (int)y + dy;
+                if (xx < 0 || yy < 0 || xx >= (int)P.width || yy >= (int)P.height) continue;
+                uint32_t j = (uint32_t)yy * P.width + (uint32_t)xx;
+                float w = similarity(gp, normal_at(gnorm, j, n), 1.0f);
+                float4 aj = accum[j];
+                float l = luminance((v3(aj.x, aj.y, aj.z) / n) / albedo_at(galb, j, n));
+                sum += w * l;
+                sum2 += w * l * l;
+                weight += w;
+            }
+        float mean = sum / fmaxf(weight, 1e-6f);
+        variance = fmaxf(sum2 / fmaxf(weight, 1e-6f) - mean * mean, 0.0f) + 0.01f * mean * mean;
+    }
+    light_out[i] = make_float4(light.x, light.y, light.z, variance);
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const float4* __restrict__ gnorm,
+                                                          const float4* __restrict__ light_in,
+                                                          float4* __restrict__ light_out) {
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= P.width || y >= P.height) return;
+    const uint32_t i = y * P.width + x;
+    const float n = (float)max(P.total_samples, 1u);
+    const float4 centre = light_in[i];
+    const float4 gp = normal_at(gnorm, i, n);
+    const float lp = luminance(v3(centre.x, centre.y, centre.z));
+    float variance = 0.0f, variance_weight = 0.0f;
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            int xx = (int)x + dx, yy = (int)y + dy;
+            if (xx >= 0 && yy >= 0 && xx < (int)P.width && yy < (int)P.height) {
+                float w = kernel_w(abs(dx)) * kernel_w(abs(dy));
+                variance += w * light_in[(uint32_t)yy * P.width + (uint32_t)xx].w;
+                variance_weight += w;
+            }
+        }
+    const float sigma = 4.0f * sqrtf(variance / variance_weight) + 1e-4f;
+    V3 sum = {0, 0, 0};
+    float sum_variance = 0.0f, weight = 0.0f;
+    for (int dy = -2; dy <= 2; dy++)
+        for (int dx = -2; dx <= 2; dx++) {
+            int xx = (int)x + dx * step, yy = (int)y + dy * step;
+            if (xx < 0 || yy < 0 || xx >= (int)P.width || yy >= (int)P.height) continue;
+            uint32_t j = (uint32_t)yy * P.width + (uint32_t)xx;
+            float4 q = light_in[j];
+            float w = kernel_w(abs(dx)) * kernel_w(abs(dy)) * similarity(gp, normal_at(gnorm, j, n), (float)step) *
+                      expf(-fabsf(lp - luminance(v3(q.x, q.y, q.z))) / sigma);
+            sum = sum + w * v3(q.x, q.y, q.z);
+            sum_variance += w * w * q.w;
+            weight += w;
+        }
+    if (weight > 0.0f) {
+        V3 r = sum / weight;
+        light_out[i] = make_float4(r.x, r.y, r.z, sum_variance / (weight * weight));
+    } else {
+        light_out[i] = centre;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_compose(Params P, const float4* __restrict__ galb,
+                                                           const float4* __restrict__ light_in, float4* out) {
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= P.width || y >= P.height) return;
+    const uint32_t i = y * P.width + x;
+    const float n = (float)max(P.total_samples, 1u);
+    float4 l = light_in[i];
+    V3 r = v3(l.x, l.y, l.z) * albedo_at(galb, i, n);
+    out[i] = make_float4(r.x, r.y, r.z, 0.0f);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t value, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = value;
+}
+
+}  // namespace rm
+
+// ------------------------------------------------------------------ launchers
+namespace rm {
+
+template <bool V, bool T, bool E, bool D>
+static void launch_main_v(const LaunchMain& L, hipStream_t stream) {
+    dim3 grid((L.P.width + 15) / 16, (L.P.height + 15) / 16);
+    hipLaunchKernelGGL((k_main<V, T, E, D>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum, L.galb,
+                       L.gnorm, L.spill, L.slots);
+}
+
+void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream) {
+    int key = (f.volumes ? 8 : 0) | (f.textures ? 4 : 0) | (f.environment ? 2 : 0) | (f.dispersion ? 1 : 0);
+    switch (key) {
+    case 0: launch_main_v<false, false, false, false>(L, stream); break;
+    case 1: launch_main_v<false, false, false, true>(L, stream); break;
+    case 2: launch_main_v<false, false, true, false>(L, stream); break;
+    case 3: launch_main_v<false, false, true, true>(L, stream); break;
+    case 8: launch_main_v<true, false, false, false>(L, stream); break;
+    case 10: launch_main_v<true, false, true, false>(L, stream); break;
+    default: launch_main_v<true, true, true, true>(L, stream); break;
+    }
+}
+
+void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, uint32_t* spill,
+                    uint32_t slots, uint2* counters, hipStream_t stream) {
+    dim3 grid((n + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    if (any_hit)
+        hipLaunchKernelGGL(k_raycast<true>, grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill, slots, counters);
+    else
+        hipLaunchKernelGGL(k_raycast<false>, grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill, slots, counters);
+}
+
+void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
+                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
+                        hipStream_t stream) {
+    dim3 grid((count + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    if (f.textures || f.dispersion) {
+        if (f.textures)
+            hipLaunchKernelGGL((k_light_trace<true, true>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
+                               heads, photons, spill, slots);
+        else
+            hipLaunchKernelGGL((k_light_trace<false, true>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
+                               heads, photons, spill, slots);
+    } else {
+        hipLaunchKernelGGL((k_light_trace<false, false>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
+                           heads, photons, spill, slots);
+    }
+}
+
+void launch_resolve(const Params& P, uint4* splats, float4* accum, hipStream_t stream) {
+    uint32_t n = P.width * P.height;
+    hipLaunchKernelGGL(k_resolve, dim3((n + BLOCK_THREADS - 1) / BLOCK_THREADS), dim3(BLOCK_THREADS), 0, stream, P,
+                       splats, accum);
+}
+
+void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
+                          float4* accum, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream) {
+    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+    if (f.textures)
+        hipLaunchKernelGGL(k_photon_gather<true>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons, accum,
+                           spill, slots);
+    else
+        hipLaunchKernelGGL(k_photon_gather<false>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons, accum,
+                           spill, slots);
+}
+
+void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* gnorm, float4* out,
+                    hipStream_t stream) {
+    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+    hipLaunchKernelGGL(k_prepare, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb, gnorm, out);
+}
+
+void launch_atrous(const Params& P, int step, const float4* gnorm, const float4* in, float4* out, hipStream_t stream) {
+    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+    hipLaunchKernelGGL(k_atrous, grid, dim3(BLOCK_THREADS), 0, stream, P, step, gnorm, in, out);
+}
+
+void launch_compose(const Params& P, const float4* galb, const float4* in, float4* out, hipStream_t stream) {
+    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+    hipLaunchKernelGGL(k_compose, grid, dim3(BLOCK_THREADS), 0, stream, P, galb, in, out);
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream) {
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, stream, p, value, n);
+}
+
+}  // namespace rm

@@ -1,0 +1,43 @@
+// Launch interface between the C-ABI layer (api.hip) and the kernels
+// (kernels.hip), with the block shape and traversal-stack depths.
+#pragma once
+#include "shading.hpp"
+
+namespace rm {
+
+// 256 threads: four wave64s covering a 16×16 pixel tile.
+constexpr int BLOCK_THREADS = 256;
+// Traversal stack entries kept in LDS per thread (the rest of the
+// reference's 64 spill to global memory). 16 KiB of LDS per block each.
+constexpr int MAIN_LDS_DEPTH = 16;
+constexpr int TRACE_LDS_DEPTH = 16;
+constexpr uint32_t SPILL_ENTRIES = STACK_LIMIT - 16u;
+
+struct Features { bool volumes, textures, environment, dispersion; };
+
+struct LaunchMain {
+    DevScene S;
+    Params P;
+    float4* accum;
+    float4* galb;
+    float4* gnorm;
+    uint32_t* spill;
+    uint32_t slots;
+};
+
+void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
+void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, uint32_t* spill,
+                    uint32_t slots, uint2* counters, hipStream_t stream);
+void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
+                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
+                        hipStream_t stream);
+void launch_resolve(const Params& P, uint4* splats, float4* accum, hipStream_t stream);
+void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
+                          float4* accum, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream);
+void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* gnorm, float4* out,
+                    hipStream_t stream);
+void launch_atrous(const Params& P, int step, const float4* gnorm, const float4* in, float4* out, hipStream_t stream);
+void launch_compose(const Params& P, const float4* galb, const float4* in, float4* out, hipStream_t stream);
+void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
+
+}  // namespace rm
