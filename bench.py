@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Raymond on one MI355X: Msamples/s of progressive path tracing at 1080p.
+
+Workload: 200 seeded spheres on a 20 × 20 m checker floor (50% diffuse, 25%
+GGX metal, 15% glass, 10% lights), sky gradient and a 0.5° sun, rendered at
+1920 × 1080 with 1 sample per pixel per frame, 8 bounces, Russian roulette
+from bounce 3 and NEE+MIS. A step is one frame exactly as the reference's
+Renderer.render() issues it with its defaults (firefly clamp off): the
+path-tracing pass, light tracing (glass is present, so caustics are on), splat
+resolve, photon gather, and the à-trous denoiser (prepare, 5 passes, compose).
+
+Prints one JSON line; `roofline` is for the dominant kernel (the path-tracing
+pass), `cpu_baseline` times the CPU oracle's camera paths on the box's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "spheres200-1920x1080"
+SCENE_OPTS = {"scene": {"seed": 2, "count": 200, "width": 1920, "height": 1080}, "maxBounces": 8,
+              "samplesPerFrame": 1, "fireflyClamp": 0.0, "denoise": True, "caustics": True}
+TRAFFIC_SUMMARY = ROOT / "profiles" / "k_main_spheres200_1080p_pmc.md"
+
+
+def traffic_from_summary(path: Path):
+    """HBM bytes per launch of the trace kernel, from the committed PMC summary."""
+    if not path.exists():
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
+def algorithmic_bytes(p, params, stride: int) -> dict:
+    """Per-sample algorithmic bytes of the path-tracing pass, from the oracle's
+    segment counts on a pixel subsample of the same frame:
+      rays:     24 B in + 16 B hit out (closest) or 4 B (shadow)
+      traversal 32 B per node record tested, 64 B per primitive record tested
+      shading:  64 B primitive + 96 B material per surface hit, +64 B attributes for smooth triangles
+      per pixel per frame: 3 × 16 B read + 3 × 16 B written (accumulation and guides)."""
+    from oracle.pyoracle import Oracle
+    st = Oracle(p).path_stats(params, stride)
+    n = st["samples"]
+    per_sample = (st["closest"] * (24 + 16) + st["shadow"] * (24 + 4) + 32 * st["nodes"] + 64 * st["prims"]
+                  + st["surfaces"] * (64 + 96) + 64 * st["smooth"]) / n
+    return {"per_sample": per_sample, "rays_per_sample": (st["closest"] + st["shadow"]) / n, "stats": st}
+
+
+def cpu_baseline(p, params, budget_s: float = 12.0) -> dict:
+    """The oracle's main() (camera paths only) on a band of rows of the same frame."""
+    from oracle.pyoracle import Oracle
+    o = Oracle(p)
+    threads = int(os.environ.get("RAYMOND_CPU_THREADS", os.cpu_count() or 1))
+    w, h = p.width, p.height
+    shape = (h, w, 4)
+    acc, ga, gn = (np.zeros(shape, np.float32) for _ in range(3))
+    rows = 16
+    t0 = time.perf_counter()
+    o.main(params, acc, ga, gn, rows=(0, rows), threads=threads)
+    dt = time.perf_counter() - t0
+    rows2 = int(min(h, max(rows, rows * budget_s / max(dt, 1e-3))))
+    y0 = (h - rows2) // 2
+    t0 = time.perf_counter()
+    o.main(params, acc, ga, gn, rows=(y0, y0 + rows2), threads=threads)
+    dt = time.perf_counter() - t0
+    samples = rows2 * w
+    return {"value": round(samples / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"oracle main() camera paths, rows {y0}-{y0 + rows2} of frame 0 ({samples} px x 1 spp, "
+                      f"{dt:.1f} s); light tracing, photon gather and denoise not included"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--stride", type=int, default=31, help="pixel subsample stride for algorithmic bytes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).", file=sys.stderr)
+        return 2
+
+    from raymond_amd import capi, scenefile
+
+    p = scenefile.pack("spheres", SCENE_OPTS)
+    w, h = p.width, p.height
+    spp = scenefile.get_param(p.params, "samples_per_frame")
+    r0 = scenefile.get_param(p.params, "photon_radius")
+    caustics = scenefile.get_param(p.params, "caustics")
+
+    def frame_params(k: int) -> np.ndarray:
+        r = r0 * (k + 1) ** (-1.0 / 6.0)
+        return p.with_params(sample_offset=k * spp, total_samples=(k + 1) * spp, photon_radius=r,
+                             photon_cell=2 * r)
+
+    ctx = capi.Context(0)
+    device, cus = ctx.device_info()
+    scene = ctx.upload_scene(p)
+    ctx.environment(p.env_rgba, p.meta["envWidth"], p.meta["envHeight"], p.env_table, p.meta["envTableWidth"],
+                    p.meta["envTableHeight"])
+    ctx.resize(w, h)
+    emitters = p.emitters if caustics else None
+
+    k = 0
+    for _ in range(args.warmup):
+        ctx.frame(scene, frame_params(k), emitters)
+        k += 1
+    ctx.synchronize()
+    ctx.timing(True)
+    ctx.reset_stage_times()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.frame(scene, frame_params(k), emitters)
+        k += 1
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    stages = ctx.stage_times()
+    ctx.timing(False)
+    scene.release()
+    ctx.close()
+
+    samples = w * h * spp * args.steps
+    value = samples / elapsed / 1e6
+    trace_ms, trace_n = stages["trace"]
+    trace_avg_s = trace_ms / max(trace_n, 1) / 1e3
+
+    alg = algorithmic_bytes(p, frame_params(0), args.stride)
+    bytes_per_launch = alg["per_sample"] * w * h * spp + 96 * w * h
+    achieved = bytes_per_launch / trace_avg_s / 1e9
+    traffic = traffic_from_summary(TRAFFIC_SUMMARY)
+
+    line = {
+        "metric": "Msamples/s at 1080p 8-bounce; served p50/p99 latency vs plain, 1 MI355X",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 200-sphere scene generated and packed by the JS host modules",
+        "config": {"workload": WORKLOAD, "width": w, "height": h, "spp_per_frame": spp, "max_bounces": 8,
+                   "rr_from_bounce": 3, "nee_mis": True, "light_tracing": bool(caustics), "photon_gather": bool(caustics),
+                   "denoise_passes": 5, "frames_timed": args.steps, "device": device, "compute_units": cus,
+                   "path": "plain (one job, megakernel, FP32 BVH)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_main (path-tracing pass)", "launch_ms": round(trace_avg_s * 1e3, 4),
+                     "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                     "bytes_per_sample": round(alg["per_sample"], 1),
+                     "rays_per_sample": round(alg["rays_per_sample"], 3)},
+        "stages_ms_per_frame": {s: round(ms / max(n, 1), 4) for s, (ms, n) in stages.items() if n},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(p, frame_params(0))
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
