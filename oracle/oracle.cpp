@@ -1652,6 +1652,24 @@ void oracle_trace64(const OracleScene* s, const float* rays, uint32_t n, double*
     });
 }
 
+void oracle_brute_force(const OracleScene* s, const float* rays, uint32_t n, uint32_t prim_count, uint32_t* hits) {
+    Scene S = view(s);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        float best = r[3];
+        uint32_t best_p = NONE;
+        for (uint32_t k = 0; k < prim_count; ++k) {
+            V3 h = intersect(S, k, {r[0], r[1], r[2]}, {r[4], r[5], r[6]}, INF);
+            if (h.x > 0.0f && h.x < best) {
+                best = h.x;
+                best_p = k;
+            }
+        }
+        hits[(size_t)i * 2] = f_bits(best);
+        hits[(size_t)i * 2 + 1] = best_p;
+    }
+}
+
 void oracle_camera_rays(const uint8_t* params, float* rays) {
     const Params& P = *reinterpret_cast<const Params*>(params);
     for (uint32_t y = 0; y < P.height; ++y)

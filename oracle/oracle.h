@@ -48,6 +48,10 @@ void oracle_trace(const OracleScene* s, const float* rays, uint32_t n, int any_h
                   uint32_t* hits, uint32_t* counters, int threads);
 void oracle_trace64(const OracleScene* s, const float* rays, uint32_t n, double* hits, int threads);
 
+/* Exhaustive closest hit over primitives [0, prim_count), each tested with
+ * an unbounded t_max: 2 words per ray (t, primitive or 0xffffffff). */
+void oracle_brute_force(const OracleScene* s, const float* rays, uint32_t n, uint32_t prim_count, uint32_t* hits);
+
 /* Pixel-centre pinhole rays for a parameter block (camera_ray, src/shaders.mjs:1148-1166). */
 void oracle_camera_rays(const uint8_t* params, float* rays);
 

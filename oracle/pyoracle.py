@@ -44,6 +44,7 @@ def load() -> C.CDLL:
             "oracle_trace": (None, [P, P, U, C.c_int, P, P, C.c_int]),
             "oracle_trace64": (None, [P, P, U, P, C.c_int]),
             "oracle_camera_rays": (None, [P, P]),
+            "oracle_brute_force": (None, [P, P, U, U, P]),
             "oracle_main": (None, [P, P, P, P, P, U, U, C.c_int]),
             "oracle_path_stats": (None, [P, P, U, P]),
             "oracle_light_trace": (None, [P, P, P, U, U, P, P, P]),
@@ -81,6 +82,8 @@ class Oracle:
 
     def __init__(self, packed, textures=None, normal_maps=None):
         self.packed = packed
+        textures = textures if textures is not None else getattr(packed, "textures", None)
+        normal_maps = normal_maps if normal_maps is not None else getattr(packed, "normal_maps", None)
         m = packed.meta
         self._keep = [packed.primitives, packed.nodes, packed.materials, packed.attributes, packed.lights,
                       packed.env_rgba, packed.env_table]
@@ -119,6 +122,12 @@ class Oracle:
         hits = np.empty((r.shape[0], 4), np.float64)
         self.lib.oracle_trace64(self.ref, _p(r), r.shape[0], _p(hits), threads)
         return hits
+
+    def brute_force(self, rays: np.ndarray):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.empty((r.shape[0], 2), np.uint32)
+        self.lib.oracle_brute_force(self.ref, _p(r), r.shape[0], self.packed.primitive_count, _p(out))
+        return out
 
     def camera_rays(self, params=None):
         params = self.packed.params if params is None else params

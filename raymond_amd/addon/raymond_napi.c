@@ -330,18 +330,18 @@ static napi_value js_device_info(napi_env env, napi_callback_info info) {
 
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor props[] = {
-        {"open", NULL, js_open, NULL, NULL, NULL, napi_default, NULL},
-        {"close", NULL, js_close, NULL, NULL, NULL, napi_default, NULL},
-        {"uploadScene", NULL, js_upload_scene, NULL, NULL, NULL, napi_default, NULL},
-        {"releaseScene", NULL, js_release_scene, NULL, NULL, NULL, napi_default, NULL},
-        {"uploadTextures", NULL, js_upload_textures, NULL, NULL, NULL, napi_default, NULL},
-        {"uploadEnvironment", NULL, js_upload_environment, NULL, NULL, NULL, napi_default, NULL},
-        {"resize", NULL, js_resize, NULL, NULL, NULL, napi_default, NULL},
-        {"frame", NULL, js_frame, NULL, NULL, NULL, napi_default, NULL},
-        {"read", NULL, js_read, NULL, NULL, NULL, napi_default, NULL},
-        {"synchronize", NULL, js_synchronize, NULL, NULL, NULL, napi_default, NULL},
-        {"raycast", NULL, js_raycast, NULL, NULL, NULL, napi_default, NULL},
-        {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_default, NULL},
+        {"open", NULL, js_open, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"close", NULL, js_close, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"uploadScene", NULL, js_upload_scene, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"releaseScene", NULL, js_release_scene, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"uploadTextures", NULL, js_upload_textures, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"uploadEnvironment", NULL, js_upload_environment, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"resize", NULL, js_resize, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"frame", NULL, js_frame, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"read", NULL, js_read, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"synchronize", NULL, js_synchronize, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"raycast", NULL, js_raycast, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_enumerable, NULL},
     };
     if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return NULL;
     return exports;

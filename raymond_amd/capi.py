@@ -115,7 +115,16 @@ class Context:
                                    _ptr(nodes), nodes.size // 8, _ptr(mats), mats.size // 24,
                                    _ptr(attr), attr.size // 4, _ptr(lights), lights.size, C.byref(h)),
                "rm_scene_upload")
-        return Scene(self, h)
+        scene = Scene(self, h)
+        for layers, normal in ((getattr(packed, "textures", None), False), (getattr(packed, "normal_maps", None), True)):
+            if layers is not None:
+                self.textures(scene, normal, *layers)
+        return scene
+
+    def upload_environment(self, packed) -> None:
+        m = packed.meta
+        self.environment(packed.env_rgba, m["envWidth"], m["envHeight"], packed.env_table, m["envTableWidth"],
+                         m["envTableHeight"])
 
     def environment(self, rgba_half: np.ndarray, width: int, height: int, table: np.ndarray, tw: int, th: int) -> None:
         rgba = np.ascontiguousarray(rgba_half, dtype=np.uint16)

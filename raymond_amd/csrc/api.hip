@@ -71,7 +71,7 @@ struct rm_context {
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t width = 0, height = 0, capacity = 0;
-    DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats;
+    DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats, guides;
     DevBuf heads, photons, emitters, spill;
     DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
@@ -209,7 +209,7 @@ int rm_close(rm_context* c) {
         for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights, &s->textures, &s->nmaps}) b->release();
         delete s;
     }
-    for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->heads,
+    for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->heads,
                       &c->photons, &c->emitters, &c->spill, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
@@ -328,7 +328,7 @@ int rm_resize(rm_context* c, uint32_t width, uint32_t height) {
     uint32_t px = width * height;
     if (px > c->capacity) {
         size_t bytes = (size_t)px * 16;
-        for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats}) {
+        for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides}) {
             b->release();
             RM_HIP(b->ensure(bytes), "allocating per-pixel buffers");
         }
@@ -423,7 +423,8 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         float4* b = c->light_b.as<float4>();
         {
             Timed t(c, RM_STAGE_PREPARE);
-            rm::launch_prepare(P, c->accum.as<const float4>(), c->galb.as<const float4>(), c->gnorm.as<const float4>(),
+            rm::launch_guides(P, c->gnorm.as<const float4>(), c->guides.as<float4>(), c->stream);
+            rm::launch_prepare(P, c->accum.as<const float4>(), c->galb.as<const float4>(), c->guides.as<const float4>(),
                                a, c->stream);
         }
         const int steps[5] = {1, 2, 4, 8, 16};
@@ -431,7 +432,7 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         float4* out = b;
         for (int k = 0; k < 5; ++k) {
             Timed t(c, RM_STAGE_ATROUS);
-            rm::launch_atrous(P, steps[k], c->gnorm.as<const float4>(), in, out, c->stream);
+            rm::launch_atrous(P, steps[k], c->guides.as<const float4>(), in, out, c->stream);
             std::swap(in, out);
         }
         {

@@ -593,14 +593,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Par
 constexpr float KERNEL0 = 0.375f, KERNEL1 = 0.25f, KERNEL2 = 0.0625f;
 RM_D float kernel_w(int k) { return k == 0 ? KERNEL0 : (k == 1 ? KERNEL1 : KERNEL2); }
 
-struct DenoiseView {
-    const Params* P;
-    RM_D float samples() const { return (float)max(P->total_samples, 1u); }
-};
 RM_D V3 albedo_at(const float4* galb, uint32_t i, float n) {
     float4 g = galb[i];
     return vmax(v3(g.x, g.y, g.z) / n, s3(0.01f));
 }
+// The average normal (unit, or zero where none) and average distance.
 RM_D float4 normal_at(const float4* gnorm, uint32_t i, float n) {
     float4 g = gnorm[i];
     V3 v = {g.x, g.y, g.z};
@@ -614,10 +611,24 @@ RM_D float similarity(float4 p, float4 q, float step) {
     float wz = expf(-fabsf(p.w - q.w) / (p.w * 0.02f * step + 1e-6f));
     return wn * wz;
 }
+RM_D float pow64(float x) {
+    x *= x; x *= x; x *= x; x *= x; x *= x; x *= x;
+    return x;
+}
 
+// Each pixel's normal_at() once per frame, so the 25-tap passes read it
+// rather than renormalising every neighbour's guide.
+__global__ __launch_bounds__(BLOCK_THREADS) void k_guides(Params P, const float4* __restrict__ gnorm,
+                                                          float4* __restrict__ guides) {
+    uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    if (i >= P.width * P.height) return;
+    guides[i] = normal_at(gnorm, i, (float)max(P.total_samples, 1u));
+}
+
+// src/shaders.mjs:1974-2021
 __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float4* __restrict__ accum,
                                                            const float4* __restrict__ galb,
-                                                           const float4* __restrict__ gnorm, float4* light_out) {
+                                                           const float4* __restrict__ guides, float4* light_out) {
     uint32_t x, y;
     tile_pixel(x, y);
     if (x >= P.width || y >= P.height) return;
@@ -625,7 +636,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
     const float n = (float)max(P.total_samples, 1u);
     float4 ac = accum[i];
     V3 light = (v3(ac.x, ac.y, ac.z) / n) / albedo_at(galb, i, n);
-    float4 gp = normal_at(gnorm, i, n);
+    float4 gp = guides[i];
     float a = luminance(albedo_at(galb, i, n));
     float variance;
     if (n >= 4.0f) {
@@ -639,7 +650,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
                 int xx = (int)x + dx, yy = (int)y + dy;
                 if (xx < 0 || yy < 0 || xx >= (int)P.width || yy >= (int)P.height) continue;
                 uint32_t j = (uint32_t)yy * P.width + (uint32_t)xx;
-                float w = similarity(gp, normal_at(gnorm, j, n), 1.0f);
+                float w = similarity(gp, guides[j], 1.0f);
                 float4 aj = accum[j];
                 float l = luminance((v3(aj.x, aj.y, aj.z) / n) / albedo_at(galb, j, n));
                 sum += w * l;
@@ -652,42 +663,67 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
     light_out[i] = make_float4(light.x, light.y, light.z, variance);
 }
 
-__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const float4* __restrict__ gnorm,
+// src/shaders.mjs:2023-2086. The 25 taps' weights are the reference's
+// B3 × max(n·n', 0)^64 × exp(-|Δz| / (0.02 z step)) × exp(-|Δl| / σ), with
+// the power by six squarings, the two exponentials merged into one exp2,
+// and the centre's divisions hoisted to reciprocals: a few ulps from the
+// reference's per-tap pow/exp/divide, several times fewer instructions.
+__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const float4* __restrict__ guides,
                                                           const float4* __restrict__ light_in,
                                                           float4* __restrict__ light_out) {
     uint32_t x, y;
     tile_pixel(x, y);
     if (x >= P.width || y >= P.height) return;
     const uint32_t i = y * P.width + x;
-    const float n = (float)max(P.total_samples, 1u);
     const float4 centre = light_in[i];
-    const float4 gp = normal_at(gnorm, i, n);
+    const float4 gp = guides[i];
     const float lp = luminance(v3(centre.x, centre.y, centre.z));
+    const int W = (int)P.width, H = (int)P.height;
     float variance = 0.0f, variance_weight = 0.0f;
     for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
             int xx = (int)x + dx, yy = (int)y + dy;
-            if (xx >= 0 && yy >= 0 && xx < (int)P.width && yy < (int)P.height) {
+            if (xx >= 0 && yy >= 0 && xx < W && yy < H) {
                 float w = kernel_w(abs(dx)) * kernel_w(abs(dy));
                 variance += w * light_in[(uint32_t)yy * P.width + (uint32_t)xx].w;
                 variance_weight += w;
             }
         }
+    const float LOG2E = 1.4426950408889634f;
     const float sigma = 4.0f * sqrtf(variance / variance_weight) + 1e-4f;
+    const float k_l = LOG2E / sigma;
+    const bool sky = gp.w <= 0.0f;
+    const bool no_normal = gp.x * gp.x + gp.y * gp.y + gp.z * gp.z == 0.0f;
+    const float k_z = LOG2E / (gp.w * 0.02f * (float)step + 1e-6f);
     V3 sum = {0, 0, 0};
     float sum_variance = 0.0f, weight = 0.0f;
-    for (int dy = -2; dy <= 2; dy++)
+#pragma unroll
+    for (int dy = -2; dy <= 2; dy++) {
+        int yy = (int)y + dy * step;
+        if (yy < 0 || yy >= H) continue;
+#pragma unroll
         for (int dx = -2; dx <= 2; dx++) {
-            int xx = (int)x + dx * step, yy = (int)y + dy * step;
-            if (xx < 0 || yy < 0 || xx >= (int)P.width || yy >= (int)P.height) continue;
+            int xx = (int)x + dx * step;
+            if (xx < 0 || xx >= W) continue;
             uint32_t j = (uint32_t)yy * P.width + (uint32_t)xx;
             float4 q = light_in[j];
-            float w = kernel_w(abs(dx)) * kernel_w(abs(dy)) * similarity(gp, normal_at(gnorm, j, n), (float)step) *
-                      expf(-fabsf(lp - luminance(v3(q.x, q.y, q.z))) / sigma);
+            float4 g = guides[j];
+            float e = -fabsf(lp - luminance(v3(q.x, q.y, q.z))) * k_l;
+            float s;
+            if (sky || g.w <= 0.0f) {
+                s = (sky && g.w <= 0.0f) ? 1.0f : 0.0f;
+            } else if (no_normal) {
+                s = 0.0f;
+            } else {
+                s = pow64(fmaxf(gp.x * g.x + gp.y * g.y + gp.z * g.z, 0.0f));
+                e -= fabsf(gp.w - g.w) * k_z;
+            }
+            float w = kernel_w(abs(dx)) * kernel_w(abs(dy)) * s * exp2f(e);
             sum = sum + w * v3(q.x, q.y, q.z);
             sum_variance += w * w * q.w;
             weight += w;
         }
+    }
     if (weight > 0.0f) {
         V3 r = sum / weight;
         light_out[i] = make_float4(r.x, r.y, r.z, sum_variance / (weight * weight));
@@ -782,15 +818,21 @@ void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* he
                            spill, slots);
 }
 
-void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* gnorm, float4* out,
-                    hipStream_t stream) {
-    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
-    hipLaunchKernelGGL(k_prepare, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb, gnorm, out);
+void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStream_t stream) {
+    uint32_t n = P.width * P.height;
+    hipLaunchKernelGGL(k_guides, dim3((n + BLOCK_THREADS - 1) / BLOCK_THREADS), dim3(BLOCK_THREADS), 0, stream, P,
+                       gnorm, guides);
 }
 
-void launch_atrous(const Params& P, int step, const float4* gnorm, const float4* in, float4* out, hipStream_t stream) {
+void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* guides, float4* out,
+                    hipStream_t stream) {
     dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
-    hipLaunchKernelGGL(k_atrous, grid, dim3(BLOCK_THREADS), 0, stream, P, step, gnorm, in, out);
+    hipLaunchKernelGGL(k_prepare, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb, guides, out);
+}
+
+void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream) {
+    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+    hipLaunchKernelGGL(k_atrous, grid, dim3(BLOCK_THREADS), 0, stream, P, step, guides, in, out);
 }
 
 void launch_compose(const Params& P, const float4* galb, const float4* in, float4* out, hipStream_t stream) {

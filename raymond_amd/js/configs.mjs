@@ -233,4 +233,49 @@ export function mixed({ seed = 11, count = 60 } = {}) {
     return { scene, camera, width: 128, height: 128, maxBounces: 8 };
 }
 
-export const CONFIGS = Object.freeze({ cornell, spheres, triangles, caustics, mixed });
+import { fixture } from "./fixtures.mjs";
+
+
+
+const vec = (a) => new Vec3(a[0], a[1], a[2]);
+
+function materialFrom(m = {}) {
+    const o = Object.assign({}, m);
+    for (const k of ["color", "emission", "checker"])
+        if (Array.isArray(o[k])) o[k] = vec(o[k]);
+    return new Material(o);
+}
+
+const mat4From = (t) => (t ? new Mat4(...t) : Mat4.identity());
+
+/**
+ * A scene from a JSON description, for tests: { objects: [...], sky, camera,
+ * width, height, maxBounces }. Objects: {type: "sphere", center, radius},
+ * {type: "quad", corner, u, v}, {type: "box", transform (16, row-major)},
+ * {type: "boxCorners", min, max, rotateY}, {type: "stick", a, b, thickness},
+ * {type: "icosphere", subdivisions, transform}, {type: "torus", radius, tube,
+ * segments, sides, transform}; each with a material of Material's options.
+ */
+export function custom({ objects = [], sky = null, camera = {}, width = 64, height = 64, maxBounces = 8 } = {}) {
+    const skyOpts = {};
+    if (sky) for (const [k, v] of Object.entries(sky)) skyOpts[k] = Array.isArray(v) ? vec(v) : v;
+    const scene = new Scene({ sky: sky === "black" ? Sky.black() : new Sky(skyOpts) });
+    for (const o of objects) {
+        const m = materialFrom(o.material);
+        if (o.type == "sphere") scene.add(new Sphere(vec(o.center), o.radius, m, { rotation: mat4From(o.rotation) }));
+        else if (o.type == "quad") scene.add(new Quad(vec(o.corner), vec(o.u), vec(o.v), m));
+        else if (o.type == "box") scene.add(new Box(mat4From(o.transform), m));
+        else if (o.type == "boxCorners") scene.add(Box.fromCorners(vec(o.min), vec(o.max), m, o.rotateY || 0));
+        else if (o.type == "stick") scene.add(Box.between(vec(o.a), vec(o.b), o.thickness, m));
+        else if (o.type == "icosphere") scene.add(Mesh.icosphere(m, o.subdivisions || 3).transformed(mat4From(o.transform)));
+        else if (o.type == "torus") scene.add(Mesh.torus(m, o).transformed(mat4From(o.transform)));
+        else throw new RangeError(`Unknown object type "${o.type}".`);
+    }
+    const cam = new Camera({
+        position: vec(camera.position || [0, 0, 5]), target: vec(camera.target || [0, 0, 0]),
+        fov: camera.fov || 40 * Math.PI / 180, aperture: camera.aperture || 0, focusDistance: camera.focusDistance || 0,
+    });
+    return { scene, camera: cam, width, height, maxBounces };
+}
+
+export const CONFIGS = Object.freeze({ cornell, spheres, triangles, caustics, mixed, fixture, custom });

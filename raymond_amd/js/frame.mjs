@@ -262,3 +262,22 @@ export function environmentData(environment) {
     }
     return { rgba, width: map.width, height: map.height, table: table.data, tableWidth: table.width, tableHeight: table.height };
 }
+
+/** Largest texture layer size (src/Renderer.mjs:33). */
+export const MAX_TEXTURE_SIZE = 2048;
+
+/**
+ * Textures as one array of RGBA8 layers, each resized to the largest one's
+ * size (capped at MAX_TEXTURE_SIZE), as #uploadTextures packs them
+ * (src/Renderer.mjs:637-661): sRGB colour for textures, plain numbers for
+ * normal maps. null when there are none.
+ */
+export function textureLayers(textures, srgb) {
+    if (textures.length == 0)
+        return null;
+    const width = Math.min(MAX_TEXTURE_SIZE, Math.max(1, ...textures.map((t) => t.width)));
+    const height = Math.min(MAX_TEXTURE_SIZE, Math.max(1, ...textures.map((t) => t.height)));
+    const data = new Uint8Array(width * height * 4 * textures.length);
+    textures.forEach((t, i) => data.set(t.resized(width, height, srgb).data, i * width * height * 4));
+    return { data, width, height, layers: textures.length };
+}

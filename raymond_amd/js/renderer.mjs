@@ -9,7 +9,7 @@ import { dirname, join } from "path";
 import { fileURLToPath } from "url";
 import {
     IMAGE_KEY_BYTES, PARAMS_BYTES, VIEWS, deltaObjects, emitters, environmentData,
-    sceneTables, writeFrameParams, writeParams,
+    sceneTables, textureLayers, writeFrameParams, writeParams,
 } from "./frame.mjs";
 
 const here = dirname(fileURLToPath(import.meta.url));
@@ -58,6 +58,7 @@ export function packForDevice(scene, camera, width, height, state) {
         materials: packed.materials, materialCount: packed.materialCount,
         attributes: packed.attributes, tables, delta, emitters: emit, params,
         environment: environmentData(scene.sky.environment), causticsActive,
+        textures: textureLayers(packed.textures, true), normalMaps: textureLayers(packed.normalMaps, false),
     };
 }
 
@@ -146,6 +147,9 @@ export class Renderer {
             this.#sceneHandle = addon.uploadScene(ctx, packed.primitives, packed.primitiveCount,
                 packed.nodes, packed.nodeCount, packed.materials, packed.materialCount,
                 packed.attributes, this.#tables.table);
+            for (const [layers, normal] of [[textureLayers(packed.textures, true), false], [textureLayers(packed.normalMaps, false), true]])
+                if (layers)
+                    addon.uploadTextures(ctx, this.#sceneHandle, normal, layers.data, layers.width, layers.height, layers.layers);
             this.#uploadedScene = scene;
             this.#uploadedVersion = scene.version;
             this.#samples = 0;

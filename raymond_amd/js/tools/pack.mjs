@@ -19,6 +19,7 @@ if (!name || !out || !CONFIGS[name]) {
 const opts = json ? JSON.parse(json) : {};
 const t0 = Date.now();
 const made = CONFIGS[name](opts.scene || {});
+if (opts.camera) Object.assign(made.camera, opts.camera);
 const width = opts.width || made.width, height = opts.height || made.height;
 const state = {
     maxBounces: opts.maxBounces || made.maxBounces,
@@ -45,6 +46,15 @@ add("emitters", "f32", packed.emitters);
 add("params", "u8", new Uint8Array(packed.params));
 add("env_rgba", "u16", packed.environment.rgba);
 add("env_table", "f32", packed.environment.table);
+if (packed.textures) add("textures", "u8", packed.textures.data);
+// Which scene object each BVH primitive came from (object order of scene.add).
+{
+    const objects = made.scene.objects;
+    const index = new Map(objects.map((o, i) => [o, i]));
+    const owners = made.scene.pack().objects;
+    add("object_of", "u32", Uint32Array.from(owners, (o) => index.get(o)));
+}
+if (packed.normalMaps) add("normal_maps", "u8", packed.normalMaps.data);
 
 const meta = {
     config: name, options: opts, width, height,
@@ -55,6 +65,8 @@ const meta = {
     envWidth: packed.environment.width, envHeight: packed.environment.height,
     envTableWidth: packed.environment.tableWidth, envTableHeight: packed.environment.tableHeight,
     causticsActive: packed.causticsActive, packSeconds: tPack,
+    textures: packed.textures ? [packed.textures.width, packed.textures.height, packed.textures.layers] : null,
+    normalMaps: packed.normalMaps ? [packed.normalMaps.width, packed.normalMaps.height, packed.normalMaps.layers] : null,
     sections: [],
 };
 let offset = 0;

@@ -34,6 +34,8 @@ class Packed:
     emitters: np.ndarray | None     # f32 or None
     env_rgba: np.ndarray            # u16
     env_table: np.ndarray           # f32
+    textures: tuple | None = None       # (RGBA8 layers, width, height, layers), sRGB colour
+    normal_maps: tuple | None = None    # (RGBA8 layers, width, height, layers), plain
     extra: dict = field(default_factory=dict)
 
     @property
@@ -90,9 +92,12 @@ def read(path: str | Path) -> Packed:
     for s in meta["sections"]:
         raw = data[base + s["offset"]: base + s["offset"] + s["bytes"]]
         arrays[s["name"]] = np.frombuffer(raw, dtype=_DTYPES[s["dtype"]]).copy()
+    tex = (arrays["textures"], *meta["textures"]) if meta.get("textures") else None
+    nmap = (arrays["normal_maps"], *meta["normalMaps"]) if meta.get("normalMaps") else None
     return Packed(meta=meta, primitives=arrays["primitives"], nodes=arrays["nodes"], materials=arrays["materials"],
                   attributes=arrays["attributes"], lights=arrays["lights"], params=arrays["params"],
-                  emitters=arrays.get("emitters"), env_rgba=arrays["env_rgba"], env_table=arrays["env_table"])
+                  emitters=arrays.get("emitters"), env_rgba=arrays["env_rgba"], env_table=arrays["env_table"],
+                  textures=tex, normal_maps=nmap, extra={"object_of": arrays.get("object_of")})
 
 
 def pack(config: str, options: dict | None = None, cache: Path | None = None, timeout: float = 600) -> Packed:

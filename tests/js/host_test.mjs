@@ -1,0 +1,277 @@
+// Host-side behaviour of the JS modules, restating the reference's unit
+// tests: BVH invariants (tests/unit/bvh.test.mjs:78-146), packing and the
+// FP64 raycast (tests/unit/scene.test.mjs), camera (camera.test.mjs), the
+// parameter block layout (shaders.test.mjs:55-80), materials and the
+// environment. Prints one line per test and exits non-zero on a failure.
+import {
+    BVH, Box, Camera, EnvironmentMap, Mat4, Material, Mesh, Quad, Renderer, Scene, Sky, Sphere, Texture, Vec2, Vec3,
+} from "../../raymond_amd/js/index.mjs";
+import { lightChances, toHalf, writeParams, writeFrameParams, sceneTables, deltaObjects, emitters } from "../../raymond_amd/js/frame.mjs";
+
+let failed = 0;
+const tests = [];
+const test = (name, fn) => tests.push([name, fn]);
+const ok = (cond, msg) => { if (!cond) throw new Error(msg || "assertion failed"); };
+const close = (a, b, eps, msg) => ok(Math.abs(a - b) <= eps, `${msg || ""} expected ${b}, got ${a}`);
+const WHITE = Material.diffuse(new Vec3(0.8, 0.8, 0.8));
+
+function random(seed) {
+    return () => {
+        seed = (seed * 1664525 + 1013904223) % 4294967296;
+        return seed / 4294967296;
+    };
+}
+
+function walk(bvh, visit) {
+    const f = new Float32Array(bvh.nodes), u = new Uint32Array(bvh.nodes);
+    const stack = [[0, 0, []]];
+    while (stack.length) {
+        const [n, depth, ancestors] = stack.pop();
+        const box = { min: [f[n * 8], f[n * 8 + 1], f[n * 8 + 2]], max: [f[n * 8 + 4], f[n * 8 + 5], f[n * 8 + 6]] };
+        const first = u[n * 8 + 3], count = u[n * 8 + 7];
+        visit(n, depth, box, first, count, ancestors);
+        if (count == 0)
+            for (const c of [first, first + 1]) stack.push([c, depth + 1, ancestors.concat([box])]);
+    }
+}
+
+function randomBoxes(n, seed, spread = 100, size = 2) {
+    const r = random(seed);
+    const mins = new Float64Array(n * 3), maxs = new Float64Array(n * 3);
+    for (let i = 0; i < n * 3; ++i) {
+        mins[i] = r() * spread;
+        maxs[i] = mins[i] + r() * size;
+    }
+    return { mins, maxs };
+}
+
+test("every primitive is in exactly one leaf, inside every ancestor's box", () => {
+    const { mins, maxs } = randomBoxes(3000, 7);
+    const bvh = BVH.build(mins, maxs);
+    const seen = new Uint32Array(3000);
+    walk(bvh, (n, depth, box, first, count, ancestors) => {
+        if (count == 0) return;
+        for (let i = first; i < first + count; ++i) {
+            const p = bvh.order[i];
+            seen[p]++;
+            for (const b of ancestors.concat([box]))
+                for (let a = 0; a < 3; ++a)
+                    ok(mins[p * 3 + a] >= b.min[a] - 1e-4 * Math.max(1, Math.abs(b.min[a])) &&
+                        maxs[p * 3 + a] <= b.max[a] + 1e-4 * Math.max(1, Math.abs(b.max[a])), "primitive outside its box");
+        }
+    });
+    ok(seen.every((c) => c == 1), "a primitive is missing or repeated");
+});
+
+test("depth stays under 64 for 20,000 random boxes, and leaves hold at most 4 when separable", () => {
+    const { mins, maxs } = randomBoxes(20000, 3);
+    const bvh = BVH.build(mins, maxs);
+    let maxDepth = 0, maxLeaf = 0;
+    walk(bvh, (n, depth, box, first, count) => {
+        maxDepth = Math.max(maxDepth, depth);
+        maxLeaf = Math.max(maxLeaf, count);
+    });
+    ok(maxDepth < 64, `depth ${maxDepth}`);
+    ok(maxLeaf <= 4, `leaf of ${maxLeaf}`);
+});
+
+test("identical boxes still split into small leaves", () => {
+    const n = 40, mins = new Float64Array(n * 3).fill(1), maxs = new Float64Array(n * 3).fill(2);
+    let maxLeaf = 0;
+    walk(BVH.build(mins, maxs), (n_, d, b, first, count) => { maxLeaf = Math.max(maxLeaf, count); });
+    ok(maxLeaf <= 4, `leaf of ${maxLeaf}`);
+});
+
+test("two clusters are separated at the root", () => {
+    const mins = [], maxs = [];
+    for (let i = 0; i < 8; ++i) { mins.push(i * 0.1, 0, 0); maxs.push(i * 0.1 + 0.05, 0.05, 0.05); }
+    for (let i = 0; i < 8; ++i) { mins.push(100 + i * 0.1, 0, 0); maxs.push(100 + i * 0.1 + 0.05, 0.05, 0.05); }
+    const bvh = BVH.build(new Float64Array(mins), new Float64Array(maxs));
+    const u = new Uint32Array(bvh.nodes), f = new Float32Array(bvh.nodes);
+    const left = u[3];
+    ok(u[7] == 0, "root is inner");
+    ok(f[left * 8 + 4] < 50 || f[(left + 1) * 8 + 4] < 50, "a child holds only the near cluster");
+});
+
+test("an empty tree is one node of no size", () => {
+    const bvh = BVH.build(new Float64Array(0), new Float64Array(0));
+    ok(bvh.nodeCount == 1 && new Uint32Array(bvh.nodes)[7] == 0);
+});
+
+test("packing: counts, material dedupe, lights and the empty scene", () => {
+    const lamp = Material.light(new Vec3(1, 1, 1), 5);
+    const s = new Scene().add(
+        new Sphere(new Vec3(0, 0, 0), 1, WHITE), new Sphere(new Vec3(3, 0, 0), 1, WHITE),
+        new Quad(new Vec3(-1, 2, -1), new Vec3(2, 0, 0), new Vec3(0, 0, 2), lamp),
+        Mesh.icosphere(WHITE, 1).transformed(Mat4.translate(0, 5, 0)),
+        new Sphere(new Vec3(0, 0, 5), 1, Material.volume(new Vec3(1, 1, 1), 0.5)));
+    const p = s.pack();
+    ok(p.primitiveCount == 3 + 80, `${p.primitiveCount} primitives`);
+    ok(p.materialCount == 3, "materials dedupe by identity");
+    ok(p.lights.length == 1 && new Uint32Array(p.primitives)[p.lights[0] * 16 + 2] == 0, "the lamp is light 0");
+    ok(p.volumes.length == 1 && p.volumes[0] == p.primitiveCount, "volumes follow the BVH's primitives");
+    ok(new Uint32Array(p.primitives)[p.volumes[0] * 16] == 0, "the volume is a sphere record");
+    ok(p.attributes.length == 80 * 16, "one attribute block per smooth triangle");
+    const empty = new Scene().pack();
+    ok(empty.primitiveCount == 0 && new Uint32Array(empty.primitives)[0] == 0xffffffff, "empty scenes bind a NONE primitive");
+    ok(new Uint32Array(empty.nodes)[7] == 1, "and a root leaf");
+    ok(new Scene().raycast(new Vec3(0, 0, 5), new Vec3(0, 0, -1)) === null);
+    ok(new Scene().bounds() === null);
+});
+
+test("FP64 raycast: sphere, inside, quad sides, turned box, normals", () => {
+    const ball = new Scene().add(new Sphere(new Vec3(0, 0, 0), 1, WHITE));
+    const out = ball.raycast(new Vec3(0, 0, 5), new Vec3(0, 0, -1));
+    close(out.distance, 4, 1e-6); close(out.normal.z, 1, 1e-6); ok(out.frontFace);
+    const inside = ball.raycast(new Vec3(0, 0, 0), new Vec3(1, 0, 0));
+    close(inside.distance, 1, 1e-6); close(inside.normal.x, -1, 1e-6); ok(!inside.frontFace);
+    const quad = new Scene().add(new Quad(new Vec3(-1, 0, 1), new Vec3(2, 0, 0), new Vec3(0, 0, -2), WHITE));
+    const below = quad.raycast(new Vec3(0.5, -3, 0.5), new Vec3(0, 1, 0));
+    ok(!below.frontFace); close(below.normal.y, -1, 1e-6);
+    const box = new Scene().add(new Box(Mat4.translate(0, 1, 0).multiplyMat4(Mat4.rotateY(Math.PI / 2)).multiplyMat4(Mat4.scale(2, 1, 1)), WHITE));
+    const side = box.raycast(new Vec3(10, 1, 0), new Vec3(-1, 0, 0));
+    close(side.distance, 9, 1e-5); close(side.normal.x, 1, 1e-6);
+    const top = box.raycast(new Vec3(0.5, 10, 1.5), new Vec3(0, -1, 0));
+    close(top.distance, 8, 1e-5); close(top.normal.y, 1, 1e-6);
+});
+
+test("smooth icosphere normals stay near the sphere's", () => {
+    const scene = new Scene().add(Mesh.icosphere(WHITE, 3));
+    const next = random(5);
+    for (let n = 0; n < 50; ++n) {
+        const target = new Vec3(next() - 0.5, next() - 0.5, next() - 0.5).normalized().scale(0.3);
+        const origin = new Vec3(next() - 0.5, next() - 0.5, next() - 0.5).normalized().scale(4);
+        const hit = scene.raycast(origin, target.subtract(origin));
+        close(hit.point.length(), 1.0, 0.01, "radius");
+        ok(hit.normal.dot(hit.point.normalized()) > 0.999);
+    }
+});
+
+test("uv on sphere, quad and box faces", () => {
+    const q = new Scene().add(new Quad(new Vec3(0, 0, 0), new Vec3(2, 0, 0), new Vec3(0, 2, 0), WHITE));
+    const h = q.raycast(new Vec3(0.5, 1.5, 5), new Vec3(0, 0, -1));
+    close(h.uv.x, 0.25, 1e-9); close(h.uv.y, 0.25, 1e-9);
+    const s = new Scene().add(new Sphere(new Vec3(0, 0, 0), 1, WHITE));
+    const top = s.raycast(new Vec3(0, 5, 0), new Vec3(0, -1, 0));
+    close(top.uv.y, 0, 1e-6, "north pole at v = 0");
+    const eq = s.raycast(new Vec3(5, 0, 0), new Vec3(-1, 0, 0));
+    close(eq.uv.x, 0, 1e-9); close(eq.uv.y, 0.5, 1e-9);
+});
+
+test("camera basis and pixel rays", () => {
+    const c = new Camera({ position: new Vec3(0, 0, 5), target: new Vec3(0, 0, 0), fov: Math.PI / 2 });
+    const { right, up, forward } = c.basis();
+    close(right.x, 1, 1e-12); close(up.y, 1, 1e-12); close(forward.z, -1, 1e-12);
+    const r = c.pixelRay(0, 0, 2, 2);
+    close(r.direction.length(), 1, 1e-12);
+    close(r.direction.x / -r.direction.z, -0.5, 1e-12, "half a pixel in from the left edge");
+});
+
+test("parameter block layout and image key", () => {
+    const scene = new Scene({ sky: new Sky({ sunStrength: 2 }) }).add(new Sphere(new Vec3(0, 0, 0), 1, WHITE));
+    const camera = new Camera({ position: new Vec3(1, 2, 3), target: new Vec3(0, 0, 0), fov: 1.0, aperture: 0.1 });
+    const buf = new ArrayBuffer(208);
+    const state = { lightCount: 3, volumeCount: 2, maxBounces: 9, samplesPerFrame: 4, exposure: 1.5, fireflyClamp: 7,
+        view: "depth", denoise: true, caustics: false, samples: 12, delta: null };
+    writeParams(buf, scene, camera, 640, 480, state);
+    writeFrameParams(buf, camera, 480, state);
+    const f = new Float32Array(buf), u = new Uint32Array(buf);
+    close(f[0], 1, 0); close(f[2], 3, 0); close(f[3], 0.1, 1e-7);
+    close(f[7], Math.tan(0.5), 1e-7); close(f[15], 640 / 480, 1e-7);
+    ok(u[23] == 1 && u[31] == 3 && u[35] == 9 && u[36] == 3 && u[37] == 2);
+    ok(u[40] == 640 && u[41] == 480 && u[42] == 12 && u[43] == 4 && u[44] == 16);
+    close(f[45], 1.5, 0); ok(u[46] == 1 && u[47] == 0);
+    close(f[49], 2 * f[48], 0);
+    close(f[19], 2 * Math.sin(0.05 / 2) ** 2, 1e-9, "1 - cos of the sun's radius");
+});
+
+test("half floats round to nearest and clamp", () => {
+    ok(toHalf(1.0) == 0x3c00 && toHalf(0.5) == 0x3800 && toHalf(2.0) == 0x4000);
+    ok(toHalf(65504) == 0x7bff && toHalf(1e9) == 0x7bff && toHalf(-1) == 0 && toHalf(NaN) == 0);
+    ok(toHalf(1 + 1 / 2048) == 0x3c00 || toHalf(1 + 1 / 2048) == 0x3c01);
+    ok(toHalf(5.9605e-8) == 1, "smallest subnormal");
+});
+
+test("light chances: a quarter uniform, the rest by power", () => {
+    const c = lightChances([1, 3]);
+    close(c[0], 0.125 + 0.75 * 0.25, 1e-12); close(c[1], 0.125 + 0.75 * 0.75, 1e-12);
+    const z = lightChances([0, 0]);
+    close(z[0], 0.5, 0);
+});
+
+test("lights table and emitters for light tracing", () => {
+    const scene = new Scene({ sky: new Sky({ sunStrength: 1 }) }).add(
+        new Quad(new Vec3(-20, 0, 20), new Vec3(40, 0, 0), new Vec3(0, 0, -40), WHITE),
+        new Sphere(new Vec3(0, 0.9, 0), 0.5, Material.glass(1.5)),
+        new Sphere(new Vec3(0.3, 2.6, -0.2), 0.15, Material.light(new Vec3(1, 1, 1), 60)));
+    const t = sceneTables(scene.pack());
+    ok(t.lightCount == 1 && t.volumeCount == 0);
+    ok(new Float32Array(t.table.buffer)[1] == 1, "the last cumulative chance is exactly 1");
+    const d = deltaObjects(scene);
+    ok(d && d.targets.length == 1);
+    close(d.targets[0].radius, Math.sqrt(3) * 0.5 * 1.001, 1e-6);
+    const e = emitters(scene.sky, t.powers, d);
+    const u = new Uint32Array(e.buffer);
+    ok(u[0] == 2 && u[1] == 1, "the lamp and the sun; one target");
+    ok(e[4 + 4] == 1.0, "cumulative chance ends at 1");
+    ok(u[4 + 4 + 1] == 0xffffffff, "the sun has no primitive");
+});
+
+test("crown glass disperses as its Abbe number says", () => {
+    const g = Material.glass(1.5168, new Vec3(1, 1, 1), 64.17);
+    close(g.iorAt(486.1) - g.iorAt(656.3), (1.5168 - 1) / 64.17, 1e-9);
+    close(g.iorAt(587.6), 1.5168, 1e-12);
+    ok(Material.glass(1.5).dispersion == 0);
+});
+
+test("environment maps sanitise input and round-trip directions", () => {
+    const env = new EnvironmentMap(2, 1, [NaN, -1, 2, 1, Infinity, 0.5]);
+    ok(env.data[0] == 0 && env.data[1] == 0 && env.data[2] == 2 && env.data[4] == 0);
+    for (const [u, v] of [[0.1, 0.2], [0.7, 0.9], [0.5, 0.5]]) {
+        const [u2, v2] = EnvironmentMap.uv(EnvironmentMap.direction(u, v));
+        close(u2, u, 1e-12); close(v2, v, 1e-12);
+    }
+});
+
+test("Material clamps and rejects unknown types; views are checked", () => {
+    const m = new Material({ roughness: 3, ior: 0.5, anisotropy: 2 });
+    ok(m.roughness == 1 && m.ior == 1 && m.anisotropy == 0.99);
+    let threw = false;
+    try { new Material({ type: "chrome" }); } catch (e) { threw = e instanceof RangeError; }
+    ok(threw, "unknown type");
+    let bad = false;
+    try { new Scene().add({}); } catch (e) { bad = e instanceof TypeError; }
+    ok(bad, "scenes take scene objects only");
+});
+
+test("textures sample bilinearly and wrap", () => {
+    const t = new Texture(2, 1, [0, 0, 0, 255, 255, 255, 255, 255]);
+    const mid = t.sample(0.5, 0.5, false);
+    close(mid.x, 0.5, 1e-12);
+    const wrap = t.sample(0.0, 0.5, false);
+    close(wrap.x, 0.5, 1e-12, "the left edge blends with the right");
+});
+
+test("Renderer.create rejects with an Error when no GPU can be opened", async () => {
+    if (process.env.RAYMOND_EXPECT_GPU) return;
+    let threw = null;
+    try { await Renderer.create({ width: 4, height: 4 }); } catch (e) { threw = e; }
+    ok(threw instanceof Error, "create() should reject");
+});
+
+async function main() {
+    for (const [name, fn] of tests) {
+        try {
+            await fn();
+            console.log(`ok ${name}`);
+        }
+        catch (e) {
+            failed++;
+            console.log(`FAIL ${name}: ${e.stack}`);
+        }
+    }
+    console.log(`${tests.length - failed} passed, ${failed} failed`);
+    process.exit(failed ? 1 : 0);
+}
+
+main();

@@ -1,0 +1,34 @@
+// Drives the JS Renderer through the N-API addon on the GPU: renders the
+// Cornell box for a few frames and writes the averaged and denoised pixels to
+// the files given, for the Python side to compare with the C ABI's output.
+import { writeFileSync } from "fs";
+import { Renderer } from "../../raymond_amd/js/index.mjs";
+import { cornell } from "../../raymond_amd/js/configs.mjs";
+
+async function main() {
+    const [, , outAvg, outDen, frames] = process.argv;
+    const { scene, camera } = cornell();
+    const renderer = await Renderer.create({ width: 64, height: 64 });
+    renderer.scene = scene;
+    renderer.camera = camera;
+    renderer.fireflyClamp = 0;
+    renderer.caustics = false;
+    let threw = false;
+    try { await renderer.readPixels(); } catch (e) { threw = /Nothing has been rendered yet/.test(e.message); }
+    if (!threw) throw new Error("readPixels before render should throw");
+    for (let k = 0; k < Number(frames); ++k)
+        renderer.render();
+    await renderer.device.queue.onSubmittedWorkDone();
+    const avg = await renderer.readPixels();
+    const den = await renderer.readPixels({ denoised: true });
+    renderer.view = "albedo";
+    let bad = false;
+    try { await renderer.readPixels({ denoised: true }); } catch (e) { bad = /Only the color view/.test(e.message); }
+    if (!bad) throw new Error("denoised readback outside the colour view should throw");
+    writeFileSync(outAvg, Buffer.from(avg.buffer));
+    writeFileSync(outDen, Buffer.from(den.buffer));
+    console.log(JSON.stringify({ samples: renderer.samples }));
+    renderer.dispose();
+}
+
+main().catch((e) => { console.error(e.stack); process.exit(1); });

@@ -1,0 +1,54 @@
+"""Converged accuracy of the plain path: every pixel's mean after 1024 spp
+against a 65,536-spp reference rendered from a disjoint range of sample
+indices (so the two share no Sobol points), on the 200-sphere scene's camera
+at 160 × 90 with light tracing off. A pixel passes when its luminance is within
+3σ of the reference, σ from the 1024-spp render's own second moment (the
+accumulation's w holds Σ luminance²). Under a normal model 0.27% of pixels fall
+outside 3σ by chance; heavy-tailed paths (caustics through glass) push that up,
+so the bound asserted is 1%, and the image's relMSE (tools/converge.mjs:51-61)
+must be at most 1e-3."""
+import numpy as np
+import pytest
+
+from helpers import gpu_render, rel_mse
+from raymond_amd import capi
+from raymond_amd.scenefile import set_params
+
+pytestmark = pytest.mark.gpu
+
+OPTS = {"scene": {"width": 160, "height": 90}, "width": 160, "height": 90}
+
+
+def render(ctx, p, spp, per=64):
+    """spp samples per pixel, sample indices 0 .. spp-1, per samples a frame."""
+    scene = ctx.upload_scene(p)
+    try:
+        ctx.upload_environment(p)
+        ctx.resize(p.width, p.height)
+        for k in range(spp // per):
+            ctx.frame(scene, set_params(p.params.copy(), caustics=0, denoise=0, firefly_clamp=0.0,
+                                        samples_per_frame=per, sample_offset=k * per, total_samples=(k + 1) * per))
+        return ctx.read(capi.BUFFER_ACCUM).astype(np.float64) / spp
+    finally:
+        scene.release()
+
+
+def test_converged_means_within_three_sigma(ctx, pack):
+    p = pack("spheres", **OPTS)
+    test = render(ctx, p, 1024)
+    ref = render(ctx, p, 65536)
+    # The reference's first 1024 samples are the test's (same indices, same
+    # streams); subtracting them leaves the mean of samples 1024..65535.
+    ref_disjoint = (ref * 65536 - test * 1024) / (65536 - 1024)
+    lum_w = np.array([0.2126, 0.7152, 0.0722])
+    mean = test[..., :3] @ lum_w
+    second = test[..., 3]
+    var = np.maximum(second - mean * mean, 0.0) * 1024 / 1023
+    sigma = np.sqrt(var / 1024)
+    ref_l = ref_disjoint[..., :3] @ lum_w
+    outside = np.abs(mean - ref_l) > 3 * sigma + 1e-6
+    frac = outside.mean()
+    err = rel_mse(test.astype(np.float32), ref_disjoint.astype(np.float32))
+    print(f"outside 3 sigma: {frac:.4%}; relMSE at 1024 spp vs 64k-spp reference: {err:.3e}")
+    assert frac < 0.01, frac
+    assert err <= 1e-3, err

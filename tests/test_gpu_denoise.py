@@ -1,0 +1,56 @@
+"""The à-trous denoiser (src/shaders.mjs:1916-2098) against the oracle on the
+GPU's own accumulation and guides: prepare's spatial variance estimate (fewer
+than 4 samples) and its temporal one, five edge-avoiding passes, compose.
+The kernel evaluates each tap's weight with squarings and one exp2 where the
+reference calls pow and exp twice, so results agree to a relative 1e-4, the
+tolerance stated here. Also the reference's denoiser properties: it reduces
+error against a converged image, and converges to the plain average."""
+import numpy as np
+import pytest
+
+from helpers import agreement, gpu_render
+from oracle.pyoracle import Oracle
+from raymond_amd.scenefile import get_param
+
+pytestmark = pytest.mark.gpu
+
+ROOM = ("fixture", {"scene": {"name": "room", "size": 96}})
+
+
+@pytest.mark.parametrize("frames", [1, 3, 8], ids=["spatial1", "spatial3", "temporal8"])
+@pytest.mark.parametrize("config,opts", [ROOM, ("spheres", {"scene": {"width": 128, "height": 72}, "width": 128,
+                                                               "height": 72})], ids=["room", "spheres"])
+def test_denoiser_matches_oracle(ctx, pack, config, opts, frames):
+    from helpers import frame_params
+    p = pack(config, **opts)
+    params = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
+    acc, ga, gn, den = gpu_render(ctx, p, frames, params, denoised=True)
+    last = frame_params(p, frames - 1, params)
+    assert get_param(last, "total_samples") == frames
+    passes = Oracle(p).denoise(last, acc, ga, gn)
+    ref = passes[-1]
+    frac = agreement(den, ref, rel=1e-4, abs_=1e-6)
+    assert frac > 0.995, f"{frac:.4f} of pixels agree"
+    assert np.isfinite(den).all()
+
+
+def test_denoiser_reduces_error(ctx, pack):
+    """RMSE against a 1024-spp reference falls to at most 0.6× at 1 spp
+    (tests/browser/app.test.mjs:590-606)."""
+    p = pack(*ROOM[:1], **ROOM[1])
+    params = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
+    acc, ga, gn, den = gpu_render(ctx, p, 1, params, denoised=True)
+    ref = gpu_render(ctx, p, 1024, p.with_params(caustics=0, denoise=0, firefly_clamp=0.0))[0] / 1024
+    noisy = acc[..., :3]
+    rmse_noisy = np.sqrt(np.mean((noisy - ref[..., :3]) ** 2))
+    rmse_den = np.sqrt(np.mean((den[..., :3] - ref[..., :3]) ** 2))
+    assert rmse_den < 0.6 * rmse_noisy, (rmse_den, rmse_noisy)
+
+
+def test_denoiser_converges_to_average(ctx, pack):
+    """At 512 spp the denoised image is within 5% of the plain average (tests/browser/app.test.mjs:625-634)."""
+    p = pack(*ROOM[:1], **ROOM[1])
+    params = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
+    acc, ga, gn, den = gpu_render(ctx, p, 512, params, denoised=True)
+    avg = acc[..., :3] / 512
+    assert abs(den[..., :3].mean() - avg.mean()) < 0.05 * avg.mean()

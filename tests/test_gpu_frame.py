@@ -1,50 +1,82 @@
-"""The path-tracing kernel against the oracle under identical RNG streams:
-one sample per pixel, so each pixel's accumulation is one radiance() value.
-Paths only part where a transcendental (cos, exp, pow, ...) differs by an ulp
-between the GPU's and the CPU's maths library and flips a branch, so almost
-every pixel must agree to float rounding."""
+"""The path-tracing kernel against the oracle under identical RNG streams.
+
+With one sample per pixel, each pixel's accumulation is one radiance()
+value (src/shaders.mjs:1179-1359), so the GPU and the oracle compute the same
+path sample by sample. Paths only part where a transcendental (cos, exp, pow,
+atan2 ...) differs by an ulp between the GPU's and the CPU's maths library and
+flips a branch, so almost every pixel must agree to float rounding. Each case
+exercises a feature: materials and NEE+MIS (spheres), glass and boxes
+(cornell), volumes + environment map (triangles with fog), dispersion
+(caustics), textures and normal maps, patterned lamps, environment NEE."""
 import numpy as np
 import pytest
 
-from oracle.pyoracle import Oracle
-from raymond_amd import capi
+from helpers import agreement, gpu_render, oracle_render
 
 pytestmark = pytest.mark.gpu
 
-
-def render_first_frame(ctx, p, params):
-    scene = ctx.upload_scene(p)
-    try:
-        ctx.resize(p.width, p.height)
-        ctx.frame(scene, params)
-        return ctx.read(capi.BUFFER_ACCUM), ctx.read(capi.BUFFER_GUIDE_ALBEDO), ctx.read(capi.BUFFER_GUIDE_NORMAL)
-    finally:
-        scene.release()
-
-
-def oracle_first_frame(p, params):
-    o = Oracle(p)
-    shape = (p.height, p.width, 4)
-    acc, ga, gn = (np.zeros(shape, np.float32) for _ in range(3))
-    o.main(params, acc, ga, gn)
-    return acc, ga, gn
-
-
-def agreement(a, b, rel=1e-4, abs_=1e-5):
-    close = np.abs(a - b) <= abs_ + rel * np.abs(b)
-    return close.all(axis=-1).mean()
-
-
-@pytest.mark.parametrize("config,opts", [
+CASES = [
     ("spheres", {"scene": {"width": 160, "height": 90}, "width": 160, "height": 90}),
     ("cornell", {"width": 96, "height": 96}),
-])
+    ("mixed", {"width": 96, "height": 96}),
+    ("triangles", {"scene": {"grid": 48, "extras": 3, "envWidth": 128, "envHeight": 64}, "width": 96, "height": 54,
+                   "maxBounces": 12}),
+    ("caustics", {"scene": {"res": 24}, "width": 96, "height": 54}),
+    ("fixture", {"scene": {"name": "textured", "size": 48}}),
+    ("fixture", {"scene": {"name": "bumpy", "size": 48}}),
+    ("fixture", {"scene": {"name": "patternedLamps", "size": 48}}),
+    ("fixture", {"scene": {"name": "environmentFloor", "size": 48}}),
+    ("fixture", {"scene": {"name": "furnaceDispersive", "size": 48}}),
+    ("fixture", {"scene": {"name": "lampInFog", "size": 48}}),
+    ("fixture", {"scene": {"name": "blackSmoke", "size": 48}}),
+]
+
+
+def case_id(c):
+    return c[1]["scene"]["name"] if c[0] == "fixture" else c[0]
+
+
+@pytest.mark.parametrize("config,opts", CASES, ids=[case_id(c) for c in CASES])
 def test_first_frame_per_sample(ctx, pack, config, opts):
     p = pack(config, **opts)
     params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0)
-    gpu = render_first_frame(ctx, p, params)
-    cpu = oracle_first_frame(p, params)
+    gpu = gpu_render(ctx, p, 1, params)
+    cpu = oracle_render(p, 1, params)
     for name, g, c in zip(("accum", "albedo", "normal"), gpu, cpu):
         assert np.isfinite(g).all(), name
         frac = agreement(g, c)
-        assert frac > 0.97, f"{config} {name}: only {frac:.4f} of pixels agree"
+        assert frac > 0.97, f"{case_id((config, opts))} {name}: only {frac:.4f} of pixels agree"
+
+
+@pytest.mark.parametrize("config,opts", [CASES[0], CASES[3], CASES[5]], ids=["spheres", "triangles", "textured"])
+def test_later_frames_accumulate(ctx, pack, config, opts):
+    """Four frames: sample offsets 0..3 draw different Sobol points, and frames after the first add."""
+    p = pack(config, **opts)
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0)
+    gpu = gpu_render(ctx, p, 4, params)
+    cpu = oracle_render(p, 4, params)
+    frac = agreement(gpu[0], cpu[0], rel=1e-4, abs_=1e-4)
+    assert frac > 0.93, f"{config}: only {frac:.4f} of pixels agree after 4 frames"
+
+
+@pytest.mark.parametrize("view", [1, 2, 3], ids=["albedo", "normal", "depth"])
+@pytest.mark.parametrize("config,opts", [CASES[2], CASES[6]], ids=["mixed", "bumpy"])
+def test_inspection_views(ctx, pack, config, opts, view):
+    """Inspection views trace the pixel centre once (src/shaders.mjs:1377-1407): no randomness."""
+    p = pack(config, **opts)
+    params = p.with_params(caustics=0, denoise=0, view=view)
+    g = gpu_render(ctx, p, 1, params)[0]
+    c = oracle_render(p, 1, params)[0]
+    frac = agreement(g, c, rel=2e-6, abs_=1e-6)
+    assert frac > 0.995, f"view {view}: {frac:.4f}"
+    if view == 3:  # depth: the hit distance itself, bit-exact
+        assert np.array_equal(g[..., 0].view(np.uint32), c[..., 0].view(np.uint32))
+
+
+def test_firefly_clamp(ctx, pack):
+    p = pack(*CASES[0][:1], **CASES[0][1])
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=1.5)
+    g = gpu_render(ctx, p, 1, params)[0]
+    assert g[..., :3].max() <= 1.5 * (1 + 1e-6)
+    c = oracle_render(p, 1, params)[0]
+    assert agreement(g, c) > 0.97

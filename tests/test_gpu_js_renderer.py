@@ -1,0 +1,34 @@
+"""The JS Renderer end to end on the GPU: Renderer.create → render() ×4 →
+readPixels (average and denoised) → dispose, through raymond.node. Its
+average must equal, bit for bit, the same frames driven through the C ABI
+from the same packed buffers, since both run the same kernels on the same
+inputs."""
+import json
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import gpu_render
+from raymond_amd import capi
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_renderer_matches_c_abi(ctx, pack, tmp_path):
+    frames = 4
+    res = subprocess.run(["node", str(ROOT / "tests" / "js" / "renderer_gpu.mjs"), str(tmp_path / "avg.bin"),
+                          str(tmp_path / "den.bin"), str(frames)], capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, RAYMOND_EXPECT_GPU="1"))
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert json.loads(res.stdout.strip().splitlines()[-1])["samples"] == frames
+    avg = np.fromfile(tmp_path / "avg.bin", np.float32).reshape(64, 64, 4)
+    den = np.fromfile(tmp_path / "den.bin", np.float32).reshape(64, 64, 4)
+    p = pack("cornell", width=64, height=64, caustics=False)
+    params = p.with_params(firefly_clamp=0.0, caustics=0, denoise=1)
+    acc, _, _, d = gpu_render(ctx, p, frames, params, denoised=True)
+    np.testing.assert_array_equal(avg, acc / frames)
+    np.testing.assert_array_equal(den, d)

@@ -1,0 +1,50 @@
+"""Light tracing, splats and photon mapping against the oracle.
+
+A frame with caustics on runs the trace pass (which leaves caustic paths to
+light tracing), light_trace (photons through mirrors and glass, splatted to
+the camera in 1/65536 fixed point and deposited in the hash grid),
+resolve_splats and photon_gather (src/shaders.mjs:1425-1831). Photon paths use
+the same PCG streams on both sides; splats are integer sums, so they agree
+exactly wherever the paths do. Photons land in the grid in a different order
+on the GPU (one counter atomic per wave), so gathered sums agree to float
+rounding rather than bit for bit."""
+import numpy as np
+import pytest
+
+from helpers import agreement, gpu_render, oracle_render
+
+pytestmark = pytest.mark.gpu
+
+CAUSTIC_SCENES = [
+    ("fixture", {"scene": {"name": "causticLamp", "size": 64}}),
+    ("fixture", {"scene": {"name": "causticSun", "size": 64}}),
+    ("fixture", {"scene": {"name": "causticMirror", "size": 64}}),
+    ("fixture", {"scene": {"name": "causticThroughGlass", "size": 64}}),
+    ("cornell", {"width": 96, "height": 96}),
+    ("caustics", {"scene": {"res": 24}, "width": 96, "height": 54}),
+]
+IDS = ["lamp", "sun", "mirror", "through_glass", "cornell", "dispersive_water"]
+
+
+@pytest.mark.parametrize("config,opts", CAUSTIC_SCENES, ids=IDS)
+def test_caustic_frame_matches_oracle(ctx, pack, config, opts):
+    p = pack(config, **opts)
+    assert p.meta["causticsActive"], "the scene should turn light tracing on"
+    params = p.with_params(caustics=1, denoise=0, firefly_clamp=0.0)
+    g = gpu_render(ctx, p, 2, params)[0]
+    c = oracle_render(p, 2, params)[0]
+    assert np.isfinite(g).all()
+    frac = agreement(g, c, rel=1e-3, abs_=1e-4)
+    assert frac > 0.97, f"only {frac:.4f} of pixels agree"
+    total_g, total_c = g[..., :3].sum(), c[..., :3].sum()
+    assert abs(total_g - total_c) <= 0.01 * abs(total_c) + 1e-3, (total_g, total_c)
+
+
+def test_light_tracing_adds_light(ctx, pack):
+    """With caustics on, the caustic under the glass ball comes from splats and gathers, not from nothing."""
+    p = pack(*CAUSTIC_SCENES[0][:1], **CAUSTIC_SCENES[0][1])
+    on = gpu_render(ctx, p, 4, p.with_params(caustics=1, denoise=0, firefly_clamp=0.0))[0]
+    off_params = p.with_params(caustics=1, denoise=0, firefly_clamp=0.0)
+    # The trace pass alone with caustics set leaves the caustic paths out.
+    only_trace = oracle_render(p, 4, off_params, light=False)[0]
+    assert on[..., :3].sum() > only_trace[..., :3].sum() * 1.01
