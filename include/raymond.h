@@ -47,6 +47,12 @@ extern "C" {
 #define RM_STAGE_RAYCAST 8
 #define RM_STAGE_COUNT 9
 
+/* Options for rm_set_option. */
+#define RM_OPTION_PERSISTENT 0         /* 1 (default): the path pass keeps lanes busy by regenerating finished
+                                          paths from a tile queue; 0: one thread per pixel. Same results. */
+#define RM_OPTION_PERSISTENT_BLOCKS 1  /* blocks of 256 threads for the persistent pass; 0 (default): fill the GPU */
+#define RM_OPTION_COUNT 2
+
 typedef struct rm_context rm_context;
 typedef struct rm_scene rm_scene;
 
@@ -120,6 +126,10 @@ int rm_raycast(rm_context* ctx, const rm_scene* scene, const float* rays, uint32
 int rm_timing(rm_context* ctx, int enabled);
 int rm_stage_times(rm_context* ctx, double* ms, uint32_t* launches, uint32_t n);
 int rm_reset_stage_times(rm_context* ctx);
+
+/* Sets an RM_OPTION_* for later frames. No counterpart in the reference:
+ * these choose among equivalent kernel schedules (and let a tuner pick). */
+int rm_set_option(rm_context* ctx, int option, int64_t value);
 
 /* The device's name and compute-unit count. */
 int rm_device_info(rm_context* ctx, char* name, size_t name_len, int* compute_units);

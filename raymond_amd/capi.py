@@ -37,7 +37,9 @@ SIGNATURES = {
     "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
     "rm_reset_stage_times": (C.c_int, [_P]),
     "rm_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
+    "rm_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
+OPTION_PERSISTENT, OPTION_PERSISTENT_BLOCKS = 0, 1
 
 _lib = None
 
@@ -162,6 +164,9 @@ class Context:
         _check(load().rm_raycast(self.handle, scene.handle, _ptr(r), n, int(any_hit), _ptr(hits), _ptr(cnt)),
                "rm_raycast")
         return (hits, cnt) if counters else hits
+
+    def set_option(self, option: int, value: int) -> None:
+        _check(load().rm_set_option(self.handle, option, value), "rm_set_option")
 
     def timing(self, enabled: bool) -> None:
         _check(load().rm_timing(self.handle, int(enabled)), "rm_timing")

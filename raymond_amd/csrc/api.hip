@@ -75,7 +75,9 @@ struct rm_context {
     DevBuf heads, photons, emitters, spill;
     DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
-    DevBuf rays, hits, counters;
+    DevBuf rays, hits, counters, work;
+    int compute_units = 0;
+    int64_t options[RM_OPTION_COUNT] = {1, 0};
     bool timing = false;
     StageTimer timer;
     std::vector<rm_scene*> scenes;
@@ -197,7 +199,24 @@ int rm_open(int device, rm_context** out) {
         rm_close(c);
         return r;
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->compute_units = prop.multiProcessorCount;
+    if (c->work.ensure(256) != hipSuccess) {
+        rm_close(c);
+        return fail(RM_ERR_MEMORY, "allocating the work counter");
+    }
     *out = c;
+    return RM_OK;
+}
+
+int rm_set_option(rm_context* c, int option, int64_t value) {
+    if (!c) return fail(RM_ERR_INVALID, "rm_set_option needs a context.");
+    if (option < 0 || option >= RM_OPTION_COUNT) return fail(RM_ERR_INVALID, "Unknown option " + std::to_string(option) + ".");
+    if (option == RM_OPTION_PERSISTENT && value != 0 && value != 1)
+        return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT is 0 or 1.");
+    if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
+        return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
+    c->options[option] = value;
     return RM_OK;
 }
 
@@ -210,7 +229,7 @@ int rm_close(rm_context* c) {
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->heads,
-                      &c->photons, &c->emitters, &c->spill, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters})
+                      &c->photons, &c->emitters, &c->spill, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
@@ -377,13 +396,25 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
     rm::DevScene S = dev_scene(c, s);
     rm::Features f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
                       s->dispersive};
-    const uint32_t slots = c->width * c->height;
+    uint32_t slots = c->width * c->height;
+    uint32_t persistent = 0;
+    if (c->options[RM_OPTION_PERSISTENT] && P.view == 0u) {
+        int64_t blocks = c->options[RM_OPTION_PERSISTENT_BLOCKS];
+        if (blocks == 0) blocks = (int64_t)(c->compute_units > 0 ? c->compute_units : 1) * rm::main_resident_blocks_per_cu(f);
+        persistent = (uint32_t)blocks;
+        uint32_t threads = persistent * rm::BLOCK_THREADS;
+        int r = ensure_spill(c, threads > slots ? threads : slots);
+        if (r != RM_OK) return r;
+        slots = threads;
+        rm::launch_fill_u32(c->work.as<uint32_t>(), 0u, 1, c->stream);
+    }
     {
         Timed t(c, RM_STAGE_TRACE);
         rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(),
-                         c->spill.as<uint32_t>(), slots};
+                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>()};
         rm::launch_main(L, f, c->stream);
     }
+    slots = c->width * c->height;
     RM_HIP(hipGetLastError(), "launching the trace");
 
     if (caustics) {

@@ -61,135 +61,164 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_raycast(DevScene S, const flo
 // ------------------------------------------------------------------ radiance
 struct Guides { V3 albedo; float4 normal; };
 
-// One camera path through pixel position px (src/shaders.mjs:1179-1359).
+// A camera path between bounces: radiance()'s loop variables
+// (src/shaders.mjs:1179-1205), so the loop can also run one bounce at a time.
+struct PathState {
+    V3 o, d, throughput, result, last_p;
+    float last_pdf, travelled, wavelength, spectrum_u;
+    uint32_t depth;
+    bool last_delta, guided, caustic, diffuse_seen;
+};
+
+// Seeds a sample's RNG streams and starts its camera path: main()'s per-sample
+// setup and radiance()'s prologue (src/shaders.mjs:1852-1865, 1181-1204).
+RM_D void path_begin(const Params& P, Rng& g, PathState& ps, Guides& gd, uint32_t x, uint32_t y, uint32_t index,
+                     uint32_t sample) {
+    g.state = pcg(index ^ pcg(sample));
+    g.qmc = true;
+    g.seed = pcg(index + 0x9e3779b9u);
+    g.index = sample;
+    g.dim = 0u;
+    ps.spectrum_u = (float)((pcg(index * 7919u + 1u) + sample * 2654435769u) >> 8u) * (1.0f / 16777216.0f);
+    V2 j = g.rand2();
+    gd.albedo = {1.0f, 1.0f, 1.0f};
+    gd.normal = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ps.guided = false;
+    ps.travelled = 0.0f;
+    Ray ray = camera_ray(P, g, {(float)x + j.x, (float)y + j.y}, true);
+    ps.o = ray.o;
+    ps.d = ray.d;
+    ps.throughput = {1.0f, 1.0f, 1.0f};
+    ps.result = {0.0f, 0.0f, 0.0f};
+    ps.last_pdf = 0.0f;
+    ps.last_delta = true;
+    ps.last_p = ray.o;
+    ps.wavelength = 0.0f;
+    ps.caustic = false;
+    ps.diffuse_seen = false;
+    ps.depth = 0u;
+}
+
+// One pass of radiance()'s bounce loop (src/shaders.mjs:1206-1356) at
+// ps.depth. Returns true when the path has ended (the loop's break, or its
+// last bounce); ps.result then holds the sample.
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, class STACK>
-RM_D V3 radiance(const Ctx& C, Rng& g, STACK& stack, V2 px, Guides& gd, float spectrum_u) {
+RM_D bool path_step(const Ctx& C, Rng& g, STACK& stack, PathState& ps, Guides& gd) {
     const Params& P = *C.P;
     const DevScene& S = C.S;
     NoCount nc;
-    gd.albedo = {1.0f, 1.0f, 1.0f};
-    gd.normal = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    bool guided = false;
-    float travelled = 0.0f;
-    Ray ray = camera_ray(P, g, px, true);
-    V3 o = ray.o, d = ray.d;
-    V3 throughput = {1.0f, 1.0f, 1.0f};
-    V3 result = {0.0f, 0.0f, 0.0f};
-    float last_pdf = 0.0f;
-    bool last_delta = true;
-    V3 last_p = o;
     const uint32_t choices = light_choices<ENV>(C);
-    float wavelength = 0.0f;
-    bool caustic = false, diffuse_seen = false;
+    const uint32_t depth = ps.depth;
+    g.dim = (depth + 1u) << 16u;
+    Hit hit = trace<false>(S, ps.o, ps.d, INF, stack, nc);
 
-    for (uint32_t depth = 0u; depth <= P.max_bounces; depth++) {
-        g.dim = (depth + 1u) << 16u;
-        Hit hit = trace<false>(S, o, d, INF, stack, nc);
-
-        if (VOLUMES && P.volume_count > 0u) {
-            Scatter sc = scatter_in_volumes(C, g, o, d, hit.t);
-            if (sc.volume != NONE) {
-                Mat vm = load_mat(S, __float_as_uint(S.prims[(size_t)sc.volume * 4].y));
-                V3 p = o + d * sc.t;
-                travelled += sc.t;
-                if (!guided) {
-                    guided = true;
-                    gd.albedo = throughput * vm.color;
-                    gd.normal = make_float4(-d.x, -d.y, -d.z, travelled);
-                }
-                if (depth == P.max_bounces) break;
-                throughput = throughput * vm.color;
-                if (choices > 0u) {
-                    LightSample ls = sample_light<ENV, TEXTURES>(C, g, p);
-                    if (ls.pdf > 0.0f) {
-                        float phase = henyey_greenstein(dot(d, ls.wi), vm.anisotropy);
-                        float reach = ls.dist * (float)(1.0 - 1e-3);
-                        if (trace<true>(S, p, ls.wi, reach, stack, nc).prim == NONE)
-                            result = result + throughput * ls.radiance *
-                                                  (phase * transmittance(C, p, ls.wi, reach) *
-                                                   power_heuristic(ls.pdf, phase) / ls.pdf);
-                    }
-                }
-                V3 wi = sample_henyey_greenstein(g, d, vm.anisotropy);
-                last_pdf = henyey_greenstein(dot(d, wi), vm.anisotropy);
-                last_delta = false;
-                last_p = p;
-                o = p;
-                d = wi;
-                caustic = false;
-                diffuse_seen = true;
-                if (!roulette(g, throughput, depth)) break;
-                continue;
+    if (VOLUMES && P.volume_count > 0u) {
+        Scatter sc = scatter_in_volumes(C, g, ps.o, ps.d, hit.t);
+        if (sc.volume != NONE) {
+            Mat vm = load_mat(S, __float_as_uint(S.prims[(size_t)sc.volume * 4].y));
+            V3 p = ps.o + ps.d * sc.t;
+            ps.travelled += sc.t;
+            if (!ps.guided) {
+                ps.guided = true;
+                gd.albedo = ps.throughput * vm.color;
+                gd.normal = make_float4(-ps.d.x, -ps.d.y, -ps.d.z, ps.travelled);
             }
-        }
-
-        if (hit.prim == NONE) {
-            if (!guided) gd.albedo = throughput;
-            float sky_weight = 1.0f;
-            if (has_env<ENV>(C) && !last_delta)
-                sky_weight = power_heuristic(last_pdf, env_pdf(C, d) / (float)choices);
-            result = result + throughput * sky<ENV>(C, d) * sky_weight;
-            if (in_sun(P, d) && !(caustic && last_delta)) {
-                float w = 1.0f;
-                if (!last_delta) w = power_heuristic(last_pdf, 1.0f / ((float)choices * 2.0f * PI * P.sun_omc));
-                result = result + throughput * pv3(P.sun_radiance) * w;
-            }
-            break;
-        }
-
-        Surface s = surface<TEXTURES>(S, o, d, hit);
-        Mat m = load_mat(S, s.material);
-        travelled += hit.t;
-
-        if (!guided && (m.kind == LIGHT || !is_delta(m) || depth == P.max_bounces)) {
-            guided = true;
-            gd.albedo = throughput * (m.kind == LIGHT ? v3(1.0f, 1.0f, 1.0f) : albedo<TEXTURES>(S, m, s.p, s.uv));
-            gd.normal = make_float4(s.n.x, s.n.y, s.n.z, travelled);
-        }
-
-        bool glows = m.emission.x > 0.0f || m.emission.y > 0.0f || m.emission.z > 0.0f;
-        if (s.front && glows && !(caustic && last_delta && s.light != NONE)) {
-            float w = 1.0f;
-            if (!last_delta && s.light != NONE) w = power_heuristic(last_pdf, light_pdf<ENV>(C, hit.prim, last_p, d, hit.t));
-            result = result + throughput * emitted<TEXTURES>(S, m, s.uv) * w;
-        }
-
-        if (m.kind == LIGHT) break;
-        bool delta = is_delta(m);
-        if (!delta) {
-            caustic = P.caustics != 0u && !diffuse_seen;
-            diffuse_seen = true;
-        }
-        if (depth == P.max_bounces) break;
-
-        V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
-        V3 wo = -d;
-
-        if (!delta && choices > 0u) {
-            LightSample ls = sample_light<ENV, TEXTURES>(C, g, s.p);
-            if (ls.pdf > 0.0f && dot(ls.wi, s.ng) > 0.0f) {
-                Eval e = eval_bsdf(m, base, s.n, wo, ls.wi);
-                V3 origin = offset(s, ls.wi);
-                float reach = (ls.dist - dot(origin - s.p, ls.wi)) * (float)(1.0 - 1e-3);
-                if (e.pdf > 0.0f && trace<true>(S, origin, ls.wi, reach, stack, nc).prim == NONE) {
-                    float tr = VOLUMES ? transmittance(C, origin, ls.wi, reach) : 1.0f;
-                    result = result + throughput * e.f * ls.radiance * (tr * power_heuristic(ls.pdf, e.pdf) / ls.pdf);
+            if (depth == P.max_bounces) return true;
+            ps.throughput = ps.throughput * vm.color;
+            if (choices > 0u) {
+                LightSample ls = sample_light<ENV, TEXTURES>(C, g, p);
+                if (ls.pdf > 0.0f) {
+                    float phase = henyey_greenstein(dot(ps.d, ls.wi), vm.anisotropy);
+                    float reach = ls.dist * (float)(1.0 - 1e-3);
+                    if (trace<true>(S, p, ls.wi, reach, stack, nc).prim == NONE)
+                        ps.result = ps.result + ps.throughput * ls.radiance *
+                                                    (phase * transmittance(C, p, ls.wi, reach) *
+                                                     power_heuristic(ls.pdf, phase) / ls.pdf);
                 }
             }
+            V3 wi = sample_henyey_greenstein(g, ps.d, vm.anisotropy);
+            ps.last_pdf = henyey_greenstein(dot(ps.d, wi), vm.anisotropy);
+            ps.last_delta = false;
+            ps.last_p = p;
+            ps.o = p;
+            ps.d = wi;
+            ps.caustic = false;
+            ps.diffuse_seen = true;
+            if (!roulette(g, ps.throughput, depth)) return true;
+            ps.depth = depth + 1u;
+            return ps.depth > P.max_bounces;
         }
-
-        Mat dm = dispersed<DISPERSION>(m, wavelength, throughput, spectrum_u);
-        BsdfSample bs = sample_bsdf(g, dm, base, s, wo);
-        if (!bs.ok) break;
-        throughput = throughput * bs.weight;
-        last_pdf = bs.pdf;
-        last_delta = bs.delta;
-        last_p = s.p;
-        o = offset(s, bs.wi);
-        d = bs.wi;
-        if (!roulette(g, throughput, depth)) break;
     }
-    return result;
+
+    if (hit.prim == NONE) {
+        if (!ps.guided) gd.albedo = ps.throughput;
+        float sky_weight = 1.0f;
+        if (has_env<ENV>(C) && !ps.last_delta)
+            sky_weight = power_heuristic(ps.last_pdf, env_pdf(C, ps.d) / (float)choices);
+        ps.result = ps.result + ps.throughput * sky<ENV>(C, ps.d) * sky_weight;
+        if (in_sun(P, ps.d) && !(ps.caustic && ps.last_delta)) {
+            float w = 1.0f;
+            if (!ps.last_delta) w = power_heuristic(ps.last_pdf, 1.0f / ((float)choices * 2.0f * PI * P.sun_omc));
+            ps.result = ps.result + ps.throughput * pv3(P.sun_radiance) * w;
+        }
+        return true;
+    }
+
+    Surface s = surface<TEXTURES>(S, ps.o, ps.d, hit);
+    Mat m = load_mat(S, s.material);
+    ps.travelled += hit.t;
+
+    if (!ps.guided && (m.kind == LIGHT || !is_delta(m) || depth == P.max_bounces)) {
+        ps.guided = true;
+        V3 a = m.kind == LIGHT ? v3(1.0f, 1.0f, 1.0f) : albedo<TEXTURES>(S, m, s.p, s.uv);
+        gd.albedo = ps.throughput * a;
+        gd.normal = make_float4(s.n.x, s.n.y, s.n.z, ps.travelled);
+    }
+
+    bool glows = m.emission.x > 0.0f || m.emission.y > 0.0f || m.emission.z > 0.0f;
+    if (s.front && glows && !(ps.caustic && ps.last_delta && s.light != NONE)) {
+        float w = 1.0f;
+        if (!ps.last_delta && s.light != NONE)
+            w = power_heuristic(ps.last_pdf, light_pdf<ENV>(C, hit.prim, ps.last_p, ps.d, hit.t));
+        ps.result = ps.result + ps.throughput * emitted<TEXTURES>(S, m, s.uv) * w;
+    }
+
+    if (m.kind == LIGHT) return true;
+    bool delta = is_delta(m);
+    if (!delta) {
+        ps.caustic = P.caustics != 0u && !ps.diffuse_seen;
+        ps.diffuse_seen = true;
+    }
+    if (depth == P.max_bounces) return true;
+
+    V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
+    V3 wo = -ps.d;
+
+    if (!delta && choices > 0u) {
+        LightSample ls = sample_light<ENV, TEXTURES>(C, g, s.p);
+        if (ls.pdf > 0.0f && dot(ls.wi, s.ng) > 0.0f) {
+            Eval e = eval_bsdf(m, base, s.n, wo, ls.wi);
+            V3 origin = offset(s, ls.wi);
+            float reach = (ls.dist - dot(origin - s.p, ls.wi)) * (float)(1.0 - 1e-3);
+            if (e.pdf > 0.0f && trace<true>(S, origin, ls.wi, reach, stack, nc).prim == NONE) {
+                float tr = VOLUMES ? transmittance(C, origin, ls.wi, reach) : 1.0f;
+                ps.result = ps.result + ps.throughput * e.f * ls.radiance * (tr * power_heuristic(ls.pdf, e.pdf) / ls.pdf);
+            }
+        }
+    }
+
+    m.ior = dispersed_ior<DISPERSION>(m, ps.wavelength, ps.throughput, ps.spectrum_u);
+    BsdfSample bs = sample_bsdf(g, m, base, s, wo);
+    if (!bs.ok) return true;
+    ps.throughput = ps.throughput * bs.weight;
+    ps.last_pdf = bs.pdf;
+    ps.last_delta = bs.delta;
+    ps.last_p = s.p;
+    ps.o = offset(s, bs.wi);
+    ps.d = bs.wi;
+    if (!roulette(g, ps.throughput, depth)) return true;
+    ps.depth = depth + 1u;
+    return ps.depth > P.max_bounces;
 }
 
 // The albedo, normal or depth seen through a pixel centre (src/shaders.mjs:1377-1407).
@@ -213,9 +242,51 @@ RM_D V3 inspect(const Ctx& C, Rng& g, STACK& stack, V2 px) {
     return s3(hit.t);
 }
 
+// A pixel's sums over the frame's samples (src/shaders.mjs:1846-1894).
+struct PixelSums {
+    V3 c, albedo;
+    float w;
+    float4 normal;
+    RM_D void clear() {
+        c = {0, 0, 0};
+        albedo = {0, 0, 0};
+        w = 0.0f;
+        normal = make_float4(0, 0, 0, 0);
+    }
+    // Adds one sample, dropped whole if anything in it overflowed.
+    RM_D void add(const Params& P, V3 v, const Guides& gd, bool clamp) {
+        if (clamp) {
+            float peak = fmaxf(v.x, fmaxf(v.y, v.z));
+            if (P.firefly_clamp > 0.0f && peak > P.firefly_clamp) v = v * (P.firefly_clamp / peak);
+        }
+        if (finite3(v) && finite3(gd.albedo) && finite3(v3(gd.normal.x, gd.normal.y, gd.normal.z)) &&
+            finite3(s3(gd.normal.w))) {
+            float l = luminance(v);
+            c = c + v;
+            w += l * l;
+            albedo = albedo + gd.albedo;
+            normal = make_float4(normal.x + gd.normal.x, normal.y + gd.normal.y, normal.z + gd.normal.z,
+                                 normal.w + gd.normal.w);
+        }
+    }
+    // The first frame (sample offset 0) overwrites; later ones add.
+    RM_D void store(const Params& P, uint32_t index, float4* accum, float4* galb, float4* gnorm) const {
+        if (P.sample_offset == 0u) {
+            accum[index] = make_float4(c.x, c.y, c.z, w);
+            galb[index] = make_float4(albedo.x, albedo.y, albedo.z, 0.0f);
+            gnorm[index] = normal;
+        } else {
+            float4 a = accum[index], ga = galb[index], gn = gnorm[index];
+            accum[index] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + w);
+            galb[index] = make_float4(ga.x + albedo.x, ga.y + albedo.y, ga.z + albedo.z, ga.w + 0.0f);
+            gnorm[index] = make_float4(gn.x + normal.x, gn.y + normal.y, gn.z + normal.z, gn.w + normal.w);
+        }
+    }
+};
+
 // ------------------------------------------------------------------ main
-// src/shaders.mjs:1838-1895. Each pixel's sum over the frame's samples
-// replaces (first frame) or adds to its accumulation and guides.
+// src/shaders.mjs:1838-1895, one thread per pixel: the frame's samples one
+// after another, each path run to its end.
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_main(DevScene S, Params P, float4* __restrict__ accum,
                                                         float4* __restrict__ guide_albedo,
@@ -228,53 +299,94 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main(DevScene S, Params P, fl
     const Ctx C{S, &P};
     const uint32_t index = y * P.width + x;
     auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, index, slots);
-
-    V3 sum = {0, 0, 0};
-    float sum_w = 0.0f;
-    V3 sum_albedo = {0, 0, 0};
-    float4 sum_normal = make_float4(0, 0, 0, 0);
+    PixelSums sums;
+    sums.clear();
     for (uint32_t k = 0; k < P.samples_per_frame; k++) {
         Rng g;
-        g.state = pcg(index ^ pcg(P.sample_offset + k));
-        g.qmc = true;
-        g.seed = pcg(index + 0x9e3779b9u);
-        g.index = P.sample_offset + k;
-        g.dim = 0u;
-        float spectrum_u = (float)((pcg(index * 7919u + 1u) + (P.sample_offset + k) * 2654435769u) >> 8u) *
-                           (1.0f / 16777216.0f);
+        PathState ps;
         Guides gd;
-        gd.albedo = {0, 0, 0};
-        gd.normal = make_float4(0, 0, 0, 0);
-        V3 c;
         if (P.view == VIEW_COLOR) {
-            V2 j = g.rand2();
-            c = radiance<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, stack, {(float)x + j.x, (float)y + j.y}, gd,
-                                                            spectrum_u);
-            float peak = fmaxf(c.x, fmaxf(c.y, c.z));
-            if (P.firefly_clamp > 0.0f && peak > P.firefly_clamp) c = c * (P.firefly_clamp / peak);
+            path_begin(P, g, ps, gd, x, y, index, P.sample_offset + k);
+            while (!path_step<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, stack, ps, gd)) {
+            }
+            sums.add(P, ps.result, gd, true);
         } else {
-            c = inspect<TEXTURES, ENV>(C, g, stack, {(float)x + 0.5f, (float)y + 0.5f});
-        }
-        if (finite3(c) && finite3(gd.albedo) && finite3(v3(gd.normal.x, gd.normal.y, gd.normal.z)) &&
-            finite3(s3(gd.normal.w))) {
-            float l = luminance(c);
-            sum = sum + c;
-            sum_w += l * l;
-            sum_albedo = sum_albedo + gd.albedo;
-            sum_normal = make_float4(sum_normal.x + gd.normal.x, sum_normal.y + gd.normal.y,
-                                     sum_normal.z + gd.normal.z, sum_normal.w + gd.normal.w);
+            gd.albedo = {0, 0, 0};
+            gd.normal = make_float4(0, 0, 0, 0);
+            sums.add(P, inspect<TEXTURES, ENV>(C, g, stack, {(float)x + 0.5f, (float)y + 0.5f}), gd, false);
         }
     }
-    if (P.sample_offset == 0u) {
-        accum[index] = make_float4(sum.x, sum.y, sum.z, sum_w);
-        guide_albedo[index] = make_float4(sum_albedo.x, sum_albedo.y, sum_albedo.z, 0.0f);
-        guide_normal[index] = sum_normal;
-    } else {
-        float4 a = accum[index], ga = guide_albedo[index], gn = guide_normal[index];
-        accum[index] = make_float4(a.x + sum.x, a.y + sum.y, a.z + sum.z, a.w + sum_w);
-        guide_albedo[index] = make_float4(ga.x + sum_albedo.x, ga.y + sum_albedo.y, ga.z + sum_albedo.z, ga.w + 0.0f);
-        guide_normal[index] = make_float4(gn.x + sum_normal.x, gn.y + sum_normal.y, gn.z + sum_normal.z,
-                                          gn.w + sum_normal.w);
+    sums.store(P, index, accum, guide_albedo, guide_normal);
+}
+
+// The same samples with lanes kept busy: waves claim 8×8 pixel tiles from a
+// global counter (one atomic per 64 pixels), and a lane whose path ends
+// starts its pixel's next sample, or the next unclaimed pixel, while the rest
+// of the wave keeps bouncing. A pixel's samples stay in one lane, in order,
+// so every sum is bit-identical to k_main's; only the colour view runs here.
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, Params P, float4* __restrict__ accum,
+                                                                   float4* __restrict__ guide_albedo,
+                                                                   float4* __restrict__ guide_normal, uint32_t* spill,
+                                                                   uint32_t slots, uint32_t* counter) {
+    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    const Ctx C{S, &P};
+    const uint32_t gid = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, gid, slots);
+    const uint32_t lane = __lane_id();
+    const uint32_t tiles_x = (P.width + 7u) / 8u;
+    const uint32_t items = tiles_x * ((P.height + 7u) / 8u) * 64u;
+
+    bool alive = false;
+    uint32_t x = 0, y = 0, index = 0, k = 0;
+    Rng g;
+    PathState ps;
+    Guides gd;
+    PixelSums sums;
+    uint32_t next = 0, end = 0;  // the wave's claimed, unassigned items
+    bool exhausted = false;
+
+    for (;;) {
+        uint64_t need = __ballot(!alive);
+        while (need != 0 && !exhausted) {
+            if (next >= end) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(counter, 64u);
+                base = __shfl(base, 0);
+                if (base >= items) {
+                    exhausted = true;
+                    break;
+                }
+                next = base;
+                end = min(base + 64u, items);
+            }
+            uint32_t avail = end - next;
+            uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+            if (((need >> lane) & 1ull) && rank < avail) {
+                uint32_t item = next + rank, tile = item >> 6, l = item & 63u;
+                x = (tile % tiles_x) * 8u + (l & 7u);
+                y = (tile / tiles_x) * 8u + (l >> 3);
+                if (x < P.width && y < P.height) {
+                    alive = true;
+                    index = y * P.width + x;
+                    k = 0;
+                    sums.clear();
+                    path_begin(P, g, ps, gd, x, y, index, P.sample_offset);
+                }
+            }
+            next += min((uint32_t)__popcll(need), avail);
+            need = __ballot(!alive);
+        }
+        if (__ballot(alive) == 0) break;
+        if (alive && path_step<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, stack, ps, gd)) {
+            sums.add(P, ps.result, gd, true);
+            if (++k < P.samples_per_frame) {
+                path_begin(P, g, ps, gd, x, y, index, P.sample_offset + k);
+            } else {
+                sums.store(P, index, accum, guide_albedo, guide_normal);
+                alive = false;
+            }
+        }
     }
 }
 
@@ -462,8 +574,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Param
                 break;
             }
             float u = g.rand();
-            Mat dm = dispersed<DISPERSION>(m, wavelength, carried, u);
-            BsdfSample bs = sample_bsdf(g, dm, base, s, -d);
+            m.ior = dispersed_ior<DISPERSION>(m, wavelength, carried, u);
+            BsdfSample bs = sample_bsdf(g, m, base, s, -d);
             if (!bs.ok) break;
             carried = carried * bs.weight;
             bounces++;
@@ -757,9 +869,40 @@ namespace rm {
 
 template <bool V, bool T, bool E, bool D>
 static void launch_main_v(const LaunchMain& L, hipStream_t stream) {
+    if (L.persistent_blocks > 0 && L.P.view == VIEW_COLOR) {
+        hipLaunchKernelGGL((k_main_persistent<V, T, E, D>), dim3(L.persistent_blocks), dim3(BLOCK_THREADS), 0, stream,
+                           L.S, L.P, L.accum, L.galb, L.gnorm, L.spill, L.slots, L.counter);
+        return;
+    }
     dim3 grid((L.P.width + 15) / 16, (L.P.height + 15) / 16);
     hipLaunchKernelGGL((k_main<V, T, E, D>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum, L.galb,
                        L.gnorm, L.spill, L.slots);
+}
+
+template <bool V, bool T, bool E, bool D>
+static int resident_blocks_v() {
+    static int blocks = -1;
+    if (blocks < 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_main_persistent<V, T, E, D>, BLOCK_THREADS, 0) !=
+            hipSuccess)
+            n = 1;
+        blocks = n > 0 ? n : 1;
+    }
+    return blocks;
+}
+
+int main_resident_blocks_per_cu(const Features& f) {
+    int key = (f.volumes ? 8 : 0) | (f.textures ? 4 : 0) | (f.environment ? 2 : 0) | (f.dispersion ? 1 : 0);
+    switch (key) {
+    case 0: return resident_blocks_v<false, false, false, false>();
+    case 1: return resident_blocks_v<false, false, false, true>();
+    case 2: return resident_blocks_v<false, false, true, false>();
+    case 3: return resident_blocks_v<false, false, true, true>();
+    case 8: return resident_blocks_v<true, false, false, false>();
+    case 10: return resident_blocks_v<true, false, true, false>();
+    default: return resident_blocks_v<true, true, true, true>();
+    }
 }
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream) {

@@ -23,9 +23,13 @@ struct LaunchMain {
     float4* gnorm;
     uint32_t* spill;
     uint32_t slots;
+    uint32_t persistent_blocks;  // 0: one thread per pixel (k_main); else k_main_persistent's grid
+    uint32_t* counter;           // k_main_persistent's tile counter, zeroed before the launch
 };
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
+// How many 256-thread blocks of the persistent path kernel fit on one CU.
+int main_resident_blocks_per_cu(const Features& f);
 void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, uint32_t* spill,
                     uint32_t slots, uint2* counters, hipStream_t stream);
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,

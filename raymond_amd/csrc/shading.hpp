@@ -264,13 +264,15 @@ RM_D bool in_sun(const Params& P, V3 d) { return P.sun_on != 0u && dot(d, pv3(P.
 // ------------------------------------------------------------------ materials
 template <bool TEXTURES>
 RM_D V3 albedo(const DevScene& S, const Mat& m, V3 p, V2 uv) {
-    V3 c = m.color;
+    // Component-wise selects: choosing between two struct members as a whole
+    // makes the compiler index the struct in private memory.
+    bool odd = false;
     if (m.checker_size > 0.0f) {
         V3 q = p / m.checker_size + s3(1e-3f);
         int32_t cx = to_i32(floorf(q.x)), cy = to_i32(floorf(q.y)), cz = to_i32(floorf(q.z));
-        int32_t odd = (int32_t)((uint32_t)cx + (uint32_t)cy + (uint32_t)cz) & 1;
-        c = odd != 0 ? m.checker : m.color;
+        odd = (((uint32_t)cx + (uint32_t)cy + (uint32_t)cz) & 1u) != 0u;
     }
+    V3 c = {odd ? m.checker.x : m.color.x, odd ? m.checker.y : m.color.y, odd ? m.checker.z : m.color.z};
     if (TEXTURES && m.texture != NONE && S.tex_layers > 0)
         c = c * tex_sample(S.textures, S.tex_w, S.tex_h, m.texture, {uv.x * m.tsx, uv.y * m.tsy}, true);
     return c;
@@ -540,18 +542,20 @@ RM_D V3 wavelength_rgb(float l) {
               0.0557f * x - 0.2040f * y + 1.0570f * z};
     return vmax(rgb, s3(0.0f)) * v3(1.92985f, 2.94689f, 3.11081f);
 }
+// The index of refraction a path sees at glass: its own, or at the path's
+// wavelength for glass that splits colours, choosing the wavelength (and
+// weighting the throughput by its colour) at the first such glass.
 template <bool DISPERSION>
-RM_D Mat dispersed(const Mat& m, float& wavelength, V3& throughput, float u) {
-    Mat sm = m;
+RM_D float dispersed_ior(const Mat& m, float& wavelength, V3& throughput, float u) {
     if (DISPERSION && m.kind == GLASS && m.dispersion > 0.0f) {
         if (wavelength == 0.0f) {
             wavelength = 380.0f + 340.0f * u;
             throughput = throughput * wavelength_rgb(wavelength);
         }
         float l = wavelength * 1e-3f;
-        sm.ior = m.ior + m.dispersion * (1.0f / (l * l) - INV_D_LINE2);
+        return m.ior + m.dispersion * (1.0f / (l * l) - INV_D_LINE2);
     }
-    return sm;
+    return m.ior;
 }
 
 // ------------------------------------------------------------------ camera

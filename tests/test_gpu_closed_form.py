@@ -34,9 +34,10 @@ def centre_hits(p):
     o = Oracle(p)
     rays = o.camera_rays()
     h = o.trace64(rays)
-    obj = p.extra["object_of"]
+    obj = np.asarray(p.extra["object_of"], np.int64)
     prim = h[:, 1].astype(np.int64)
-    owner = np.where(prim >= 0, obj[np.maximum(prim, 0)], -1)
+    owner = np.full(prim.shape, -1, np.int64)
+    owner[prim >= 0] = obj[prim[prim >= 0]]
     points = rays[:, 0:3].astype(np.float64) + rays[:, 4:7] * h[:, 0:1]
     return owner.reshape(p.height, p.width), points.reshape(p.height, p.width, 3), rays
 

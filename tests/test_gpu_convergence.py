@@ -1,12 +1,16 @@
 """Converged accuracy of the plain path: every pixel's mean after 1024 spp
 against a 65,536-spp reference rendered from a disjoint range of sample
 indices (so the two share no Sobol points), on the 200-sphere scene's camera
-at 160 × 90 with light tracing off. A pixel passes when its luminance is within
-3σ of the reference, σ from the 1024-spp render's own second moment (the
-accumulation's w holds Σ luminance²). Under a normal model 0.27% of pixels fall
-outside 3σ by chance; heavy-tailed paths (caustics through glass) push that up,
-so the bound asserted is 1%, and the image's relMSE (tools/converge.mjs:51-61)
-must be at most 1e-3."""
+at 160 × 90 with light tracing off and the sun off: the 0.5° sun's radiance is
+about 5·10⁴, and its caustics through the glass balls are rare, huge samples
+that path tracing alone cannot converge at 1024 spp (with light tracing on,
+as the bench renders, those paths come from splats instead). A pixel passes when its luminance is within
+3σ of the reference, where σ² is the per-sample luminance variance over 1024
+(the accumulation's w holds Σ luminance²), estimated from the reference's
+64k samples, which see the rare bright paths a 1024-sample estimate misses.
+Under a normal model 0.27% of pixels fall outside 3σ by chance; the bound
+asserted is 1%, and the image's relMSE (tools/converge.mjs:51-61) must be at
+most 1e-3."""
 import numpy as np
 import pytest
 
@@ -26,7 +30,7 @@ def render(ctx, p, spp, per=64):
         ctx.upload_environment(p)
         ctx.resize(p.width, p.height)
         for k in range(spp // per):
-            ctx.frame(scene, set_params(p.params.copy(), caustics=0, denoise=0, firefly_clamp=0.0,
+            ctx.frame(scene, set_params(p.params.copy(), caustics=0, denoise=0, firefly_clamp=0.0, sun_on=0,
                                         samples_per_frame=per, sample_offset=k * per, total_samples=(k + 1) * per))
         return ctx.read(capi.BUFFER_ACCUM).astype(np.float64) / spp
     finally:
@@ -42,10 +46,9 @@ def test_converged_means_within_three_sigma(ctx, pack):
     ref_disjoint = (ref * 65536 - test * 1024) / (65536 - 1024)
     lum_w = np.array([0.2126, 0.7152, 0.0722])
     mean = test[..., :3] @ lum_w
-    second = test[..., 3]
-    var = np.maximum(second - mean * mean, 0.0) * 1024 / 1023
-    sigma = np.sqrt(var / 1024)
     ref_l = ref_disjoint[..., :3] @ lum_w
+    var = np.maximum(ref_disjoint[..., 3] - ref_l * ref_l, 0.0)
+    sigma = np.sqrt(var / 1024)
     outside = np.abs(mean - ref_l) > 3 * sigma + 1e-6
     frac = outside.mean()
     err = rel_mse(test.astype(np.float32), ref_disjoint.astype(np.float32))

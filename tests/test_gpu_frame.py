@@ -80,3 +80,27 @@ def test_firefly_clamp(ctx, pack):
     assert g[..., :3].max() <= 1.5 * (1 + 1e-6)
     c = oracle_render(p, 1, params)[0]
     assert agreement(g, c) > 0.97
+
+
+@pytest.mark.parametrize("config,opts,spp", [(CASES[0][0], CASES[0][1], 1), (CASES[0][0], CASES[0][1], 3),
+                                              (CASES[3][0], CASES[3][1], 2), (CASES[4][0], CASES[4][1], 1),
+                                              (CASES[5][0], CASES[5][1], 2)],
+                         ids=["spheres", "spheres-3spp", "triangles-fog-env", "dispersive", "textured"])
+def test_persistent_schedule_is_bit_identical(ctx, pack, config, opts, spp):
+    """The lane-regenerating schedule computes the same samples in the same
+    per-pixel order as one thread per pixel: accumulation and guides equal bit for bit."""
+    from raymond_amd import capi
+    p = pack(config, **opts)
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0, samples_per_frame=spp)
+    try:
+        ctx.set_option(capi.OPTION_PERSISTENT, 0)
+        plain = gpu_render(ctx, p, 3, params)
+        ctx.set_option(capi.OPTION_PERSISTENT, 1)
+        for blocks in (0, 7):
+            ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, blocks)
+            fast = gpu_render(ctx, p, 3, params)
+            for a, b in zip(plain, fast):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    finally:
+        ctx.set_option(capi.OPTION_PERSISTENT, 1)
+        ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
