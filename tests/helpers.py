@@ -70,3 +70,20 @@ def rel_mse(a, r) -> float:
     a = a[..., :3].astype(np.float64)
     r = r[..., :3].astype(np.float64)
     return float(np.mean((a - r) ** 2 / (r * r + 1e-2)))
+
+
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+def load_golden(name: str):
+    """A committed fixture: (Packed scene, rays, expected closest, any-hit, counters).
+    The environment is replaced by a 1 × 1 black texel: traversal never reads it."""
+    import json
+    from raymond_amd.scenefile import Packed
+    z = np.load(GOLDEN / name)
+    meta = json.loads(bytes(z["meta"]).decode())
+    meta.update(envWidth=1, envHeight=1, envTableWidth=1, envTableHeight=0)
+    p = Packed(meta=meta, primitives=z["primitives"], nodes=z["nodes"], materials=z["materials"],
+               attributes=z["attributes"], lights=z["lights"], params=z["params"], emitters=None,
+               env_rgba=np.zeros(4, np.uint16), env_table=np.zeros(1, np.float32))
+    return p, z["rays"], z["closest"], z["any_hit"], z["counters"]
