@@ -87,7 +87,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--stride", type=int, default=31, help="pixel subsample stride for algorithmic bytes")
-    ap.add_argument("--schedule", choices=["persistent", "per-pixel"], default="persistent",
+    ap.add_argument("--schedule", choices=["per-pixel", "persistent"], default="per-pixel",
                     help="path-pass schedule: lanes regenerate paths from a tile queue, or one thread per pixel")
     ap.add_argument("--persistent-blocks", type=int, default=0, help="blocks for the persistent pass (0: fill the GPU)")
     args = ap.parse_args()

@@ -48,8 +48,8 @@ extern "C" {
 #define RM_STAGE_COUNT 9
 
 /* Options for rm_set_option. */
-#define RM_OPTION_PERSISTENT 0         /* 1 (default): the path pass keeps lanes busy by regenerating finished
-                                          paths from a tile queue; 0: one thread per pixel. Same results. */
+#define RM_OPTION_PERSISTENT 0         /* 0 (default): one thread per pixel; 1: persistent lanes that regenerate
+                                          finished paths from a tile queue. Same results, bit for bit. */
 #define RM_OPTION_PERSISTENT_BLOCKS 1  /* blocks of 256 threads for the persistent pass; 0 (default): fill the GPU */
 #define RM_OPTION_COUNT 2
 

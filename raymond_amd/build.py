@@ -26,11 +26,15 @@ ARCH = os.environ.get("RAYMOND_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 # FP32 exactly as written: no FMA contraction, IEEE division and square root,
-# and no packed-FP32 instructions.
+# and no packed-FP32 instructions. No SLP vectorisation either: with it, the
+# persistent path kernel's guide normal at light hits came out as the value
+# from before the assignment, although the optimised IR selects the new one
+# (a <4 x float> phi across a divergent switch, lowered wrongly for gfx950);
+# without it the kernel matches the one-thread-per-pixel kernel bit for bit.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-fno-slp-vectorize",
     f"-I{ROOT / 'include'}", "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["kernels.hip", "api.hip"]

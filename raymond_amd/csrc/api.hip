@@ -77,7 +77,7 @@ struct rm_context {
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
     DevBuf rays, hits, counters, work;
     int compute_units = 0;
-    int64_t options[RM_OPTION_COUNT] = {1, 0};
+    int64_t options[RM_OPTION_COUNT] = {0, 0};
     bool timing = false;
     StageTimer timer;
     std::vector<rm_scene*> scenes;

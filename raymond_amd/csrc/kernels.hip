@@ -22,6 +22,35 @@ RM_D void tile_pixel(uint32_t& x, uint32_t& y) {
     y = blockIdx.y * 16u + (w >> 1) * 8u + (l >> 3);
 }
 
+// Stencil passes (the denoiser) read a neighbourhood up to 32 pixels away.
+// Workgroups are dealt to the 8 XCDs round-robin by linear block id, and each
+// XCD has its own L2, so a plain row-major grid makes every L2 fetch nearly
+// the whole image. Instead the 16×16 tiles are ordered in vertical strips
+// STRIP_TILES wide (row-major inside a strip), and each XCD takes one
+// contiguous run of that order: its in-flight blocks then share a compact
+// footprint that fits its L2. Launch with stencil_blocks() blocks.
+constexpr uint32_t XCDS = 8u;
+constexpr uint32_t STRIP_TILES = 16u;
+__host__ __device__ inline uint32_t stencil_blocks(uint32_t width, uint32_t height) {
+    uint32_t tiles = ((width + 15u) / 16u) * ((height + 15u) / 16u);
+    return (tiles + XCDS - 1u) / XCDS * XCDS;
+}
+RM_D bool stencil_pixel(uint32_t width, uint32_t height, uint32_t& x, uint32_t& y) {
+    const uint32_t tiles_x = (width + 15u) / 16u, tiles_y = (height + 15u) / 16u;
+    const uint32_t tiles = tiles_x * tiles_y;
+    const uint32_t per_xcd = (tiles + XCDS - 1u) / XCDS;
+    const uint32_t t = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+    if (t >= tiles) return false;
+    const uint32_t strip = t / (STRIP_TILES * tiles_y);
+    const uint32_t r = t - strip * STRIP_TILES * tiles_y;
+    const uint32_t w = min(STRIP_TILES, tiles_x - strip * STRIP_TILES);
+    const uint32_t ty = r / w, tx = strip * STRIP_TILES + (r - ty * w);
+    const uint32_t th = threadIdx.x, wave = th >> 6, l = th & 63u;
+    x = tx * 16u + (wave & 1u) * 8u + (l & 7u);
+    y = ty * 16u + (wave >> 1) * 8u + (l >> 3);
+    return x < width && y < height;
+}
+
 template <int LDS_DEPTH>
 struct BlockStack {
     using S = Stack<LDS_DEPTH, BLOCK_THREADS>;
@@ -57,6 +86,9 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_raycast(DevScene S, const flo
     }
     hits[i] = make_uint4(__float_as_uint(h.t), h.prim, __float_as_uint(h.u), __float_as_uint(h.v));
 }
+
+// Items (pixels) a wave of the persistent path pass claims per atomic.
+constexpr uint32_t CLAIM_ITEMS = 256u;
 
 // ------------------------------------------------------------------ radiance
 struct Guides { V3 albedo; float4 normal; };
@@ -99,18 +131,38 @@ RM_D void path_begin(const Params& P, Rng& g, PathState& ps, Guides& gd, uint32_
     ps.depth = 0u;
 }
 
+// A next-event shadow ray waiting to be traced, and what it adds to the
+// path's radiance when nothing blocks it: vec · (phase · T · mis / pdf) in a
+// medium, vec · (T · mis / pdf) at a surface, T being the volumes'
+// transmittance along the ray (1 without volumes). The factors are radiance()'s
+// own, in its order (src/shaders.mjs:1247-1262, 1311-1326), split at the trace.
+struct PendingNee {
+    V3 o, d, vec;
+    float reach, phase, mis, pdf;
+    bool active, medium;
+};
+
+template <bool VOLUMES>
+RM_D void apply_nee(const Ctx& C, PathState& ps, const PendingNee& pn) {
+    float tr = VOLUMES ? transmittance(C, pn.o, pn.d, pn.reach) : 1.0f;
+    float k = pn.medium ? pn.phase * tr * pn.mis / pn.pdf : tr * pn.mis / pn.pdf;
+    ps.result = ps.result + pn.vec * k;
+}
+
 // One pass of radiance()'s bounce loop (src/shaders.mjs:1206-1356) at
-// ps.depth. Returns true when the path has ended (the loop's break, or its
-// last bounce); ps.result then holds the sample.
-template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, class STACK>
-RM_D bool path_step(const Ctx& C, Rng& g, STACK& stack, PathState& ps, Guides& gd) {
+// ps.depth, after its closest hit `hit` has been traced: the next-event
+// shadow ray, if any, is left in pn for the caller to trace (and apply_nee)
+// before the next bounce. Returns true when the path has ended (the loop's
+// break, or its last bounce); ps.result then holds the sample once pn is
+// applied.
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+RM_D bool path_shade(const Ctx& C, Rng& g, PathState& ps, Guides& gd, const Hit& hit, PendingNee& pn) {
     const Params& P = *C.P;
     const DevScene& S = C.S;
-    NoCount nc;
     const uint32_t choices = light_choices<ENV>(C);
     const uint32_t depth = ps.depth;
     g.dim = (depth + 1u) << 16u;
-    Hit hit = trace<false>(S, ps.o, ps.d, INF, stack, nc);
+    pn.active = false;
 
     if (VOLUMES && P.volume_count > 0u) {
         Scatter sc = scatter_in_volumes(C, g, ps.o, ps.d, hit.t);
@@ -129,11 +181,15 @@ RM_D bool path_step(const Ctx& C, Rng& g, STACK& stack, PathState& ps, Guides& g
                 LightSample ls = sample_light<ENV, TEXTURES>(C, g, p);
                 if (ls.pdf > 0.0f) {
                     float phase = henyey_greenstein(dot(ps.d, ls.wi), vm.anisotropy);
-                    float reach = ls.dist * (float)(1.0 - 1e-3);
-                    if (trace<true>(S, p, ls.wi, reach, stack, nc).prim == NONE)
-                        ps.result = ps.result + ps.throughput * ls.radiance *
-                                                    (phase * transmittance(C, p, ls.wi, reach) *
-                                                     power_heuristic(ls.pdf, phase) / ls.pdf);
+                    pn.o = p;
+                    pn.d = ls.wi;
+                    pn.reach = ls.dist * (float)(1.0 - 1e-3);
+                    pn.vec = ps.throughput * ls.radiance;
+                    pn.phase = phase;
+                    pn.mis = power_heuristic(ls.pdf, phase);
+                    pn.pdf = ls.pdf;
+                    pn.medium = true;
+                    pn.active = true;
                 }
             }
             V3 wi = sample_henyey_greenstein(g, ps.d, vm.anisotropy);
@@ -199,10 +255,16 @@ RM_D bool path_step(const Ctx& C, Rng& g, STACK& stack, PathState& ps, Guides& g
         if (ls.pdf > 0.0f && dot(ls.wi, s.ng) > 0.0f) {
             Eval e = eval_bsdf(m, base, s.n, wo, ls.wi);
             V3 origin = offset(s, ls.wi);
-            float reach = (ls.dist - dot(origin - s.p, ls.wi)) * (float)(1.0 - 1e-3);
-            if (e.pdf > 0.0f && trace<true>(S, origin, ls.wi, reach, stack, nc).prim == NONE) {
-                float tr = VOLUMES ? transmittance(C, origin, ls.wi, reach) : 1.0f;
-                ps.result = ps.result + ps.throughput * e.f * ls.radiance * (tr * power_heuristic(ls.pdf, e.pdf) / ls.pdf);
+            if (e.pdf > 0.0f) {
+                pn.o = origin;
+                pn.d = ls.wi;
+                pn.reach = (ls.dist - dot(origin - s.p, ls.wi)) * (float)(1.0 - 1e-3);
+                pn.vec = ps.throughput * e.f * ls.radiance;
+                pn.phase = 1.0f;
+                pn.mis = power_heuristic(ls.pdf, e.pdf);
+                pn.pdf = ls.pdf;
+                pn.medium = false;
+                pn.active = true;
             }
         }
     }
@@ -219,6 +281,18 @@ RM_D bool path_step(const Ctx& C, Rng& g, STACK& stack, PathState& ps, Guides& g
     if (!roulette(g, ps.throughput, depth)) return true;
     ps.depth = depth + 1u;
     return ps.depth > P.max_bounces;
+}
+
+// One bounce with its two traces in sequence: the closest hit, the shading,
+// then the shadow ray. Returns true when the path has ended.
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, class STACK>
+RM_D bool path_step(const Ctx& C, Rng& g, STACK& stack, PathState& ps, Guides& gd) {
+    NoCount nc;
+    Hit hit = trace<false>(C.S, ps.o, ps.d, INF, stack, nc);
+    PendingNee pn;
+    bool ended = path_shade<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, ps, gd, hit, pn);
+    if (pn.active && trace<true>(C.S, pn.o, pn.d, pn.reach, stack, nc).prim == NONE) apply_nee<VOLUMES>(C, ps, pn);
+    return ended;
 }
 
 // The albedo, normal or depth seen through a pixel centre (src/shaders.mjs:1377-1407).
@@ -320,10 +394,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main(DevScene S, Params P, fl
 }
 
 // The same samples with lanes kept busy: waves claim 8×8 pixel tiles from a
-// global counter (one atomic per 64 pixels), and a lane whose path ends
-// starts its pixel's next sample, or the next unclaimed pixel, while the rest
-// of the wave keeps bouncing. A pixel's samples stay in one lane, in order,
-// so every sum is bit-identical to k_main's; only the colour view runs here.
+// global counter (one atomic per CLAIM_ITEMS pixels), and a lane whose path
+// ends starts its pixel's next sample, or the next unclaimed pixel, while the
+// rest of the wave keeps bouncing. A pixel's samples stay in one lane, in
+// order, so every sum is bit-identical to k_main's; only the colour view runs
+// here. Regeneration mixes fresh primary rays with deep secondary ones in a
+// wave, which costs the primary rays' coherence: at one sample per pixel per
+// frame, k_main is the faster schedule.
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, Params P, float4* __restrict__ accum,
                                                                    float4* __restrict__ guide_albedo,
@@ -339,10 +416,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, P
 
     bool alive = false;
     uint32_t x = 0, y = 0, index = 0, k = 0;
-    Rng g;
-    PathState ps;
-    Guides gd;
+    Rng g = {};
+    PathState ps = {};
+    Guides gd = {};
     PixelSums sums;
+    sums.clear();
     uint32_t next = 0, end = 0;  // the wave's claimed, unassigned items
     bool exhausted = false;
 
@@ -351,14 +429,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, P
         while (need != 0 && !exhausted) {
             if (next >= end) {
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(counter, 64u);
+                if (lane == 0) base = atomicAdd(counter, CLAIM_ITEMS);
                 base = __shfl(base, 0);
                 if (base >= items) {
                     exhausted = true;
                     break;
                 }
                 next = base;
-                end = min(base + 64u, items);
+                end = min(base + CLAIM_ITEMS, items);
             }
             uint32_t avail = end - next;
             uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
@@ -742,8 +820,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
                                                            const float4* __restrict__ galb,
                                                            const float4* __restrict__ guides, float4* light_out) {
     uint32_t x, y;
-    tile_pixel(x, y);
-    if (x >= P.width || y >= P.height) return;
+    if (!stencil_pixel(P.width, P.height, x, y)) return;
     const uint32_t i = y * P.width + x;
     const float n = (float)max(P.total_samples, 1u);
     float4 ac = accum[i];
@@ -784,8 +861,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, co
                                                           const float4* __restrict__ light_in,
                                                           float4* __restrict__ light_out) {
     uint32_t x, y;
-    tile_pixel(x, y);
-    if (x >= P.width || y >= P.height) return;
+    if (!stencil_pixel(P.width, P.height, x, y)) return;
     const uint32_t i = y * P.width + x;
     const float4 centre = light_in[i];
     const float4 gp = guides[i];
@@ -969,12 +1045,12 @@ void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStre
 
 void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* guides, float4* out,
                     hipStream_t stream) {
-    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+    dim3 grid(stencil_blocks(P.width, P.height));
     hipLaunchKernelGGL(k_prepare, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb, guides, out);
 }
 
 void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream) {
-    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
+    dim3 grid(stencil_blocks(P.width, P.height));
     hipLaunchKernelGGL(k_atrous, grid, dim3(BLOCK_THREADS), 0, stream, P, step, guides, in, out);
 }
 

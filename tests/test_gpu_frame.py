@@ -102,5 +102,5 @@ def test_persistent_schedule_is_bit_identical(ctx, pack, config, opts, spp):
             for a, b in zip(plain, fast):
                 assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     finally:
-        ctx.set_option(capi.OPTION_PERSISTENT, 1)
+        ctx.set_option(capi.OPTION_PERSISTENT, 0)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)

@@ -146,6 +146,20 @@ def test_sobol2_is_linear_and_a_net():
             assert len(cells) == 1 << m, (m, k)
 
 
+def test_sobol2_superset_transform():
+    """The kernels compute sobol2 as a superset XOR transform over bit
+    positions followed by a bit reversal (the generator matrix is Pascal's
+    triangle mod 2); it equals the oracle's bit walk."""
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.integers(0, 1 << 32, 1 << 14, dtype=np.uint64),
+                         np.array([0, 1, 2, 3, 0x80000000, 0xFFFFFFFF, 0x55555555, 0xAAAAAAAA], np.uint64)])
+    w = xs.copy()
+    for shift, mask in [(1, 0x55555555), (2, 0x33333333), (4, 0x0F0F0F0F), (8, 0x00FF00FF), (16, 0x0000FFFF)]:
+        w ^= (w >> np.uint64(shift)) & np.uint64(mask)
+    for x, v in zip(xs.tolist(), w.tolist()):
+        assert pyoracle.sobol2(x) == int(f"{v:032b}"[::-1], 2), hex(x)
+
+
 def test_owen_scrambled_pairs_keep_the_net():
     """A pixel's first 256 rand2() pairs in one dimension stay a (0, 8, 2)-net
     after Owen scrambling (src/shaders.mjs:161-212)."""
