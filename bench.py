@@ -87,8 +87,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--stride", type=int, default=31, help="pixel subsample stride for algorithmic bytes")
-    ap.add_argument("--schedule", choices=["per-pixel", "persistent"], default="per-pixel",
-                    help="path-pass schedule: lanes regenerate paths from a tile queue, or one thread per pixel")
+    ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent"], default="per-pixel",
+                    help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
+                         "or lanes regenerating paths from a tile queue")
     ap.add_argument("--persistent-blocks", type=int, default=0, help="blocks for the persistent pass (0: fill the GPU)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -110,7 +111,7 @@ def main() -> int:
 
     ctx = capi.Context(0)
     device, cus = ctx.device_info()
-    ctx.set_option(capi.OPTION_PERSISTENT, 1 if args.schedule == "persistent" else 0)
+    ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULES[args.schedule])
     ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, args.persistent_blocks)
     scene = ctx.upload_scene(p)
     ctx.environment(p.env_rgba, p.meta["envWidth"], p.meta["envHeight"], p.env_table, p.meta["envTableWidth"],
@@ -166,7 +167,7 @@ def main() -> int:
                    "path": "plain (one job, megakernel, FP32 BVH)", "schedule": args.schedule},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_main_persistent" if args.schedule == "persistent" else "k_main", "launch_ms": round(trace_avg_s * 1e3, 4),
+                     "kernel": {"compact": "k_main_compact", "per-pixel": "k_main", "persistent": "k_main_persistent"}[args.schedule], "launch_ms": round(trace_avg_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": round(bytes_per_launch),
                      "bytes_per_sample": round(alg["per_sample"], 1),
                      "rays_per_sample": round(alg["rays_per_sample"], 3)},

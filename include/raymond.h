@@ -48,10 +48,14 @@ extern "C" {
 #define RM_STAGE_COUNT 9
 
 /* Options for rm_set_option. */
-#define RM_OPTION_PERSISTENT 0         /* 0 (default): one thread per pixel; 1: persistent lanes that regenerate
-                                          finished paths from a tile queue. Same results, bit for bit. */
-#define RM_OPTION_PERSISTENT_BLOCKS 1  /* blocks of 256 threads for the persistent pass; 0 (default): fill the GPU */
+#define RM_OPTION_SCHEDULE 0           /* how the path-tracing pass maps paths to lanes, RM_SCHEDULE_*;
+                                          every schedule gives the same results, bit for bit */
+#define RM_OPTION_PERSISTENT_BLOCKS 1  /* blocks of 256 threads for RM_SCHEDULE_PERSISTENT; 0 (default): fill the GPU */
 #define RM_OPTION_COUNT 2
+
+#define RM_SCHEDULE_PIXEL 0            /* default: one thread per pixel, each path run to its end */
+#define RM_SCHEDULE_COMPACT 1          /* a block's live paths are packed into its first waves between bounces */
+#define RM_SCHEDULE_PERSISTENT 2       /* persistent lanes that regenerate finished paths from a tile queue */
 
 typedef struct rm_context rm_context;
 typedef struct rm_scene rm_scene;

@@ -39,7 +39,9 @@ SIGNATURES = {
     "rm_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
     "rm_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
-OPTION_PERSISTENT, OPTION_PERSISTENT_BLOCKS = 0, 1
+OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS = 0, 1
+SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT = 0, 1, 2
+SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT}
 
 _lib = None
 

@@ -75,7 +75,7 @@ struct rm_context {
     DevBuf heads, photons, emitters, spill;
     DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
-    DevBuf rays, hits, counters, work;
+    DevBuf rays, hits, counters, work, staging;
     int compute_units = 0;
     int64_t options[RM_OPTION_COUNT] = {0, 0};
     bool timing = false;
@@ -212,8 +212,8 @@ int rm_open(int device, rm_context** out) {
 int rm_set_option(rm_context* c, int option, int64_t value) {
     if (!c) return fail(RM_ERR_INVALID, "rm_set_option needs a context.");
     if (option < 0 || option >= RM_OPTION_COUNT) return fail(RM_ERR_INVALID, "Unknown option " + std::to_string(option) + ".");
-    if (option == RM_OPTION_PERSISTENT && value != 0 && value != 1)
-        return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT is 0 or 1.");
+    if (option == RM_OPTION_SCHEDULE && (value < RM_SCHEDULE_PIXEL || value > RM_SCHEDULE_PERSISTENT))
+        return fail(RM_ERR_INVALID, "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT or _PERSISTENT.");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
     c->options[option] = value;
@@ -229,7 +229,7 @@ int rm_close(rm_context* c) {
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->heads,
-                      &c->photons, &c->emitters, &c->spill, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work})
+                      &c->photons, &c->emitters, &c->spill, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
@@ -398,7 +398,9 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
                       s->dispersive};
     uint32_t slots = c->width * c->height;
     uint32_t persistent = 0;
-    if (c->options[RM_OPTION_PERSISTENT] && P.view == 0u) {
+    float4* staging = nullptr;
+    const int64_t schedule = c->options[RM_OPTION_SCHEDULE];
+    if (schedule == RM_SCHEDULE_PERSISTENT && P.view == 0u) {
         int64_t blocks = c->options[RM_OPTION_PERSISTENT_BLOCKS];
         if (blocks == 0) blocks = (int64_t)(c->compute_units > 0 ? c->compute_units : 1) * rm::main_resident_blocks_per_cu(f);
         persistent = (uint32_t)blocks;
@@ -407,11 +409,18 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         if (r != RM_OK) return r;
         slots = threads;
         rm::launch_fill_u32(c->work.as<uint32_t>(), 0u, 1, c->stream);
+    } else if (schedule == RM_SCHEDULE_COMPACT && P.view == 0u) {
+        uint32_t threads = ((c->width + 15u) / 16u) * ((c->height + 15u) / 16u) * rm::BLOCK_THREADS;
+        int r = ensure_spill(c, threads);
+        if (r != RM_OK) return r;
+        RM_HIP(c->staging.ensure((size_t)threads * 48), "allocating the path staging records");
+        staging = c->staging.as<float4>();
+        slots = threads;
     }
     {
         Timed t(c, RM_STAGE_TRACE);
         rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(),
-                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>()};
+                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging};
         rm::launch_main(L, f, c->stream);
     }
     slots = c->width * c->height;

@@ -16,11 +16,12 @@ namespace rm {
 // ------------------------------------------------------------------ thread mapping
 // A 256-thread block covers a 16×16 pixel tile as four 8×8 wave tiles, so a
 // wave's primary rays are spatially coherent.
-RM_D void tile_pixel(uint32_t& x, uint32_t& y) {
-    uint32_t t = threadIdx.x, w = t >> 6, l = t & 63u;
+RM_D void tile_pixel_of(uint32_t t, uint32_t& x, uint32_t& y) {
+    uint32_t w = t >> 6, l = t & 63u;
     x = blockIdx.x * 16u + (w & 1u) * 8u + (l & 7u);
     y = blockIdx.y * 16u + (w >> 1) * 8u + (l >> 3);
 }
+RM_D void tile_pixel(uint32_t& x, uint32_t& y) { tile_pixel_of(threadIdx.x, x, y); }
 
 // Stencil passes (the denoiser) read a neighbourhood up to 32 pixels away.
 // Workgroups are dealt to the 8 XCDs round-robin by linear block id, and each
@@ -361,11 +362,16 @@ struct PixelSums {
 // ------------------------------------------------------------------ main
 // src/shaders.mjs:1838-1895, one thread per pixel: the frame's samples one
 // after another, each path run to its end.
+//
+// Four waves per SIMD: left to itself the compiler keeps every value of a
+// path in registers (153 VGPRs, three waves); capping it at 128 spills about
+// 112 bytes a lane of rarely touched state (the pixel's sums, the guides) and
+// the extra wave hides enough traversal latency to make the pass about 11%
+// faster on the 1080p sphere scene.
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_main(DevScene S, Params P, float4* __restrict__ accum,
-                                                        float4* __restrict__ guide_albedo,
-                                                        float4* __restrict__ guide_normal, uint32_t* spill,
-                                                        uint32_t slots) {
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main(
+    DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
+    float4* __restrict__ guide_normal, uint32_t* spill, uint32_t slots) {
     __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     uint32_t x, y;
     tile_pixel(x, y);
@@ -391,6 +397,148 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main(DevScene S, Params P, fl
         }
     }
     sums.store(P, index, accum, guide_albedo, guide_normal);
+}
+
+// The same samples with the block's live paths packed together between
+// bounces. A block's 256 paths start together (coherent primary rays); after
+// each bounce, when fewer waves could hold the paths still alive, their state
+// moves through LDS to the first lanes of the block, so emptied waves wait at
+// the barrier instead of running the bounce with a few live lanes. A path
+// that ends leaves its sample in a per-lane staging record; once every path of
+// the block has ended, each pixel's own thread adds its sample in order. Every
+// path runs exactly the same arithmetic as in k_main, so the sums are
+// bit-identical; only the colour view runs here. The barrier after every
+// bounce ties a block's waves to its slowest one, so with short paths (1 spp
+// per frame on the sphere scene) k_main is faster.
+constexpr int COMPACT_LDS_DEPTH = 8;
+constexpr int SLOT_WORDS = 27;
+
+struct SlotStore {
+    uint32_t* w;  // [SLOT_WORDS][BLOCK_THREADS]
+    RM_D void put(uint32_t slot, int k, float v) { w[k * BLOCK_THREADS + slot] = __float_as_uint(v); }
+    RM_D void putu(uint32_t slot, int k, uint32_t v) { w[k * BLOCK_THREADS + slot] = v; }
+    RM_D float get(uint32_t slot, int k) const { return __uint_as_float(w[k * BLOCK_THREADS + slot]); }
+    RM_D uint32_t getu(uint32_t slot, int k) const { return w[k * BLOCK_THREADS + slot]; }
+    RM_D void put3(uint32_t slot, int k, V3 v) { put(slot, k, v.x); put(slot, k + 1, v.y); put(slot, k + 2, v.z); }
+    RM_D V3 get3(uint32_t slot, int k) const { return {get(slot, k), get(slot, k + 1), get(slot, k + 2)}; }
+};
+
+// A live path's state between bounces. The RNG's seed, sample index and
+// spectrum offset are functions of the pixel and sample, and its dimension is
+// reset at every bounce, so they are recomputed rather than moved.
+RM_D void slot_save(SlotStore& st, uint32_t slot, const PathState& ps, const Guides& gd, uint32_t pix) {
+    st.put3(slot, 0, ps.o);
+    st.put3(slot, 3, ps.d);
+    st.put3(slot, 6, ps.throughput);
+    st.put3(slot, 9, ps.result);
+    st.put3(slot, 12, ps.last_p);
+    st.put(slot, 15, ps.last_pdf);
+    st.put(slot, 16, ps.travelled);
+    st.put(slot, 17, ps.wavelength);
+    st.putu(slot, 18, ps.depth | (ps.last_delta ? 1u << 24 : 0u) | (ps.guided ? 1u << 25 : 0u) |
+                          (ps.caustic ? 1u << 26 : 0u) | (ps.diffuse_seen ? 1u << 27 : 0u));
+    st.put3(slot, 19, gd.albedo);
+    st.put(slot, 22, gd.normal.x);
+    st.put(slot, 23, gd.normal.y);
+    st.put(slot, 24, gd.normal.z);
+    st.put(slot, 25, gd.normal.w);
+    st.putu(slot, 26, pix);
+}
+RM_D void slot_load(const SlotStore& st, uint32_t slot, const Params& P, uint32_t sample, PathState& ps, Guides& gd,
+                    Rng& g, uint32_t& pix) {
+    ps.o = st.get3(slot, 0);
+    ps.d = st.get3(slot, 3);
+    ps.throughput = st.get3(slot, 6);
+    ps.result = st.get3(slot, 9);
+    ps.last_p = st.get3(slot, 12);
+    ps.last_pdf = st.get(slot, 15);
+    ps.travelled = st.get(slot, 16);
+    ps.wavelength = st.get(slot, 17);
+    uint32_t bits = st.getu(slot, 18);
+    ps.depth = bits & 0xffffffu;
+    ps.last_delta = (bits >> 24) & 1u;
+    ps.guided = (bits >> 25) & 1u;
+    ps.caustic = (bits >> 26) & 1u;
+    ps.diffuse_seen = (bits >> 27) & 1u;
+    gd.albedo = st.get3(slot, 19);
+    gd.normal = make_float4(st.get(slot, 22), st.get(slot, 23), st.get(slot, 24), st.get(slot, 25));
+    pix = st.getu(slot, 26);
+    uint32_t x, y;
+    tile_pixel_of(pix, x, y);
+    const uint32_t index = y * P.width + x;
+    g.state = pcg(index ^ pcg(sample));
+    g.qmc = true;
+    g.seed = pcg(index + 0x9e3779b9u);
+    g.index = sample;
+    g.dim = 0u;
+    ps.spectrum_u = (float)((pcg(index * 7919u + 1u) + sample * 2654435769u) >> 8u) * (1.0f / 16777216.0f);
+}
+
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main_compact(
+    DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
+    float4* __restrict__ guide_normal, uint32_t* spill, uint32_t slots, float4* __restrict__ staging) {
+    __shared__ uint32_t lds[COMPACT_LDS_DEPTH * BLOCK_THREADS];
+    __shared__ uint32_t slot_words[SLOT_WORDS * BLOCK_THREADS];
+    __shared__ uint32_t wave_live[BLOCK_THREADS / 64];
+    const Ctx C{S, &P};
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t gid = (blockIdx.y * gridDim.x + blockIdx.x) * BLOCK_THREADS + t;
+    uint32_t x, y;
+    tile_pixel(x, y);
+    const bool valid = x < P.width && y < P.height;
+    const uint32_t index = valid ? y * P.width + x : 0u;
+    auto stack = BlockStack<COMPACT_LDS_DEPTH>::make(lds, spill, gid, slots);
+    SlotStore st{slot_words};
+    float4* const stage = staging + (size_t)(gid - t) * 3;  // this block's 256 records of 3 float4
+    PixelSums sums;
+    sums.clear();
+    for (uint32_t k = 0; k < P.samples_per_frame; k++) {
+        const uint32_t sample = P.sample_offset + k;
+        Rng g = {};
+        PathState ps = {};
+        Guides gd = {};
+        uint32_t pix = t;
+        bool live = valid;
+        if (live) path_begin(P, g, ps, gd, x, y, index, sample);
+            for (;;) {
+            if (live && path_step<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, stack, ps, gd)) {
+                stage[pix * 3] = make_float4(ps.result.x, ps.result.y, ps.result.z, 0.0f);
+                stage[pix * 3 + 1] = make_float4(gd.albedo.x, gd.albedo.y, gd.albedo.z, 0.0f);
+                stage[pix * 3 + 2] = gd.normal;
+                live = false;
+            }
+            const uint64_t mask = __ballot(live);
+            if (lane == 0) wave_live[wave] = (uint32_t)__popcll(mask);
+            __syncthreads();
+            uint32_t total = 0, before = 0, nonempty = 0;
+            for (uint32_t w = 0; w < BLOCK_THREADS / 64; w++) {
+                uint32_t n = wave_live[w];
+                before += w < wave ? n : 0u;
+                total += n;
+                nonempty += n > 0u ? 1u : 0u;
+            }
+            if (total == 0u) break;
+            if ((total + 63u) / 64u < nonempty) {
+                // Pack the live paths into lanes 0 .. total-1, in lane order.
+                if (live) slot_save(st, before + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)), ps, gd, pix);
+                __syncthreads();
+                live = t < total;
+                if (live) slot_load(st, t, P, sample, ps, gd, g, pix);
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        if (valid) {
+            float4 r = stage[t * 3], a = stage[t * 3 + 1];
+            Guides own;
+            own.albedo = {a.x, a.y, a.z};
+            own.normal = stage[t * 3 + 2];
+            sums.add(P, v3(r.x, r.y, r.z), own, true);
+        }
+        __syncthreads();
+    }
+    if (valid) sums.store(P, index, accum, guide_albedo, guide_normal);
 }
 
 // The same samples with lanes kept busy: waves claim 8×8 pixel tiles from a
@@ -951,6 +1099,11 @@ static void launch_main_v(const LaunchMain& L, hipStream_t stream) {
         return;
     }
     dim3 grid((L.P.width + 15) / 16, (L.P.height + 15) / 16);
+    if (L.staging && L.P.view == VIEW_COLOR) {
+        hipLaunchKernelGGL((k_main_compact<V, T, E, D>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
+                           L.galb, L.gnorm, L.spill, L.slots, L.staging);
+        return;
+    }
     hipLaunchKernelGGL((k_main<V, T, E, D>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum, L.galb,
                        L.gnorm, L.spill, L.slots);
 }

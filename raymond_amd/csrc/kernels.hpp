@@ -11,7 +11,9 @@ constexpr int BLOCK_THREADS = 256;
 // reference's 64 spill to global memory). 16 KiB of LDS per block each.
 constexpr int MAIN_LDS_DEPTH = 16;
 constexpr int TRACE_LDS_DEPTH = 16;
-constexpr uint32_t SPILL_ENTRIES = STACK_LIMIT - 16u;
+// Spill entries per lane: the rest of the 64 beyond the shallowest LDS stack
+// (the compacted path kernel keeps 8 entries in LDS).
+constexpr uint32_t SPILL_ENTRIES = STACK_LIMIT - 8u;
 
 struct Features { bool volumes, textures, environment, dispersion; };
 
@@ -23,8 +25,9 @@ struct LaunchMain {
     float4* gnorm;
     uint32_t* spill;
     uint32_t slots;
-    uint32_t persistent_blocks;  // 0: one thread per pixel (k_main); else k_main_persistent's grid
+    uint32_t persistent_blocks;  // > 0: k_main_persistent with this grid
     uint32_t* counter;           // k_main_persistent's tile counter, zeroed before the launch
+    float4* staging;             // non-null: k_main_compact, 3 float4 per thread of its grid
 };
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);

@@ -86,21 +86,24 @@ def test_firefly_clamp(ctx, pack):
                                               (CASES[3][0], CASES[3][1], 2), (CASES[4][0], CASES[4][1], 1),
                                               (CASES[5][0], CASES[5][1], 2)],
                          ids=["spheres", "spheres-3spp", "triangles-fog-env", "dispersive", "textured"])
-def test_persistent_schedule_is_bit_identical(ctx, pack, config, opts, spp):
-    """The lane-regenerating schedule computes the same samples in the same
-    per-pixel order as one thread per pixel: accumulation and guides equal bit for bit."""
+def test_schedules_are_bit_identical(ctx, pack, config, opts, spp):
+    """The compacting and lane-regenerating schedules compute the same samples,
+    each pixel's in the same order, as one thread per pixel: accumulation and
+    guides equal bit for bit."""
     from raymond_amd import capi
     p = pack(config, **opts)
     params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0, samples_per_frame=spp)
     try:
-        ctx.set_option(capi.OPTION_PERSISTENT, 0)
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         plain = gpu_render(ctx, p, 3, params)
-        ctx.set_option(capi.OPTION_PERSISTENT, 1)
-        for blocks in (0, 7):
+        runs = [(capi.SCHEDULE_COMPACT, 0), (capi.SCHEDULE_PERSISTENT, 0), (capi.SCHEDULE_PERSISTENT, 7)]
+        for schedule, blocks in runs:
+            ctx.set_option(capi.OPTION_SCHEDULE, schedule)
             ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, blocks)
-            fast = gpu_render(ctx, p, 3, params)
-            for a, b in zip(plain, fast):
-                assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+            other = gpu_render(ctx, p, 3, params)
+            for name, a, b in zip(["accum", "albedo", "normal"], plain, other):
+                differ = int(np.any(a.view(np.uint32) != b.view(np.uint32), axis=-1).sum())
+                assert differ == 0, f"schedule {schedule} (blocks {blocks}): {differ} pixels of {name} differ"
     finally:
-        ctx.set_option(capi.OPTION_PERSISTENT, 0)
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
