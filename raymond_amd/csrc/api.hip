@@ -70,9 +70,13 @@ struct rm_scene {
 struct rm_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Light tracing and photon gather run on `side`, alongside the path pass
+    // on `stream`; fork and join order them against the rest of the frame.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     uint32_t width = 0, height = 0, capacity = 0;
     DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats, guides;
-    DevBuf heads, photons, emitters, spill;
+    DevBuf heads, photons, emitters, spill, side_spill, gathered;
     DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
     DevBuf rays, hits, counters, work, staging;
@@ -100,17 +104,18 @@ hipEvent_t timer_event(rm_context* c) {
 struct Timed {
     rm_context* c;
     int stage;
+    hipStream_t stream;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(rm_context* ctx, int s) : c(ctx), stage(s) {
+    Timed(rm_context* ctx, int s, hipStream_t on = nullptr) : c(ctx), stage(s), stream(on ? on : ctx->stream) {
         if (c->timing) {
             a = timer_event(c);
             b = timer_event(c);
-            if (a && b) (void)hipEventRecord(a, c->stream);
+            if (a && b) (void)hipEventRecord(a, stream);
         }
     }
     ~Timed() {
         if (c->timing && a && b) {
-            (void)hipEventRecord(b, c->stream);
+            (void)hipEventRecord(b, stream);
             c->timer.pending[stage].push_back({a, b});
         }
     }
@@ -187,9 +192,16 @@ int rm_open(int device, rm_context** out) {
     rm_context* c = new rm_context();
     c->device = device;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->join) (void)hipEventDestroy(c->join);
+        if (c->fork) (void)hipEventDestroy(c->fork);
+        if (c->side) (void)hipStreamDestroy(c->side);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
-        return hip_fail(e, "creating a stream");
+        return hip_fail(e, "creating the streams");
     }
     // A 1×1 black environment and an empty table stand in until one is given.
     uint16_t black[4] = {0, 0, 0, 0x3c00};
@@ -224,12 +236,13 @@ int rm_close(rm_context* c) {
     if (!c) return fail(RM_ERR_INVALID, "rm_close needs a context.");
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (rm_scene* s : c->scenes) {
         for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights, &s->textures, &s->nmaps}) b->release();
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->heads,
-                      &c->photons, &c->emitters, &c->spill, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging})
+                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
@@ -237,6 +250,9 @@ int rm_close(rm_context* c) {
             (void)hipEventDestroy(p.second);
         }
     for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->side) (void)hipStreamDestroy(c->side);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return RM_OK;
@@ -396,7 +412,48 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
     rm::DevScene S = dev_scene(c, s);
     rm::Features f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
                       s->dispersive};
-    uint32_t slots = c->width * c->height;
+    // Light tracing and the photon gather depend on the scene and the
+    // parameters only, so they run on the side stream while the path pass
+    // runs; k_resolve then adds splats and gathered light, after the path
+    // pass, in the reference's order. The fork makes the side stream wait
+    // for everything queued before this frame (earlier frames' readers of
+    // the splats, photons and gathered light), the join orders k_resolve
+    // after it. The side stream keeps its own traversal spill.
+    const uint32_t px = c->width * c->height;
+    if (caustics) {
+        RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
+        RM_HIP(c->photons.ensure((size_t)rm::PHOTON_CAPACITY * 48), "allocating photons");
+        RM_HIP(c->gathered.ensure((size_t)px * 16), "allocating the gathered light");
+        int r = upload(c, c->emitters, emitters, (size_t)emitter_vec4s * 16, "uploading emitters");
+        if (r != RM_OK) return r;
+        uint32_t count = rm::PHOTON_CAPACITY;
+        if ((uint64_t)P.width * P.height * P.samples_per_frame < rm::PHOTON_CAPACITY)
+            count = P.width * P.height * P.samples_per_frame;
+        const uint32_t side_slots = count > px ? count : px;
+        RM_HIP(c->side_spill.ensure((size_t)side_slots * rm::SPILL_ENTRIES * sizeof(uint32_t)),
+               "allocating the side traversal spill");
+        RM_HIP(hipEventRecord(c->fork, c->stream), "forking the side stream");
+        RM_HIP(hipStreamWaitEvent(c->side, c->fork, 0), "forking the side stream");
+        {
+            Timed t(c, RM_STAGE_CLEAR, c->side);
+            rm::launch_fill_u32(c->heads.as<uint32_t>(), 0u, rm::PHOTON_TABLE + 1, c->side);
+        }
+        {
+            Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
+            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats.as<uint32_t>(),
+                                   c->heads.as<uint32_t>(), c->photons.as<float4>(), c->side_spill.as<uint32_t>(),
+                                   side_slots, count, f, c->side);
+        }
+        {
+            Timed t(c, RM_STAGE_GATHER, c->side);
+            rm::launch_photon_gather(S, P, c->heads.as<const uint32_t>(), c->photons.as<const float4>(),
+                                     c->gathered.as<float4>(), c->side_spill.as<uint32_t>(), side_slots, f, c->side);
+        }
+        RM_HIP(hipEventRecord(c->join, c->side), "joining the side stream");
+        RM_HIP(hipGetLastError(), "launching light tracing");
+    }
+
+    uint32_t slots = px;
     uint32_t persistent = 0;
     float4* staging = nullptr;
     const int64_t schedule = c->options[RM_OPTION_SCHEDULE];
@@ -423,39 +480,16 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
                          c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging};
         rm::launch_main(L, f, c->stream);
     }
-    slots = c->width * c->height;
     RM_HIP(hipGetLastError(), "launching the trace");
 
     if (caustics) {
-        RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
-        RM_HIP(c->photons.ensure((size_t)rm::PHOTON_CAPACITY * 48), "allocating photons");
-        int r = upload(c, c->emitters, emitters, (size_t)emitter_vec4s * 16, "uploading emitters");
-        if (r != RM_OK) return r;
-        uint32_t count = P.width * P.height * P.samples_per_frame;
-        if (count > rm::PHOTON_CAPACITY || (uint64_t)P.width * P.height * P.samples_per_frame > rm::PHOTON_CAPACITY)
-            count = rm::PHOTON_CAPACITY;
-        r = ensure_spill(c, count > slots ? count : slots);
-        if (r != RM_OK) return r;
-        {
-            Timed t(c, RM_STAGE_CLEAR);
-            rm::launch_fill_u32(c->heads.as<uint32_t>(), 0u, rm::PHOTON_TABLE + 1, c->stream);
-        }
-        {
-            Timed t(c, RM_STAGE_LIGHT_TRACE);
-            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats.as<uint32_t>(),
-                                   c->heads.as<uint32_t>(), c->photons.as<float4>(), c->spill.as<uint32_t>(),
-                                   count > slots ? count : slots, count, f, c->stream);
-        }
+        RM_HIP(hipStreamWaitEvent(c->stream, c->join, 0), "joining the side stream");
         {
             Timed t(c, RM_STAGE_RESOLVE);
-            rm::launch_resolve(P, c->splats.as<uint4>(), c->accum.as<float4>(), c->stream);
+            rm::launch_resolve(P, c->splats.as<uint4>(), c->gathered.as<const float4>(), c->accum.as<float4>(),
+                               c->stream);
         }
-        {
-            Timed t(c, RM_STAGE_GATHER);
-            rm::launch_photon_gather(S, P, c->heads.as<const uint32_t>(), c->photons.as<const float4>(),
-                                     c->accum.as<float4>(), c->spill.as<uint32_t>(), slots, f, c->stream);
-        }
-        RM_HIP(hipGetLastError(), "launching light tracing");
+        RM_HIP(hipGetLastError(), "launching the splat resolve");
     }
 
     if (P.denoise != 0u) {

@@ -830,8 +830,12 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Param
     photons[(size_t)i * 3 + 2] = make_float4(-dep_d.x, -dep_d.y, -dep_d.z, 0.0f);
 }
 
-// src/shaders.mjs:1819-1831
-__global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* splats, float4* accum) {
+// resolve_splats() then photon_gather()'s sum into the accumulation
+// (src/shaders.mjs:1819-1831, 1610-1624), in that order, for one pixel. The
+// gather pass leaves its light in `gathered` (w = 1 where it adds any), so it
+// can run while the path-tracing pass still writes the accumulation.
+__global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* splats, const float4* gathered,
+                                                           float4* accum) {
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     if (i >= P.width * P.height) return;
     uint4 s = splats[i];
@@ -840,7 +844,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* spla
     float l = luminance(light);
     float spp = (float)P.samples_per_frame;
     float4 a = accum[i];
-    accum[i] = make_float4(a.x + light.x * spp, a.y + light.y * spp, a.z + light.z * spp, a.w + l * l * spp);
+    a = make_float4(a.x + light.x * spp, a.y + light.y * spp, a.z + light.z * spp, a.w + l * l * spp);
+    float4 g = gathered[i];
+    if (g.w != 0.0f) {
+        V3 gl = v3(g.x, g.y, g.z);
+        float lg = luminance(gl);
+        a = make_float4(a.x + gl.x * spp, a.y + gl.y * spp, a.z + gl.z * spp, a.w + lg * lg * spp);
+    }
+    accum[i] = a;
 }
 
 // The photons within the radius of a surface, times the BSDF, over the
@@ -877,18 +888,13 @@ RM_D V3 gather(const Ctx& C, const uint32_t* heads, const float4* photons, const
     return sum / (PI * r2);
 }
 
-// src/shaders.mjs:1577-1626
+// src/shaders.mjs:1577-1626. The pixel's gathered light goes to
+// gathered[index] (w = 1), or zeros when it adds none; k_resolve adds it.
 template <bool TEXTURES>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Params P, const uint32_t* heads,
-                                                                 const float4* __restrict__ photons, float4* accum,
-                                                                 uint32_t* spill, uint32_t slots) {
-    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
-    uint32_t x, y;
-    tile_pixel(x, y);
-    if (x >= P.width || y >= P.height) return;
-    const Ctx C{S, &P};
-    const uint32_t index = y * P.width + x;
-    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, index, slots);
+RM_D float4 gather_pixel(const Ctx& C, const uint32_t* heads, const float4* photons, uint32_t x, uint32_t y,
+                         uint32_t index, Stack<MAIN_LDS_DEPTH, BLOCK_THREADS>& stack) {
+    const Params& P = *C.P;
+    const DevScene& S = C.S;
     NoCount nc;
     Rng g;
     g.qmc = false;
@@ -900,30 +906,40 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Par
     V3 throughput = {1, 1, 1};
     for (uint32_t depth = 0u; depth < P.max_bounces; depth++) {
         Hit hit = trace<false>(S, o, d, INF, stack, nc);
-        if (hit.prim == NONE) return;
+        if (hit.prim == NONE) break;
         Surface s = surface<TEXTURES>(S, o, d, hit);
         Mat m = load_mat(S, s.material);
-        if (m.kind == LIGHT) return;
+        if (m.kind == LIGHT) break;
         V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
         if (!is_delta(m)) {
             if (depth > 0u) {
                 V3 light = throughput * gather(C, heads, photons, s, m, base, -d);
-                if (finite3(light)) {
-                    float l = luminance(light);
-                    float spp = (float)P.samples_per_frame;
-                    float4 a = accum[index];
-                    accum[index] = make_float4(a.x + light.x * spp, a.y + light.y * spp, a.z + light.z * spp,
-                                               a.w + l * l * spp);
-                }
+                if (finite3(light)) return make_float4(light.x, light.y, light.z, 1.0f);
             }
-            return;
+            break;
         }
         BsdfSample bs = sample_bsdf(g, m, base, s, -d);
-        if (!bs.ok) return;
+        if (!bs.ok) break;
         throughput = throughput * bs.weight;
         o = offset(s, bs.wi);
         d = bs.wi;
     }
+    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+template <bool TEXTURES>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Params P, const uint32_t* heads,
+                                                                 const float4* __restrict__ photons,
+                                                                 float4* __restrict__ gathered, uint32_t* spill,
+                                                                 uint32_t slots) {
+    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x >= P.width || y >= P.height) return;
+    const Ctx C{S, &P};
+    const uint32_t index = y * P.width + x;
+    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, index, slots);
+    gathered[index] = gather_pixel<TEXTURES>(C, heads, photons, x, y, index, stack);
 }
 
 // ------------------------------------------------------------------ denoiser
@@ -1173,21 +1189,21 @@ void launch_light_trace(const DevScene& S, const Params& P, const float4* emitte
     }
 }
 
-void launch_resolve(const Params& P, uint4* splats, float4* accum, hipStream_t stream) {
+void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, hipStream_t stream) {
     uint32_t n = P.width * P.height;
     hipLaunchKernelGGL(k_resolve, dim3((n + BLOCK_THREADS - 1) / BLOCK_THREADS), dim3(BLOCK_THREADS), 0, stream, P,
-                       splats, accum);
+                       splats, gathered, accum);
 }
 
 void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
-                          float4* accum, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream) {
+                          float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream) {
     dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
     if (f.textures)
-        hipLaunchKernelGGL(k_photon_gather<true>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons, accum,
-                           spill, slots);
+        hipLaunchKernelGGL(k_photon_gather<true>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons,
+                           gathered, spill, slots);
     else
-        hipLaunchKernelGGL(k_photon_gather<false>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons, accum,
-                           spill, slots);
+        hipLaunchKernelGGL(k_photon_gather<false>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons,
+                           gathered, spill, slots);
 }
 
 void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStream_t stream) {

@@ -38,9 +38,9 @@ void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hi
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
                         float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
                         hipStream_t stream);
-void launch_resolve(const Params& P, uint4* splats, float4* accum, hipStream_t stream);
+void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, hipStream_t stream);
 void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
-                          float4* accum, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream);
+                          float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream);
 void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStream_t stream);
 void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* guides, float4* out,
                     hipStream_t stream);
