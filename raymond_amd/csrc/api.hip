@@ -74,8 +74,14 @@ struct rm_context {
     // on `stream`; fork and join order them against the rest of the frame.
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // The denoiser's à-trous passes and compose run on `post` once prepare
+    // has read the frame's accumulation, so the next frame's path pass can
+    // start meanwhile; `prepared_ev` and `denoised_ev` order the two.
+    hipStream_t post = nullptr;
+    hipEvent_t prepared_ev = nullptr, denoised_ev = nullptr;
+    bool denoise_pending = false;
     uint32_t width = 0, height = 0, capacity = 0;
-    DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats, guides;
+    DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats, guides, albedo_frame;
     DevBuf heads, photons, emitters, spill, side_spill, gathered;
     DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
@@ -123,6 +129,8 @@ struct Timed {
 
 int collect_times(rm_context* c) {
     RM_HIP(hipStreamSynchronize(c->stream), "waiting for the stream");
+    RM_HIP(hipStreamSynchronize(c->side), "waiting for the side stream");
+    RM_HIP(hipStreamSynchronize(c->post), "waiting for the denoiser's stream");
     for (int s = 0; s < RM_STAGE_COUNT; ++s) {
         for (auto& p : c->timer.pending[s]) {
             float ms = 0.0f;
@@ -195,7 +203,13 @@ int rm_open(int device, rm_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->post, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->prepared_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->denoised_ev, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
+        if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
+        if (c->post) (void)hipStreamDestroy(c->post);
         if (c->join) (void)hipEventDestroy(c->join);
         if (c->fork) (void)hipEventDestroy(c->fork);
         if (c->side) (void)hipStreamDestroy(c->side);
@@ -237,11 +251,12 @@ int rm_close(rm_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->post) (void)hipStreamSynchronize(c->post);
     for (rm_scene* s : c->scenes) {
         for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights, &s->textures, &s->nmaps}) b->release();
         delete s;
     }
-    for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->heads,
+    for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->albedo_frame, &c->heads,
                       &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
@@ -250,6 +265,9 @@ int rm_close(rm_context* c) {
             (void)hipEventDestroy(p.second);
         }
     for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+    if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
+    if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
+    if (c->post) (void)hipStreamDestroy(c->post);
     if (c->join) (void)hipEventDestroy(c->join);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -495,24 +513,31 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
     if (P.denoise != 0u) {
         float4* a = c->light_a.as<float4>();
         float4* b = c->light_b.as<float4>();
+        RM_HIP(c->albedo_frame.ensure((size_t)c->width * c->height * 16), "allocating the denoiser's albedo");
+        // The previous frame's passes still read light_a/b, guides and the albedo.
+        if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "ordering the denoiser");
         {
             Timed t(c, RM_STAGE_PREPARE);
             rm::launch_guides(P, c->gnorm.as<const float4>(), c->guides.as<float4>(), c->stream);
             rm::launch_prepare(P, c->accum.as<const float4>(), c->galb.as<const float4>(), c->guides.as<const float4>(),
-                               a, c->stream);
+                               a, c->albedo_frame.as<float4>(), c->stream);
         }
+        RM_HIP(hipEventRecord(c->prepared_ev, c->stream), "ordering the denoiser");
+        RM_HIP(hipStreamWaitEvent(c->post, c->prepared_ev, 0), "ordering the denoiser");
         const int steps[5] = {1, 2, 4, 8, 16};
         float4* in = a;
         float4* out = b;
         for (int k = 0; k < 5; ++k) {
-            Timed t(c, RM_STAGE_ATROUS);
-            rm::launch_atrous(P, steps[k], c->guides.as<const float4>(), in, out, c->stream);
+            Timed t(c, RM_STAGE_ATROUS, c->post);
+            rm::launch_atrous(P, steps[k], c->guides.as<const float4>(), in, out, c->post);
             std::swap(in, out);
         }
         {
-            Timed t(c, RM_STAGE_COMPOSE);
-            rm::launch_compose(P, c->galb.as<const float4>(), in, c->denoised.as<float4>(), c->stream);
+            Timed t(c, RM_STAGE_COMPOSE, c->post);
+            rm::launch_compose(P, c->albedo_frame.as<const float4>(), in, c->denoised.as<float4>(), c->post);
         }
+        RM_HIP(hipEventRecord(c->denoised_ev, c->post), "ordering the denoiser");
+        c->denoise_pending = true;
         RM_HIP(hipGetLastError(), "launching the denoiser");
     }
     return RM_OK;
@@ -532,6 +557,7 @@ int rm_read(rm_context* c, int buffer, float* dst, size_t floats) {
     size_t have = (size_t)c->width * c->height * 4;
     if (floats > have) return fail(RM_ERR_INVALID, "rm_read asked for more than the image holds.");
     RM_HIP(hipSetDevice(c->device), "selecting the device");
+    if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "waiting for the denoiser");
     RM_HIP(hipMemcpyAsync(dst, b->ptr, floats * 4, hipMemcpyDeviceToHost, c->stream), "reading back pixels");
     RM_HIP(hipStreamSynchronize(c->stream), "reading back pixels");
     return RM_OK;
@@ -540,6 +566,7 @@ int rm_read(rm_context* c, int buffer, float* dst, size_t floats) {
 int rm_synchronize(rm_context* c) {
     if (!c) return fail(RM_ERR_INVALID, "rm_synchronize needs a context.");
     RM_HIP(hipSetDevice(c->device), "selecting the device");
+    if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "waiting for the denoiser");
     RM_HIP(hipStreamSynchronize(c->stream), "waiting for the GPU");
     return RM_OK;
 }

@@ -979,18 +979,23 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_guides(Params P, const float4
     guides[i] = normal_at(gnorm, i, (float)max(P.total_samples, 1u));
 }
 
-// src/shaders.mjs:1974-2021
+// src/shaders.mjs:1974-2021. The pixel's albedo (albedo_at) also goes to
+// albedo_out for k_compose, so that the passes after this one no longer read
+// the accumulation buffers the next frame writes.
 __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float4* __restrict__ accum,
                                                            const float4* __restrict__ galb,
-                                                           const float4* __restrict__ guides, float4* light_out) {
+                                                           const float4* __restrict__ guides, float4* light_out,
+                                                           float4* __restrict__ albedo_out) {
     uint32_t x, y;
     if (!stencil_pixel(P.width, P.height, x, y)) return;
     const uint32_t i = y * P.width + x;
     const float n = (float)max(P.total_samples, 1u);
     float4 ac = accum[i];
-    V3 light = (v3(ac.x, ac.y, ac.z) / n) / albedo_at(galb, i, n);
+    const V3 own_albedo = albedo_at(galb, i, n);
+    albedo_out[i] = make_float4(own_albedo.x, own_albedo.y, own_albedo.z, 0.0f);
+    V3 light = (v3(ac.x, ac.y, ac.z) / n) / own_albedo;
     float4 gp = guides[i];
-    float a = luminance(albedo_at(galb, i, n));
+    float a = luminance(own_albedo);
     float variance;
     if (n >= 4.0f) {
         float mean = luminance(v3(ac.x, ac.y, ac.z)) / n;
@@ -1084,15 +1089,15 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, co
     }
 }
 
-__global__ __launch_bounds__(BLOCK_THREADS) void k_compose(Params P, const float4* __restrict__ galb,
+// src/shaders.mjs:2088-2098, with the albedo k_prepare recorded.
+__global__ __launch_bounds__(BLOCK_THREADS) void k_compose(Params P, const float4* __restrict__ albedo,
                                                            const float4* __restrict__ light_in, float4* out) {
     uint32_t x, y;
     tile_pixel(x, y);
     if (x >= P.width || y >= P.height) return;
     const uint32_t i = y * P.width + x;
-    const float n = (float)max(P.total_samples, 1u);
-    float4 l = light_in[i];
-    V3 r = v3(l.x, l.y, l.z) * albedo_at(galb, i, n);
+    float4 l = light_in[i], a = albedo[i];
+    V3 r = v3(l.x, l.y, l.z) * v3(a.x, a.y, a.z);
     out[i] = make_float4(r.x, r.y, r.z, 0.0f);
 }
 
@@ -1213,9 +1218,9 @@ void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStre
 }
 
 void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* guides, float4* out,
-                    hipStream_t stream) {
+                    float4* albedo_out, hipStream_t stream) {
     dim3 grid(stencil_blocks(P.width, P.height));
-    hipLaunchKernelGGL(k_prepare, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb, guides, out);
+    hipLaunchKernelGGL(k_prepare, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb, guides, out, albedo_out);
 }
 
 void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream) {
@@ -1223,9 +1228,9 @@ void launch_atrous(const Params& P, int step, const float4* guides, const float4
     hipLaunchKernelGGL(k_atrous, grid, dim3(BLOCK_THREADS), 0, stream, P, step, guides, in, out);
 }
 
-void launch_compose(const Params& P, const float4* galb, const float4* in, float4* out, hipStream_t stream) {
+void launch_compose(const Params& P, const float4* albedo, const float4* in, float4* out, hipStream_t stream) {
     dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
-    hipLaunchKernelGGL(k_compose, grid, dim3(BLOCK_THREADS), 0, stream, P, galb, in, out);
+    hipLaunchKernelGGL(k_compose, grid, dim3(BLOCK_THREADS), 0, stream, P, albedo, in, out);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream) {

@@ -43,9 +43,9 @@ void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* he
                           float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream);
 void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStream_t stream);
 void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* guides, float4* out,
-                    hipStream_t stream);
+                    float4* albedo_out, hipStream_t stream);
 void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream);
-void launch_compose(const Params& P, const float4* galb, const float4* in, float4* out, hipStream_t stream);
+void launch_compose(const Params& P, const float4* albedo, const float4* in, float4* out, hipStream_t stream);
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 
 }  // namespace rm
