@@ -177,7 +177,7 @@ int upload(rm_context* c, DevBuf& b, const void* src, size_t bytes, const char* 
 }
 
 int ensure_spill(rm_context* c, uint32_t slots) {
-    RM_HIP(c->spill.ensure((size_t)slots * rm::SPILL_ENTRIES * sizeof(uint32_t)), "allocating the traversal spill");
+    RM_HIP(c->spill.ensure((size_t)slots * rm::SPILL_ENTRIES * sizeof(uint2)), "allocating the traversal spill");
     return RM_OK;
 }
 
@@ -448,7 +448,7 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         if ((uint64_t)P.width * P.height * P.samples_per_frame < rm::PHOTON_CAPACITY)
             count = P.width * P.height * P.samples_per_frame;
         const uint32_t side_slots = count > px ? count : px;
-        RM_HIP(c->side_spill.ensure((size_t)side_slots * rm::SPILL_ENTRIES * sizeof(uint32_t)),
+        RM_HIP(c->side_spill.ensure((size_t)side_slots * rm::SPILL_ENTRIES * sizeof(uint2)),
                "allocating the side traversal spill");
         RM_HIP(hipEventRecord(c->fork, c->stream), "forking the side stream");
         RM_HIP(hipStreamWaitEvent(c->side, c->fork, 0), "forking the side stream");

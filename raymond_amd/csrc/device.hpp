@@ -305,18 +305,20 @@ struct Hit { float t; uint32_t prim; float u, v; };
 // A lane's traversal stack: the first LDS_DEPTH entries in LDS, laid out
 // entry-major across the block ([entry][thread]) so a wave's lanes hit
 // distinct banks; deeper entries spill to this thread's slice of global
-// memory. Capacity is the reference's 64 entries; a full stack drops the
-// far child exactly as the reference does.
+// memory. An entry is a pushed node's (first, count), which the parent's
+// child test has already loaded, so a pop costs no global load. Capacity is
+// the reference's 64 entries; a full stack drops the far child exactly as the
+// reference does.
 template <int LDS_DEPTH, int BLOCK>
 struct Stack {
-    uint32_t* lds;      // &shared[0][threadIdx.x]
-    uint32_t* spill;    // this thread's column of an [entry][slot] spill array
-    uint32_t stride;    // slots in the spill array
-    RM_D void put(uint32_t sp, uint32_t v) {
+    uint2* lds;      // &shared[0][threadIdx.x]
+    uint2* spill;    // this thread's column of an [entry][slot] spill array
+    uint32_t stride; // slots in the spill array
+    RM_D void put(uint32_t sp, uint2 v) {
         if (LDS_DEPTH >= (int)STACK_LIMIT || sp < (uint32_t)LDS_DEPTH) lds[sp * BLOCK] = v;
         else spill[(size_t)(sp - LDS_DEPTH) * stride] = v;
     }
-    RM_D uint32_t get(uint32_t sp) const {
+    RM_D uint2 get(uint32_t sp) const {
         if (LDS_DEPTH >= (int)STACK_LIMIT || sp < (uint32_t)LDS_DEPTH) return lds[sp * BLOCK];
         return spill[(size_t)(sp - LDS_DEPTH) * stride];
     }
@@ -367,7 +369,8 @@ RM_D Hit trace(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& 
             bool left_first = t_left <= t_right;
             if (fminf(t_left, t_right) < INF) {
                 if (fmaxf(t_left, t_right) < INF && sp < STACK_LIMIT) {
-                    stack.put(sp, left_first ? first + 1u : first);
+                    stack.put(sp, make_uint2(__float_as_uint(left_first ? q0.w : l0.w),
+                                             __float_as_uint(left_first ? q1.w : l1.w)));
                     sp++;
                 }
                 first = __float_as_uint(left_first ? l0.w : q0.w);
@@ -378,9 +381,9 @@ RM_D Hit trace(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& 
         if (!descend) {
             if (sp == 0u) break;
             sp--;
-            uint32_t node = stack.get(sp);
-            first = __float_as_uint(N[(size_t)node * 2].w);
-            count = __float_as_uint(N[(size_t)node * 2 + 1].w);
+            uint2 e = stack.get(sp);
+            first = e.x;
+            count = e.y;
         }
     }
     return hit;

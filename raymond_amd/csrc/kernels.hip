@@ -55,10 +55,10 @@ RM_D bool stencil_pixel(uint32_t width, uint32_t height, uint32_t& x, uint32_t& 
 template <int LDS_DEPTH>
 struct BlockStack {
     using S = Stack<LDS_DEPTH, BLOCK_THREADS>;
-    RM_D static S make(uint32_t* lds, uint32_t* spill_base, uint32_t slot, uint32_t slots) {
+    RM_D static S make(uint2* lds, uint32_t* spill_base, uint32_t slot, uint32_t slots) {
         S s;
         s.lds = lds + threadIdx.x;
-        s.spill = spill_base + slot;
+        s.spill = reinterpret_cast<uint2*>(spill_base) + slot;
         s.stride = slots;
         return s;
     }
@@ -71,7 +71,7 @@ template <bool ANY>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_raycast(DevScene S, const float4* __restrict__ rays, uint32_t n,
                                                            uint4* __restrict__ hits, uint32_t* spill, uint32_t slots,
                                                            uint2* counters) {
-    __shared__ uint32_t lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
+    __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     if (i >= n) return;
     auto stack = BlockStack<TRACE_LDS_DEPTH>::make(lds, spill, i, slots);
@@ -372,7 +372,7 @@ template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main(
     DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
     float4* __restrict__ guide_normal, uint32_t* spill, uint32_t slots) {
-    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     uint32_t x, y;
     tile_pixel(x, y);
     if (x >= P.width || y >= P.height) return;
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
 // bit-identical; only the colour view runs here. The barrier after every
 // bounce ties a block's waves to its slowest one, so with short paths (1 spp
 // per frame on the sphere scene) k_main is faster.
-constexpr int COMPACT_LDS_DEPTH = 8;
+constexpr int COMPACT_LDS_DEPTH = 4;
 constexpr int SLOT_WORDS = 27;
 
 struct SlotStore {
@@ -478,7 +478,7 @@ template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main_compact(
     DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
     float4* __restrict__ guide_normal, uint32_t* spill, uint32_t slots, float4* __restrict__ staging) {
-    __shared__ uint32_t lds[COMPACT_LDS_DEPTH * BLOCK_THREADS];
+    __shared__ uint2 lds[COMPACT_LDS_DEPTH * BLOCK_THREADS];
     __shared__ uint32_t slot_words[SLOT_WORDS * BLOCK_THREADS];
     __shared__ uint32_t wave_live[BLOCK_THREADS / 64];
     const Ctx C{S, &P};
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, P
                                                                    float4* __restrict__ guide_albedo,
                                                                    float4* __restrict__ guide_normal, uint32_t* spill,
                                                                    uint32_t slots, uint32_t* counter) {
-    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     const Ctx C{S, &P};
     const uint32_t gid = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, gid, slots);
@@ -666,7 +666,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Param
                                                                uint32_t* splats, uint32_t* heads,
                                                                float4* __restrict__ photons, uint32_t* spill,
                                                                uint32_t slots) {
-    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     const Ctx C{S, &P};
     const uint32_t id = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     const uint32_t photons_n = photon_count(P);
@@ -892,7 +892,7 @@ RM_D V3 gather(const Ctx& C, const uint32_t* heads, const float4* photons, const
 // gathered[index] (w = 1), or zeros when it adds none; k_resolve adds it.
 template <bool TEXTURES>
 RM_D float4 gather_pixel(const Ctx& C, const uint32_t* heads, const float4* photons, uint32_t x, uint32_t y,
-                         uint32_t index, Stack<MAIN_LDS_DEPTH, BLOCK_THREADS>& stack) {
+                         uint32_t index, Stack<GATHER_LDS_DEPTH, BLOCK_THREADS>& stack) {
     const Params& P = *C.P;
     const DevScene& S = C.S;
     NoCount nc;
@@ -932,13 +932,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Par
                                                                  const float4* __restrict__ photons,
                                                                  float4* __restrict__ gathered, uint32_t* spill,
                                                                  uint32_t slots) {
-    __shared__ uint32_t lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    __shared__ uint2 lds[GATHER_LDS_DEPTH * BLOCK_THREADS];
     uint32_t x, y;
     tile_pixel(x, y);
     if (x >= P.width || y >= P.height) return;
     const Ctx C{S, &P};
     const uint32_t index = y * P.width + x;
-    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, index, slots);
+    auto stack = BlockStack<GATHER_LDS_DEPTH>::make(lds, spill, index, slots);
     gathered[index] = gather_pixel<TEXTURES>(C, heads, photons, x, y, index, stack);
 }
 

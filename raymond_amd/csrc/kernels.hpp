@@ -8,12 +8,15 @@ namespace rm {
 // 256 threads: four wave64s covering a 16×16 pixel tile.
 constexpr int BLOCK_THREADS = 256;
 // Traversal stack entries kept in LDS per thread (the rest of the
-// reference's 64 spill to global memory). 16 KiB of LDS per block each.
+// reference's 64 spill to global memory), 8 bytes each: 32 KiB of LDS per
+// block at 16, which still lets four blocks of the path pass share a CU; the
+// gather and raycast kernels hold more waves, so they keep fewer.
 constexpr int MAIN_LDS_DEPTH = 16;
-constexpr int TRACE_LDS_DEPTH = 16;
-// Spill entries per lane: the rest of the 64 beyond the shallowest LDS stack
-// (the compacted path kernel keeps 8 entries in LDS).
-constexpr uint32_t SPILL_ENTRIES = STACK_LIMIT - 8u;
+constexpr int TRACE_LDS_DEPTH = 8;
+constexpr int GATHER_LDS_DEPTH = 8;
+// Spill entries (8 bytes each) per lane: the rest of the 64 beyond the
+// shallowest LDS stack (the compacted path kernel keeps 4 entries in LDS).
+constexpr uint32_t SPILL_ENTRIES = STACK_LIMIT - 4u;
 
 struct Features { bool volumes, textures, environment, dispersion; };
 
