@@ -138,6 +138,52 @@ int rm_set_option(rm_context* ctx, int option, int64_t value);
 /* The device's name and compute-unit count. */
 int rm_device_info(rm_context* ctx, char* name, size_t name_len, int* compute_units);
 
+/* ------------------------------------------------------------------ serving
+ * A job server: render jobs (a scene, a parameter block, a target sample
+ * count) arrive while others run. Up to `slots` jobs are active at once, each
+ * in its own context with its own streams, so their frames overlap on the
+ * device; more wait in a queue until a slot is free and the memory the job
+ * needs fits in `memory_budget` bytes. Scenes stay resident: upload them on
+ * rm_server_context(server) and refer to them from any number of jobs; release
+ * a scene only after its jobs are done. A job renders frames exactly as
+ * rm_frame does on a context of its own (sample offsets, running totals and
+ * the photon radius advancing as the Renderer advances them), so its image is
+ * the same whatever else is running. No counterpart in the reference, which
+ * renders one image per Renderer; this is the serving loop its callers would
+ * otherwise write around Renderer.render. */
+typedef struct rm_server rm_server;
+
+typedef struct rm_job {
+    const rm_scene* scene;         /* uploaded on rm_server_context() */
+    const void* params;            /* the first frame's 208-byte block: size, samples per frame, features */
+    const float* emitters;         /* light tracing's emitters when params.caustics is set, else NULL */
+    uint32_t emitter_vec4s;
+    const uint16_t* env_rgba;      /* the environment (as rm_environment takes it), or NULL for the sky */
+    uint32_t env_width, env_height;
+    const float* env_table;
+    uint32_t table_width, table_height;
+    uint32_t target_samples;       /* samples per pixel at which the job is done */
+} rm_job;
+
+typedef struct rm_job_stats {
+    double queued_ms;              /* submit → admitted to a slot */
+    double first_frame_ms;         /* submit → its first frame (denoised if params.denoise) complete */
+    double done_ms;                /* submit → target reached and the image copied out */
+    uint32_t samples;              /* samples per pixel accumulated */
+    uint32_t frames;
+} rm_job_stats;
+
+int rm_server_open(int device, uint32_t slots, uint64_t memory_budget, rm_server** out);
+/* The context scenes are uploaded on (rm_scene_upload, rm_scene_textures). */
+rm_context* rm_server_context(rm_server* server);
+/* Copies the job's parameters, emitters and environment; returns its id. */
+int rm_server_submit(rm_server* server, const rm_job* job, uint64_t* id);
+/* Waits for a job and copies its image: the denoised buffer when the job's
+ * params.denoise is set, else the accumulation (W·H·4 floats; dst may be NULL). */
+int rm_server_wait(rm_server* server, uint64_t id, float* dst, size_t floats, rm_job_stats* stats);
+/* Waits for every job, then frees the server and its scenes. */
+int rm_server_close(rm_server* server);
+
 #ifdef __cplusplus
 }
 #endif

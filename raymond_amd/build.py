@@ -37,7 +37,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-fno-slp-vectorize",
     f"-I{ROOT / 'include'}", "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["kernels.hip", "api.hip"]
+SOURCES = ["kernels.hip", "api.hip", "server.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -38,6 +38,11 @@ SIGNATURES = {
     "rm_reset_stage_times": (C.c_int, [_P]),
     "rm_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
     "rm_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "rm_server_open": (C.c_int, [C.c_int, _U32, C.c_uint64, C.POINTER(_P)]),
+    "rm_server_context": (_P, [_P]),
+    "rm_server_submit": (C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
+    "rm_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t, _P]),
+    "rm_server_close": (C.c_int, [_P]),
 }
 OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS = 0, 1
 SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT = 0, 1, 2
@@ -88,17 +93,20 @@ class Scene:
 class Context:
     """One device context: a stream, per-pixel buffers and uploaded scenes."""
 
-    def __init__(self, device: int = 0):
-        lib = load()
-        h = C.c_void_p()
-        _check(lib.rm_open(device, C.byref(h)), "rm_open")
-        self.handle = h
+    def __init__(self, device: int = 0, handle=None):
+        """Opens device `device`, or wraps an existing handle (not closed by close())."""
+        self.owned = handle is None
+        if handle is None:
+            lib = load()
+            handle = C.c_void_p()
+            _check(lib.rm_open(device, C.byref(handle)), "rm_open")
+        self.handle = handle
         self.width = self.height = 0
 
     def close(self) -> None:
-        if self.handle:
+        if self.handle and self.owned:
             _check(load().rm_close(self.handle), "rm_close")
-            self.handle = None
+        self.handle = None
 
     def __enter__(self):
         return self
@@ -188,3 +196,70 @@ class Context:
         cus = C.c_int()
         _check(load().rm_device_info(self.handle, buf, 256, C.byref(cus)), "rm_device_info")
         return buf.value.decode(), cus.value
+
+
+class _Job(C.Structure):
+    _fields_ = [("scene", _P), ("params", _P), ("emitters", _P), ("emitter_vec4s", _U32), ("env_rgba", _P),
+                ("env_width", _U32), ("env_height", _U32), ("env_table", _P), ("table_width", _U32),
+                ("table_height", _U32), ("target_samples", _U32)]
+
+
+class _JobStats(C.Structure):
+    _fields_ = [("queued_ms", C.c_double), ("first_frame_ms", C.c_double), ("done_ms", C.c_double),
+                ("samples", _U32), ("frames", _U32)]
+
+
+class Server:
+    """The job server (rm_server_*): scenes uploaded once on `context`, jobs
+    submitted as they arrive, admitted into `slots` concurrent slots within
+    `memory_budget` bytes, each job's image and timings returned by wait()."""
+
+    def __init__(self, device: int = 0, slots: int = 8, memory_budget: int = 200 << 30):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.rm_server_open(device, slots, memory_budget, C.byref(h)), "rm_server_open")
+        self.handle = h
+        self.context = Context(handle=C.c_void_p(lib.rm_server_context(h)))
+        self.sizes = {}
+
+    def submit(self, scene: Scene, params: np.ndarray, target_samples: int, emitters: np.ndarray | None = None,
+               packed=None) -> int:
+        """Queues a job; `packed` (a scenefile.Packed) supplies the environment when it has a map."""
+        p = np.ascontiguousarray(params, dtype=np.uint8)
+        e = None if emitters is None else np.ascontiguousarray(emitters, dtype=np.float32)
+        job = _Job(scene=scene.handle, params=_ptr(p), emitters=_ptr(e), emitter_vec4s=0 if e is None else e.size // 4,
+                   target_samples=target_samples)
+        keep = [p, e]
+        if packed is not None and packed.meta.get("envTableHeight", 0) > 0:
+            rgba = np.ascontiguousarray(packed.env_rgba, dtype=np.uint16)
+            table = np.ascontiguousarray(packed.env_table, dtype=np.float32)
+            keep += [rgba, table]
+            m = packed.meta
+            job.env_rgba, job.env_width, job.env_height = _ptr(rgba), m["envWidth"], m["envHeight"]
+            job.env_table, job.table_width, job.table_height = _ptr(table), m["envTableWidth"], m["envTableHeight"]
+        job_id = C.c_uint64()
+        _check(load().rm_server_submit(self.handle, C.byref(job), C.byref(job_id)), "rm_server_submit")
+        words = p.view(np.uint32)
+        self.sizes[job_id.value] = (int(words[41]), int(words[40]))
+        return job_id.value
+
+    def wait(self, job_id: int):
+        """(image as (H, W, 4) float32, timing dict) of a job, once it is done."""
+        h, w = self.sizes.pop(job_id)
+        out = np.empty((h, w, 4), dtype=np.float32)
+        st = _JobStats()
+        _check(load().rm_server_wait(self.handle, job_id, _ptr(out), out.size, C.byref(st)), "rm_server_wait")
+        return out, {"queued_ms": st.queued_ms, "first_frame_ms": st.first_frame_ms, "done_ms": st.done_ms,
+                     "samples": st.samples, "frames": st.frames}
+
+    def close(self) -> None:
+        if self.handle:
+            _check(load().rm_server_close(self.handle), "rm_server_close")
+            self.handle = None
+            self.context.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

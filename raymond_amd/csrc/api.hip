@@ -1,15 +1,13 @@
 // The C ABI (include/raymond.h) over the kernels: device memory, the frame's
 // dispatch order (src/Renderer.mjs:441-489), readback and stage timing.
-#include <raymond.h>
-
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
-#include "kernels.hpp"
+#include "context.hpp"
 
-namespace {
+namespace rmi {
 
 thread_local std::string g_error;
 
@@ -23,75 +21,12 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define RM_HIP(call, what)                              \
-    do {                                                \
-        hipError_t e_ = (call);                         \
-        if (e_ != hipSuccess) return hip_fail(e_, what); \
-    } while (0)
+}  // namespace rmi
 
-// A growable device allocation.
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t need) {
-        if (need <= bytes) return hipSuccess;
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&ptr, need);
-        if (e == hipSuccess) bytes = need;
-        return e;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() const { return static_cast<T*>(ptr); }
-};
-
-struct StageTimer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[RM_STAGE_COUNT];
-    std::vector<hipEvent_t> pool;
-    double ms[RM_STAGE_COUNT] = {};
-    uint32_t launches[RM_STAGE_COUNT] = {};
-};
-
-}  // namespace
-
-struct rm_scene {
-    DevBuf prims, nodes, mats, attr, lights, textures, nmaps;
-    uint32_t primitive_slots = 0, primitive_count = 0, node_count = 0, material_count = 0;
-    uint32_t tex_w = 0, tex_h = 0, tex_layers = 0, nmap_w = 0, nmap_h = 0, nmap_layers = 0;
-    bool dispersive = false;
-    std::vector<uint32_t> lights_host;   // the lights table, for checking parameters against
-};
-
-struct rm_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    // Light tracing and photon gather run on `side`, alongside the path pass
-    // on `stream`; fork and join order them against the rest of the frame.
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    // The denoiser's à-trous passes and compose run on `post` once prepare
-    // has read the frame's accumulation, so the next frame's path pass can
-    // start meanwhile; `prepared_ev` and `denoised_ev` order the two.
-    hipStream_t post = nullptr;
-    hipEvent_t prepared_ev = nullptr, denoised_ev = nullptr;
-    bool denoise_pending = false;
-    uint32_t width = 0, height = 0, capacity = 0;
-    DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats, guides, albedo_frame;
-    DevBuf heads, photons, emitters, spill, side_spill, gathered;
-    DevBuf env, env_table;
-    uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
-    DevBuf rays, hits, counters, work, staging;
-    int compute_units = 0;
-    int64_t options[RM_OPTION_COUNT] = {0, 0};
-    bool timing = false;
-    StageTimer timer;
-    std::vector<rm_scene*> scenes;
-};
+using rmi::DevBuf;
+using rmi::fail;
+using rmi::g_error;
+using rmi::hip_fail;
 
 namespace {
 

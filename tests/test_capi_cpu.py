@@ -10,7 +10,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def declared():
     text = (ROOT / "include" / "raymond.h").read_text()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(rm_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|const char\*|rm_context\*)\s+(rm_\w+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -55,3 +55,14 @@ def test_oracle_is_not_linked_into_the_product():
     assert "oracle_" not in nm
     ldd = subprocess.run(["ldd", str(lib)], capture_output=True, text=True).stdout
     assert "liboracle" not in ldd
+
+
+def test_server_rejects_null_arguments_without_touching_a_device():
+    from raymond_amd import capi
+    lib = capi.load()
+    assert lib.rm_server_open(0, 4, 1 << 30, None) == -1
+    assert b"server" in lib.rm_last_error()
+    assert lib.rm_server_context(None) is None
+    assert lib.rm_server_submit(None, None, None) == -1
+    assert lib.rm_server_wait(None, 1, None, 0, None) == -1
+    assert lib.rm_server_close(None) == -1

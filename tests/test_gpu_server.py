@@ -1,0 +1,71 @@
+"""The job server (rm_server_*): jobs rendered while others run give the same
+image as the same frames rendered alone on a context of their own.
+
+Without light tracing a frame's arithmetic does not depend on scheduling, so
+the images agree bit for bit. With light tracing, photons land in the hash
+grid in atomic order, so gathered sums agree to float rounding; those jobs
+are compared as the light-tracing tests compare against the oracle."""
+import numpy as np
+import pytest
+
+from helpers import agreement, gpu_render
+from raymond_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+JOBS = [
+    ("cornell", {"width": 64, "height": 64}, dict(caustics=0, denoise=1, samples_per_frame=2), 6),
+    ("spheres", {"scene": {"width": 96, "height": 54}, "width": 96, "height": 54}, dict(caustics=0, denoise=1), 5),
+    ("triangles", {"scene": {"grid": 32, "extras": 3, "envWidth": 64, "envHeight": 32, "width": 80, "height": 48},
+                   "width": 80, "height": 48}, dict(caustics=0, denoise=1), 4),
+    ("fixture", {"scene": {"name": "causticLamp", "size": 48}}, dict(caustics=1, denoise=0), 3),
+]
+
+
+def alone(ctx, p, params, frames):
+    out = gpu_render(ctx, p, frames, params, denoised=True)
+    return out[3] if params.view(np.uint32)[46] else out[0]
+
+
+def test_jobs_match_their_frames_rendered_alone(ctx, pack):
+    packs = [pack(c, **o) for c, o, _, _ in JOBS]
+    params = [p.with_params(firefly_clamp=0.0, **f) for p, (_, _, f, _) in zip(packs, JOBS)]
+    frames = [n for *_, n in JOBS]
+    expected = [alone(ctx, p, q, n) for p, q, n in zip(packs, params, frames)]
+    with capi.Server(slots=3, memory_budget=8 << 30) as server:
+        scenes = [server.context.upload_scene(p) for p in packs]
+        ids = []
+        for _ in range(2):  # every job twice, all in flight together
+            for p, q, n, s in zip(packs, params, frames, scenes):
+                spp = int(q.view(np.uint32)[43])
+                caustics = bool(q.view(np.uint32)[47])
+                ids.append(server.submit(s, q, n * spp, p.emitters if caustics else None, packed=p))
+        results = [server.wait(i) for i in ids]
+    for k, (image, stats) in enumerate(results):
+        j = k % len(JOBS)
+        name = JOBS[j][0]
+        spp = int(params[j].view(np.uint32)[43])
+        assert stats["frames"] == frames[j] and stats["samples"] == frames[j] * spp, (name, stats)
+        assert 0.0 <= stats["queued_ms"] <= stats["first_frame_ms"] <= stats["done_ms"], (name, stats)
+        if params[j].view(np.uint32)[47]:
+            frac = agreement(image, expected[j], rel=1e-3, abs_=1e-4)
+            assert frac > 0.97, (name, frac)
+        else:
+            assert np.array_equal(image.view(np.uint32), expected[j].view(np.uint32)), name
+
+
+def test_budget_serialises_jobs(pack):
+    """A budget smaller than any job admits one at a time, and they still finish."""
+    p = pack("spheres", scene={"width": 64, "height": 36}, width=64, height=36)
+    q = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
+    with capi.Server(slots=4, memory_budget=1) as server:
+        scene = server.context.upload_scene(p)
+        ids = [server.submit(scene, q, 3) for _ in range(4)]
+        results = [server.wait(i) for i in ids]
+    first = results[0][0]
+    for image, stats in results:
+        assert stats["samples"] == 3
+        assert np.array_equal(image.view(np.uint32), first.view(np.uint32))
+    # Admitted one after another: each later job waited in the queue.
+    queued = [s["queued_ms"] for _, s in results]
+    assert queued == sorted(queued)
