@@ -20,7 +20,12 @@ Other workloads (--workload), reported beside it:
                          photon gather every frame;
   cornell-raycast-256    the 256 × 256 pinhole primary rays of the Cornell box,
                          closest hit, in Mrays/s, checked bit for bit against
-                         the oracle.
+                         the oracle;
+  serving-poisson        jobs arriving as a Poisson process (scene from a pool
+                         of 9 to 2.2 M primitives, 256² to 1080p, 4-12
+                         bounces, a target spp), served by the job server with
+                         concurrent slots and, on the same trace, one job at a
+                         time: throughput and p50/p99 latencies of both.
 
 Prints one JSON line; `roofline` is for the dominant kernel (the path-tracing
 pass, or the raycast kernel), `cpu_baseline` times the CPU oracle on the
@@ -69,6 +74,18 @@ WORKLOADS = {
     },
 }
 RAYCAST = "cornell-raycast-256"
+SERVING = "serving-poisson"
+
+# The served trace's scene pool, smallest to largest (9 to 2.2 M primitives).
+SERVING_POOL = [
+    ("cornell", {"width": 256, "height": 256}),
+    ("spheres", {"scene": {"width": 256, "height": 256}, "width": 256, "height": 256}),
+    ("mixed", {"width": 256, "height": 256}),
+    ("caustics", {"scene": {"res": 128, "width": 256, "height": 256}, "width": 256, "height": 256}),
+    ("triangles", {"scene": {"grid": 256, "width": 256, "height": 256}, "width": 256, "height": 256}),
+    ("triangles", {"scene": {"grid": 1024, "width": 256, "height": 256}, "width": 256, "height": 256}),
+]
+SERVING_SIZES = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
 
 
 def traffic_from_summary(path):
@@ -267,6 +284,134 @@ def bench_raycast(args) -> dict:
     }
 
 
+def serving_trace(rate: float, duration: float, spp_lo: int, spp_hi: int, seed: int = 5) -> list:
+    """Poisson arrivals at `rate` jobs/s over `duration` s: (arrival s, scene, size, target spp, bounces)."""
+    rng = np.random.default_rng(seed)
+    jobs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            return jobs
+        jobs.append((t, int(rng.integers(len(SERVING_POOL))), SERVING_SIZES[int(rng.integers(len(SERVING_SIZES)))],
+                     int(rng.integers(spp_lo, spp_hi + 1)), int(rng.integers(4, 13))))
+
+
+def run_trace(packs, trace, slots: int, budget: int) -> dict:
+    """Submits the trace's jobs at their arrival times to a server with `slots`
+    slots; returns throughput and latency percentiles."""
+    from raymond_amd import capi, scenefile
+    with capi.Server(slots=slots, memory_budget=budget) as server:
+        scenes = [server.context.upload_scene(p) for p in packs]
+        jobs = []
+        for arrival, k, (w, h), spp, bounces in trace:
+            p = packs[k]
+            params = scenefile.resized_params(p.params, w, h, bounces)
+            caustics = bool(scenefile.get_param(params, "caustics"))
+            jobs.append((arrival, scenes[k], params, spp, p.emitters if caustics else None, p, w * h * spp))
+        t0 = time.perf_counter()
+        ids = []
+        for arrival, scene, params, spp, emitters, p, _ in jobs:
+            delay = t0 + arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(server.submit(scene, params, spp, emitters, packed=p))
+        stats = [server.wait(i)[1] for i in ids]
+        elapsed = time.perf_counter() - t0
+    samples = sum(j[-1] for j in jobs)
+    first = np.array([s["first_frame_ms"] for s in stats])
+    done = np.array([s["done_ms"] for s in stats])
+    return {"Msamples_per_s": round(samples / elapsed / 1e6, 3), "wall_s": round(elapsed, 3), "jobs": len(jobs),
+            "samples": samples, "slots": slots,
+            "first_frame_ms_p50": round(float(np.percentile(first, 50)), 2),
+            "first_frame_ms_p99": round(float(np.percentile(first, 99)), 2),
+            "done_ms_p50": round(float(np.percentile(done, 50)), 2),
+            "done_ms_p99": round(float(np.percentile(done, 99)), 2)}
+
+
+def run_plain(packs, trace) -> dict:
+    """The same trace on the plain path: one context, jobs one at a time in
+    arrival order, every frame rendered and denoised as the Renderer renders
+    it, the first frame waited for (the image a viewer first sees), the last
+    read back. Latencies count from each job's arrival."""
+    from raymond_amd import capi, scenefile
+    first, done, samples = [], [], 0
+    with capi.Context(0) as ctx:
+        scenes = [ctx.upload_scene(p) for p in packs]
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for arrival, k, (w, h), spp, bounces in trace:
+            delay = t0 + arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            p = packs[k]
+            params = scenefile.resized_params(p.params, w, h, bounces)
+            caustics = bool(scenefile.get_param(params, "caustics"))
+            r0 = scenefile.get_param(params, "photon_radius")
+            ctx.upload_environment(p)
+            ctx.resize(w, h)
+            for f in range(spp):
+                r = r0 * (f + 1) ** (-1.0 / 6.0)
+                fp = scenefile.set_params(params.copy(), sample_offset=f, total_samples=f + 1, photon_radius=r,
+                                          photon_cell=2 * r)
+                ctx.frame(scenes[k], fp, p.emitters if caustics else None)
+                if f == 0:
+                    ctx.synchronize()
+                    first.append((time.perf_counter() - t0 - arrival) * 1e3)
+            ctx.read(capi.BUFFER_DENOISED)
+            done.append((time.perf_counter() - t0 - arrival) * 1e3)
+            samples += w * h * spp
+        elapsed = time.perf_counter() - t0
+    first, done = np.array(first), np.array(done)
+    return {"Msamples_per_s": round(samples / elapsed / 1e6, 3), "wall_s": round(elapsed, 3), "jobs": len(trace),
+            "samples": samples, "slots": "plain",
+            "first_frame_ms_p50": round(float(np.percentile(first, 50)), 2),
+            "first_frame_ms_p99": round(float(np.percentile(first, 99)), 2),
+            "done_ms_p50": round(float(np.percentile(done, 50)), 2),
+            "done_ms_p99": round(float(np.percentile(done, 99)), 2)}
+
+
+def bench_serving(args) -> dict:
+    """A Poisson trace of jobs over the scene pool, served with concurrent
+    slots and, on the same trace, one job at a time (the plain path)."""
+    from raymond_amd import capi, scenefile
+    packs = [scenefile.pack(c, dict(o, maxBounces=12, samplesPerFrame=1, fireflyClamp=0.0, denoise=True,
+                                    caustics=True)) for c, o in SERVING_POOL]
+    trace = serving_trace(args.rate, args.duration, args.spp_min, args.spp_max)
+    budget = 200 << 30
+    served = run_trace(packs, trace, args.slots, budget)
+    one_slot = run_trace(packs, trace, 1, budget)
+    plain = run_plain(packs, trace)
+    with capi.Context(0) as ctx:
+        device, cus = ctx.device_info()
+    return {
+        "metric": "served Msamples/s on a Poisson job trace, with p50/p99 time to first denoised frame and to "
+                  "target samples, vs the plain one-job-at-a-time path, 1 MI355X",
+        "value": served["Msamples_per_s"],
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": len(trace),
+        "warmup": 0,
+        "ms_per_step": round(served["wall_s"] * 1e3 / max(len(trace), 1), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Poisson arrivals over a pool of seeded scenes packed by the JS host modules",
+        "config": {"workload": SERVING, "rate_jobs_per_s": args.rate, "duration_s": args.duration,
+                   "spp_target": [args.spp_min, args.spp_max], "bounces": [4, 12],
+                   "sizes": [f"{w}x{h}" for w, h in SERVING_SIZES],
+                   "scene_primitives": [p.primitive_count for p in packs], "slots": args.slots,
+                   "memory_budget_bytes": budget, "device": device, "compute_units": cus,
+                   "path": "served: the plain FP32 path in concurrent slots, denoising a job's first and last "
+                           "frames; plain: one context, one job at a time, every frame denoised"},
+        "served": served,
+        "served_one_slot": one_slot,
+        "plain": plain,
+        "speedup_vs_plain": round(served["Msamples_per_s"] / max(plain["Msamples_per_s"], 1e-9), 3),
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,7 +419,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--stride", type=int, default=0, help="pixel subsample stride for algorithmic bytes")
-    ap.add_argument("--workload", choices=[*WORKLOADS, RAYCAST], default="spheres200-1920x1080")
+    ap.add_argument("--workload", choices=[*WORKLOADS, RAYCAST, SERVING], default="spheres200-1920x1080")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: job arrivals per second")
+    ap.add_argument("--duration", type=float, default=15.0, help="serving: seconds of arrivals")
+    ap.add_argument("--spp-min", type=int, default=4, help="serving: smallest target spp")
+    ap.add_argument("--spp-max", type=int, default=64, help="serving: largest target spp")
+    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent job slots")
     ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent"], default="per-pixel",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
                          "or lanes regenerating paths from a tile queue")
@@ -283,7 +433,12 @@ def main() -> int:
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).", file=sys.stderr)
         return 2
-    line = bench_raycast(args) if args.workload == RAYCAST else bench_frames(args, args.workload)
+    if args.workload == RAYCAST:
+        line = bench_raycast(args)
+    elif args.workload == SERVING:
+        line = bench_serving(args)
+    else:
+        line = bench_frames(args, args.workload)
     print(json.dumps(line))
     return 0
 

@@ -148,7 +148,9 @@ int rm_device_info(rm_context* ctx, char* name, size_t name_len, int* compute_un
  * a scene only after its jobs are done. A job renders frames exactly as
  * rm_frame does on a context of its own (sample offsets, running totals and
  * the photon radius advancing as the Renderer advances them), so its image is
- * the same whatever else is running. No counterpart in the reference, which
+ * the same whatever else is running. The denoiser, which reads only the
+ * current accumulation, runs on a job's first frame and its last only. No
+ * counterpart in the reference, which
  * renders one image per Renderer; this is the serving loop its callers would
  * otherwise write around Renderer.render. */
 typedef struct rm_server rm_server;

@@ -96,11 +96,18 @@ struct rm_server {
 
 namespace {
 
+// Whether frame k of a job runs the denoiser: only its first frame (the
+// time-to-first-image a client sees) and its last (the result). The denoiser
+// reads nothing but the current accumulation and guides, so skipping it in
+// between leaves the final image unchanged.
+bool denoises(const Job& j, uint32_t k) { return j.denoise && (k == 0 || k + 1 == j.frames_total); }
+
 // The parameter block of a job's frame k, advanced as the Renderer advances
 // it: sample offset and running total, and the photon radius shrinking as
 // (k + 1)^(-1/6).
 void frame_params(const Job& j, uint32_t k, uint32_t* out) {
     std::memcpy(out, j.params, RM_PARAMS_BYTES);
+    out[P_DENOISE] = denoises(j, k) ? 1u : 0u;
     out[P_SAMPLE_OFFSET] = k * j.spp;
     out[P_TOTAL] = (k + 1) * j.spp;
     float r = (float)((double)j.radius0 * std::pow((double)(k + 1), -1.0 / 6.0));
@@ -194,7 +201,7 @@ bool rm_server::progress(Slot& s) {
         }
         // The frame is complete when its last stream is: the denoiser's when
         // it runs (it follows the path pass, resolve and prepare), else the path stream's.
-        hipStream_t last = j.denoise ? s.ctx->post : s.ctx->stream;
+        hipStream_t last = denoises(j, j.frames_enqueued) ? s.ctx->post : s.ctx->stream;
         hipError_t e = hipEventRecord(s.frame_ev[(s.head + s.in_flight) & 1u], last);
         if (e != hipSuccess) {
             finish(s, RM_ERR_DEVICE, std::string("recording a frame event: ") + hipGetErrorString(e));

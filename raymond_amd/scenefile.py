@@ -76,6 +76,21 @@ def set_params(params: np.ndarray, **fields) -> np.ndarray:
     return params
 
 
+def resized_params(params: np.ndarray, width: int, height: int, max_bounces: int | None = None) -> np.ndarray:
+    """The parameter block for the same scene and camera at another image
+    size: size, aspect and the starting photon radius (about three pixels
+    across at the delta objects, so inversely proportional to the height;
+    raymond_amd/js/frame.mjs writeParams / writeFrameParams), and optionally
+    the bounce limit."""
+    out = params.copy()
+    h0 = get_param(params, "height")
+    r = max(float(get_param(params, "photon_radius")) * h0 / height, 1e-5)
+    fields = dict(width=width, height=height, aspect=width / height, photon_radius=r, photon_cell=2 * r)
+    if max_bounces is not None:
+        fields["max_bounces"] = max_bounces
+    return set_params(out, **fields)
+
+
 def get_param(params: np.ndarray, name: str):
     word, kind = PARAM_FIELDS[name]
     return struct.unpack("<I" if kind == "u" else "<f", params[word * 4:word * 4 + 4].tobytes())[0]
