@@ -225,6 +225,9 @@ typedef struct {
     int buffer;             /* -1: synchronize only */
     float* dst;
     size_t floats;
+    rm_server* server;      /* non-null: wait for job `job` of this server */
+    uint64_t job;
+    rm_job_stats stats;
     int result;
     char message[512];
 } AsyncJob;
@@ -232,14 +235,33 @@ typedef struct {
 static void job_execute(napi_env env, void* data) {
     (void)env;
     AsyncJob* j = (AsyncJob*)data;
-    j->result = j->buffer < 0 ? rm_synchronize(j->ctx) : rm_read(j->ctx, j->buffer, j->dst, j->floats);
+    if (j->server)
+        j->result = rm_server_wait(j->server, j->job, j->dst, j->floats, &j->stats);
+    else
+        j->result = j->buffer < 0 ? rm_synchronize(j->ctx) : rm_read(j->ctx, j->buffer, j->dst, j->floats);
     if (j->result != RM_OK) snprintf(j->message, sizeof j->message, "%s (code %d)", rm_last_error(), j->result);
+}
+
+static napi_value stats_object(napi_env env, const rm_job_stats* st) {
+    napi_value o, v;
+    napi_create_object(env, &o);
+    napi_create_double(env, st->queued_ms, &v);
+    napi_set_named_property(env, o, "queuedMs", v);
+    napi_create_double(env, st->first_frame_ms, &v);
+    napi_set_named_property(env, o, "firstFrameMs", v);
+    napi_create_double(env, st->done_ms, &v);
+    napi_set_named_property(env, o, "doneMs", v);
+    napi_create_uint32(env, st->samples, &v);
+    napi_set_named_property(env, o, "samples", v);
+    napi_create_uint32(env, st->frames, &v);
+    napi_set_named_property(env, o, "frames", v);
+    return o;
 }
 
 static void job_complete(napi_env env, napi_status status, void* data) {
     AsyncJob* j = (AsyncJob*)data;
     if (status == napi_ok && j->result == RM_OK) {
-        napi_resolve_deferred(env, j->deferred, undefined(env));
+        napi_resolve_deferred(env, j->deferred, j->server ? stats_object(env, &j->stats) : undefined(env));
     } else {
         napi_value msg, err;
         napi_create_string_utf8(env, j->result != RM_OK ? j->message : "The GPU wait was cancelled.", NAPI_AUTO_LENGTH,
@@ -328,6 +350,98 @@ static napi_value js_device_info(napi_env env, napi_callback_info info) {
     return obj;
 }
 
+/* ---- the job server ---- */
+
+static napi_value js_server_open(napi_env env, napi_callback_info info) {
+    napi_value argv[3];
+    uint32_t device, slots;
+    double budget;
+    if (!args(env, info, 3, argv) || !get_u32(env, argv[0], &device) || !get_u32(env, argv[1], &slots)) return NULL;
+    if (napi_get_value_double(env, argv[2], &budget) != napi_ok || !(budget >= 0.0)) {
+        napi_throw_type_error(env, NULL, "Expected a memory budget in bytes.");
+        return NULL;
+    }
+    rm_server* sv = NULL;
+    int r = rm_server_open((int)device, slots, (uint64_t)budget, &sv);
+    if (r != RM_OK) return throw_rm(env, r);
+    return make_external(env, sv);
+}
+
+static napi_value js_server_context(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    void* sv;
+    if (!args(env, info, 1, argv) || !get_ptr(env, argv[0], &sv)) return NULL;
+    return make_external(env, rm_server_context((rm_server*)sv));
+}
+
+/* serverSubmit(server, scene, params, targetSamples, emitters|null, envRgba|null, envWidth, envHeight,
+ *              envTable|null, tableWidth, tableHeight) → job id (a Number) */
+static napi_value js_server_submit(napi_env env, napi_callback_info info) {
+    napi_value argv[11];
+    void *sv, *scene, *params, *emitters, *rgba, *table;
+    size_t pb, eb, rb, tb;
+    uint32_t target, ew, eh, tw, th;
+    if (!args(env, info, 11, argv) || !get_ptr(env, argv[0], &sv) || !get_ptr(env, argv[1], &scene) ||
+        !get_bytes(env, argv[2], &params, &pb) || !get_u32(env, argv[3], &target) ||
+        !get_bytes(env, argv[4], &emitters, &eb) || !get_bytes(env, argv[5], &rgba, &rb) ||
+        !get_u32(env, argv[6], &ew) || !get_u32(env, argv[7], &eh) || !get_bytes(env, argv[8], &table, &tb) ||
+        !get_u32(env, argv[9], &tw) || !get_u32(env, argv[10], &th))
+        return NULL;
+    if (pb < RM_PARAMS_BYTES) {
+        napi_throw_range_error(env, NULL, "The parameter block is shorter than 208 bytes.");
+        return NULL;
+    }
+    rm_job job;
+    memset(&job, 0, sizeof job);
+    job.scene = (const rm_scene*)scene;
+    job.params = params;
+    job.emitters = (const float*)emitters;
+    job.emitter_vec4s = (uint32_t)(eb / 16);
+    if (rgba) {
+        job.env_rgba = (const uint16_t*)rgba;
+        job.env_width = ew;
+        job.env_height = eh;
+        job.env_table = (const float*)table;
+        job.table_width = tw;
+        job.table_height = th;
+    }
+    job.target_samples = target;
+    uint64_t id = 0;
+    int r = rm_server_submit((rm_server*)sv, &job, &id);
+    if (r != RM_OK) return throw_rm(env, r);
+    napi_value out;
+    CHECK(napi_create_double(env, (double)id, &out));
+    return out;
+}
+
+/* serverWait(server, id, Float32Array|null) → Promise of the job's timings */
+static napi_value js_server_wait(napi_env env, napi_callback_info info) {
+    napi_value argv[3];
+    void *sv, *dst;
+    size_t bytes;
+    double id;
+    if (!args(env, info, 3, argv) || !get_ptr(env, argv[0], &sv) || !get_bytes(env, argv[2], &dst, &bytes)) return NULL;
+    if (napi_get_value_double(env, argv[1], &id) != napi_ok) {
+        napi_throw_type_error(env, NULL, "Expected a job id.");
+        return NULL;
+    }
+    AsyncJob* j = (AsyncJob*)calloc(1, sizeof *j);
+    j->server = (rm_server*)sv;
+    j->job = (uint64_t)id;
+    j->dst = (float*)dst;
+    j->floats = bytes / 4;
+    return start_job(env, j, dst ? argv[2] : NULL);
+}
+
+static napi_value js_server_close(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    void* sv;
+    if (!args(env, info, 1, argv) || !get_ptr(env, argv[0], &sv)) return NULL;
+    int r = rm_server_close((rm_server*)sv);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor props[] = {
         {"open", NULL, js_open, NULL, NULL, NULL, napi_enumerable, NULL},
@@ -342,6 +456,11 @@ static napi_value init(napi_env env, napi_value exports) {
         {"synchronize", NULL, js_synchronize, NULL, NULL, NULL, napi_enumerable, NULL},
         {"raycast", NULL, js_raycast, NULL, NULL, NULL, napi_enumerable, NULL},
         {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"serverOpen", NULL, js_server_open, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"serverContext", NULL, js_server_context, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"serverSubmit", NULL, js_server_submit, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"serverWait", NULL, js_server_wait, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"serverClose", NULL, js_server_close, NULL, NULL, NULL, napi_enumerable, NULL},
     };
     if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return NULL;
     return exports;

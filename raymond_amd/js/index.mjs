@@ -8,3 +8,4 @@ export { Camera } from "./camera.mjs";
 export { BVH } from "./bvh.mjs";
 export { Scene, PRIMITIVE_KINDS, NONE, sphereUv, boxUv, bump } from "./scene.mjs";
 export { Renderer } from "./renderer.mjs";
+export { RenderServer } from "./server.mjs";

@@ -4,7 +4,8 @@
 // parameter block layout (shaders.test.mjs:55-80), materials and the
 // environment. Prints one line per test and exits non-zero on a failure.
 import {
-    BVH, Box, Camera, EnvironmentMap, Mat4, Material, Mesh, Quad, Renderer, Scene, Sky, Sphere, Texture, Vec2, Vec3,
+    BVH, Box, Camera, EnvironmentMap, Mat4, Material, Mesh, Quad, Renderer, RenderServer, Scene, Sky, Sphere, Texture, Vec2,
+    Vec3,
 } from "../../raymond_amd/js/index.mjs";
 import { lightChances, toHalf, writeParams, writeFrameParams, sceneTables, deltaObjects, emitters } from "../../raymond_amd/js/frame.mjs";
 
@@ -256,6 +257,13 @@ test("Renderer.create rejects with an Error when no GPU can be opened", async ()
     if (process.env.RAYMOND_EXPECT_GPU) return;
     let threw = null;
     try { await Renderer.create({ width: 4, height: 4 }); } catch (e) { threw = e; }
+    ok(threw instanceof Error, "create() should reject");
+});
+
+test("RenderServer.create rejects with an Error when no GPU can be opened", async () => {
+    if (process.env.RAYMOND_EXPECT_GPU) return;
+    let threw = null;
+    try { await RenderServer.create({ slots: 2 }); } catch (e) { threw = e; }
     ok(threw instanceof Error, "create() should reject");
 });
 

@@ -46,7 +46,8 @@ def test_addon_exports():
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().split(",") == sorted(
         ["open", "close", "uploadScene", "releaseScene", "uploadTextures", "uploadEnvironment", "resize", "frame",
-         "read", "synchronize", "raycast", "deviceInfo"])
+         "read", "synchronize", "raycast", "deviceInfo", "serverOpen", "serverContext", "serverSubmit", "serverWait",
+         "serverClose"])
 
 
 def test_oracle_is_not_linked_into_the_product():

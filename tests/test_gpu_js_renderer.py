@@ -32,3 +32,16 @@ def test_renderer_matches_c_abi(ctx, pack, tmp_path):
     acc, _, _, d = gpu_render(ctx, p, frames, params, denoised=True)
     np.testing.assert_array_equal(avg, acc / frames)
     np.testing.assert_array_equal(den, d)
+
+
+def test_render_server_matches_renderer():
+    """RenderServer jobs (one in a mix, one a copy, one without the denoiser)
+    give the Renderer's images for the same frames, bit for bit."""
+    res = subprocess.run(["node", str(ROOT / "tests" / "js" / "server_gpu.mjs")], capture_output=True, text=True,
+                         timeout=180, env=dict(os.environ, RAYMOND_EXPECT_GPU="1"))
+    assert res.returncode == 0, res.stdout + res.stderr
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    assert out["denoisedFirst"] and out["denoisedSecond"] and out["average"], out
+    for st in out["stats"]:
+        assert 0 <= st["queuedMs"] <= st["firstFrameMs"] <= st["doneMs"], st
+    assert [st["samples"] for st in out["stats"]] == [5, 3, 5, 5]
