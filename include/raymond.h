@@ -112,6 +112,12 @@ int rm_frame(rm_context* ctx, const rm_scene* scene, const void* params, const f
  * Replaces readPixels' copy and map (src/Renderer.mjs:502-528). */
 int rm_read(rm_context* ctx, int buffer, float* dst, size_t floats);
 
+/* Runs the display pass on the last frame and copies the image: W·H RGBA8
+ * pixels, what render() draws to the canvas (the view's tone mapping, sRGB,
+ * each channel rounded to 8 bits).
+ * Replaces DISPLAY_SHADER's draw (src/shaders.mjs:2100-2152, src/Renderer.mjs:476-487). */
+int rm_display(rm_context* ctx, uint8_t* rgba, size_t bytes);
+
 /* Waits for all queued work (device.queue.onSubmittedWorkDone()). */
 int rm_synchronize(rm_context* ctx);
 

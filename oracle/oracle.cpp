@@ -1837,6 +1837,49 @@ void oracle_compose(const uint8_t* params, const float* guide_albedo, const floa
     }
 }
 
+// DISPLAY_SHADER's fragment (src/shaders.mjs:2100-2152): the averaged
+// accumulation (or the denoised average), exposed, Narkowicz's ACES fit and
+// sRGB for the colour view; sRGB for albedo; n·0.5+0.5 for normals; depth as
+// 1/(1 + d/focus). Written as RGBA8, each channel round(clamp(x, 0, 1)·255)
+// as an 8-bit unorm target stores it.
+static float aces1(float x) {
+    float v = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+    return std::fmin(std::fmax(v, 0.0f), 1.0f);
+}
+static float srgb1(float c) {
+    float x = std::fmin(std::fmax(c, 0.0f), 1.0f);
+    return x <= 0.0031308f ? x * 12.92f : 1.055f * std::pow(x, 1.0f / 2.4f) - 0.055f;
+}
+static uint8_t unorm8(float x) {
+    float c = std::fmin(std::fmax(x, 0.0f), 1.0f);
+    return (uint8_t)std::nearbyint(c * 255.0f);
+}
+
+void oracle_display(const uint8_t* params, const float* accum, const float* denoised, uint8_t* out) {
+    const Params& P = *reinterpret_cast<const Params*>(params);
+    for (uint32_t i = 0; i < P.width * P.height; ++i) {
+        uint8_t* o = out + (size_t)i * 4;
+        float rgb[3] = {0.0f, 0.0f, 0.0f};
+        if (P.total_samples != 0u) {
+            const float n = (float)P.total_samples;
+            V3 c = xyz(accum + (size_t)i * 4) / n;
+            if (P.view == 0u) {
+                V3 shown = P.denoise != 0u ? xyz(denoised + (size_t)i * 4) : c;
+                V3 e = shown * P.exposure;
+                rgb[0] = srgb1(aces1(e.x)); rgb[1] = srgb1(aces1(e.y)); rgb[2] = srgb1(aces1(e.z));
+            } else if (P.view == 1u) {
+                rgb[0] = srgb1(c.x); rgb[1] = srgb1(c.y); rgb[2] = srgb1(c.z);
+            } else if (P.view == 2u) {
+                rgb[0] = c.x * 0.5f + 0.5f; rgb[1] = c.y * 0.5f + 0.5f; rgb[2] = c.z * 0.5f + 0.5f;
+            } else {
+                float g = c.x <= 0.0f ? 0.0f : 1.0f / (1.0f + c.x / P.focus_distance);
+                rgb[0] = rgb[1] = rgb[2] = g;
+            }
+        }
+        o[0] = unorm8(rgb[0]); o[1] = unorm8(rgb[1]); o[2] = unorm8(rgb[2]); o[3] = 255;
+    }
+}
+
 void oracle_intersect(const OracleScene* s, uint32_t prim, const float* ray, float t_max, float* out) {
     Scene S = view(s);
     V3 h = intersect(S, prim, {ray[0], ray[1], ray[2]}, {ray[4], ray[5], ray[6]}, t_max);

@@ -82,6 +82,9 @@ void oracle_atrous(const uint8_t* params, uint32_t step, const float* guide_norm
 void oracle_compose(const uint8_t* params, const float* guide_albedo, const float* light_in,
                     float* out);
 
+/* DISPLAY_SHADER (src/shaders.mjs:2100-2152) as RGBA8 per pixel. */
+void oracle_display(const uint8_t* params, const float* accum, const float* denoised, uint8_t* out);
+
 /* Building blocks, exposed for known-answer and property tests. */
 uint32_t oracle_pcg(uint32_t v);
 uint32_t oracle_owen(uint32_t x, uint32_t seed);

@@ -53,6 +53,7 @@ def load() -> C.CDLL:
             "oracle_prepare": (None, [P, P, P, P, P]),
             "oracle_atrous": (None, [P, U, P, P, P]),
             "oracle_compose": (None, [P, P, P, P]),
+            "oracle_display": (None, [P, P, P, P]),
             "oracle_pcg": (U, [U]),
             "oracle_owen": (U, [U, U]),
             "oracle_sobol2": (U, [U]),
@@ -179,6 +180,17 @@ class Oracle:
         r = np.ascontiguousarray(ray, np.float32)
         self.lib.oracle_intersect(self.ref, prim, _p(r), t_max, _p(out))
         return out
+
+
+def display(params: np.ndarray, accum: np.ndarray, denoised: np.ndarray | None = None) -> np.ndarray:
+    """DISPLAY_SHADER as RGBA8 (oracle_display) for an accumulation and, when
+    the parameters denoise, the denoised average."""
+    p = np.ascontiguousarray(params, dtype=np.uint8)
+    a = np.ascontiguousarray(accum, dtype=np.float32)
+    d = np.ascontiguousarray(a if denoised is None else denoised, dtype=np.float32)
+    out = np.empty(a.shape[:-1] + (4,), np.uint8)
+    load().oracle_display(_p(p), _p(a), _p(d), _p(out))
+    return out
 
 
 def pcg(v: int) -> int:

@@ -222,9 +222,9 @@ typedef struct {
     napi_deferred deferred;
     napi_ref keep;          /* the destination array, alive until done */
     rm_context* ctx;
-    int buffer;             /* -1: synchronize only */
+    int buffer;             /* -1: synchronize only; -2: the display pass into `dst` as bytes */
     float* dst;
-    size_t floats;
+    size_t floats;          /* for the display pass: bytes */
     rm_server* server;      /* non-null: wait for job `job` of this server */
     uint64_t job;
     rm_job_stats stats;
@@ -237,6 +237,8 @@ static void job_execute(napi_env env, void* data) {
     AsyncJob* j = (AsyncJob*)data;
     if (j->server)
         j->result = rm_server_wait(j->server, j->job, j->dst, j->floats, &j->stats);
+    else if (j->buffer == -2)
+        j->result = rm_display(j->ctx, (uint8_t*)j->dst, j->floats);
     else
         j->result = j->buffer < 0 ? rm_synchronize(j->ctx) : rm_read(j->ctx, j->buffer, j->dst, j->floats);
     if (j->result != RM_OK) snprintf(j->message, sizeof j->message, "%s (code %d)", rm_last_error(), j->result);
@@ -302,6 +304,20 @@ static napi_value js_read(napi_env env, napi_callback_info info) {
     j->dst = (float*)dst;
     j->floats = bytes / 4;
     return start_job(env, j, argv[2]);
+}
+
+/* display(ctx, Uint8Array of W·H·4) → Promise: the display pass on the last frame. */
+static napi_value js_display(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    void *ctx, *dst;
+    size_t bytes;
+    if (!args(env, info, 2, argv) || !get_ptr(env, argv[0], &ctx) || !get_bytes(env, argv[1], &dst, &bytes)) return NULL;
+    AsyncJob* j = (AsyncJob*)calloc(1, sizeof *j);
+    j->ctx = (rm_context*)ctx;
+    j->buffer = -2;
+    j->dst = (float*)dst;
+    j->floats = bytes;
+    return start_job(env, j, argv[1]);
 }
 
 static napi_value js_synchronize(napi_env env, napi_callback_info info) {
@@ -454,6 +470,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"frame", NULL, js_frame, NULL, NULL, NULL, napi_enumerable, NULL},
         {"read", NULL, js_read, NULL, NULL, NULL, napi_enumerable, NULL},
         {"synchronize", NULL, js_synchronize, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"display", NULL, js_display, NULL, NULL, NULL, napi_enumerable, NULL},
         {"raycast", NULL, js_raycast, NULL, NULL, NULL, napi_enumerable, NULL},
         {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverOpen", NULL, js_server_open, NULL, NULL, NULL, napi_enumerable, NULL},

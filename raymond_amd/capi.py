@@ -32,6 +32,7 @@ SIGNATURES = {
     "rm_frame": (C.c_int, [_P, _P, _P, _P, _U32]),
     "rm_read": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rm_synchronize": (C.c_int, [_P]),
+    "rm_display": (C.c_int, [_P, _P, C.c_size_t]),
     "rm_raycast": (C.c_int, [_P, _P, _P, _U32, C.c_int, _P, _P]),
     "rm_timing": (C.c_int, [_P, C.c_int]),
     "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
@@ -161,6 +162,12 @@ class Context:
     def read(self, buffer: int = BUFFER_ACCUM) -> np.ndarray:
         out = np.empty((self.height, self.width, 4), dtype=np.float32)
         _check(load().rm_read(self.handle, buffer, _ptr(out), out.size), "rm_read")
+        return out
+
+    def display(self) -> np.ndarray:
+        """The display pass on the last frame: (H, W, 4) uint8 RGBA."""
+        out = np.empty((self.height, self.width, 4), dtype=np.uint8)
+        _check(load().rm_display(self.handle, _ptr(out), out.size), "rm_display")
         return out
 
     def synchronize(self) -> None:

@@ -192,7 +192,7 @@ int rm_close(rm_context* c) {
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->albedo_frame, &c->heads,
-                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging})
+                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging, &c->display})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
@@ -475,6 +475,8 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         c->denoise_pending = true;
         RM_HIP(hipGetLastError(), "launching the denoiser");
     }
+    c->last_params = P;
+    c->rendered = true;
     return RM_OK;
 }
 
@@ -495,6 +497,25 @@ int rm_read(rm_context* c, int buffer, float* dst, size_t floats) {
     if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "waiting for the denoiser");
     RM_HIP(hipMemcpyAsync(dst, b->ptr, floats * 4, hipMemcpyDeviceToHost, c->stream), "reading back pixels");
     RM_HIP(hipStreamSynchronize(c->stream), "reading back pixels");
+    return RM_OK;
+}
+
+int rm_display(rm_context* c, uint8_t* rgba, size_t bytes) {
+    if (!c || !rgba) return fail(RM_ERR_INVALID, "rm_display needs a context and a destination.");
+    if (!c->rendered) return fail(RM_ERR_STATE, "Nothing has been rendered yet.");
+    const rm::Params& P = c->last_params;
+    const size_t need = (size_t)P.width * P.height * 4;
+    if (P.width != c->width || P.height != c->height)
+        return fail(RM_ERR_STATE, "The image was resized after the last frame.");
+    if (bytes < need) return fail(RM_ERR_INVALID, "rm_display needs W·H·4 bytes.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    RM_HIP(c->display.ensure(need), "allocating the display image");
+    if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "waiting for the denoiser");
+    rm::launch_display(P, c->accum.as<const float4>(), c->denoised.as<const float4>(), c->display.as<uchar4>(),
+                       c->stream);
+    RM_HIP(hipGetLastError(), "launching the display pass");
+    RM_HIP(hipMemcpyAsync(rgba, c->display.ptr, need, hipMemcpyDeviceToHost, c->stream), "reading back the display");
+    RM_HIP(hipStreamSynchronize(c->stream), "waiting for the GPU");
     return RM_OK;
 }
 

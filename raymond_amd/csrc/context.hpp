@@ -76,7 +76,9 @@ struct rm_context {
     rmi::DevBuf heads, photons, emitters, spill, side_spill, gathered;
     rmi::DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
-    rmi::DevBuf rays, hits, counters, work, staging;
+    rmi::DevBuf rays, hits, counters, work, staging, display;
+    rm::Params last_params = {};   // the last frame's, for rm_display
+    bool rendered = false;
     int compute_units = 0;
     int64_t options[RM_OPTION_COUNT] = {0, 0};
     bool timing = false;

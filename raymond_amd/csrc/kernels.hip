@@ -1101,6 +1101,43 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_compose(Params P, const float
     out[i] = make_float4(r.x, r.y, r.z, 0.0f);
 }
 
+// DISPLAY_SHADER's fragment (src/shaders.mjs:2100-2152) for every pixel at
+// once: what render() draws to the canvas, as RGBA8 (each channel
+// round(clamp(x, 0, 1)·255), as an 8-bit unorm target stores it).
+RM_D float aces1(float x) {
+    return clampf((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f), 0.0f, 1.0f);
+}
+RM_D float srgb1(float c) {
+    const float x = clampf(c, 0.0f, 1.0f);
+    return x <= 0.0031308f ? x * 12.92f : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+}
+RM_D uint32_t unorm8(float x) { return (uint32_t)rintf(clampf(x, 0.0f, 1.0f) * 255.0f); }
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_display(Params P, const float4* __restrict__ accum,
+                                                           const float4* __restrict__ denoised,
+                                                           uchar4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    if (i >= P.width * P.height) return;
+    V3 rgb = {0.0f, 0.0f, 0.0f};
+    if (P.total_samples != 0u) {
+        const float4 a = accum[i];
+        const V3 c = v3(a.x, a.y, a.z) / (float)P.total_samples;
+        if (P.view == VIEW_COLOR) {
+            const float4 d = denoised[i];
+            const V3 e = (P.denoise != 0u ? v3(d.x, d.y, d.z) : c) * P.exposure;
+            rgb = {srgb1(aces1(e.x)), srgb1(aces1(e.y)), srgb1(aces1(e.z))};
+        } else if (P.view == VIEW_ALBEDO) {
+            rgb = {srgb1(c.x), srgb1(c.y), srgb1(c.z)};
+        } else if (P.view == VIEW_NORMAL) {
+            rgb = {c.x * 0.5f + 0.5f, c.y * 0.5f + 0.5f, c.z * 0.5f + 0.5f};
+        } else {
+            const float g = c.x <= 0.0f ? 0.0f : 1.0f / (1.0f + c.x / P.focus_distance);
+            rgb = {g, g, g};
+        }
+    }
+    out[i] = make_uchar4((unsigned char)unorm8(rgb.x), (unsigned char)unorm8(rgb.y), (unsigned char)unorm8(rgb.z), 255);
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t value, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1226,6 +1263,12 @@ void launch_prepare(const Params& P, const float4* accum, const float4* galb, co
 void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream) {
     dim3 grid(stencil_blocks(P.width, P.height));
     hipLaunchKernelGGL(k_atrous, grid, dim3(BLOCK_THREADS), 0, stream, P, step, guides, in, out);
+}
+
+void launch_display(const Params& P, const float4* accum, const float4* denoised, uchar4* out, hipStream_t stream) {
+    uint32_t n = P.width * P.height;
+    hipLaunchKernelGGL(k_display, dim3((n + BLOCK_THREADS - 1) / BLOCK_THREADS), dim3(BLOCK_THREADS), 0, stream, P,
+                       accum, denoised, out);
 }
 
 void launch_compose(const Params& P, const float4* albedo, const float4* in, float4* out, hipStream_t stream) {

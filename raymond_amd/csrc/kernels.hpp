@@ -49,6 +49,7 @@ void launch_prepare(const Params& P, const float4* accum, const float4* galb, co
                     float4* albedo_out, hipStream_t stream);
 void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream);
 void launch_compose(const Params& P, const float4* albedo, const float4* in, float4* out, hipStream_t stream);
+void launch_display(const Params& P, const float4* accum, const float4* denoised, uchar4* out, hipStream_t stream);
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 
 }  // namespace rm

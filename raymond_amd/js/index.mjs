@@ -9,3 +9,4 @@ export { BVH } from "./bvh.mjs";
 export { Scene, PRIMITIVE_KINDS, NONE, sphereUv, boxUv, bump } from "./scene.mjs";
 export { Renderer } from "./renderer.mjs";
 export { RenderServer } from "./server.mjs";
+export { encodePng } from "./png.mjs";

@@ -219,6 +219,18 @@ export class Renderer {
     }
 
     /** Frees the device memory; the renderer can't be used afterwards. */
+    /**
+     * What render() draws to the canvas for the last frame: the display
+     * pass's tone-mapped sRGB image as W·H RGBA8 bytes (src/shaders.mjs:2100-2152).
+     */
+    async readDisplay() {
+        if (this.#width == 0 || this.#samples == 0)
+            throw new Error("Nothing has been rendered yet.");
+        const pixels = new Uint8Array(this.#width * this.#height * 4);
+        await this.#addon.display(this.#ctx, pixels);
+        return pixels;
+    }
+
     dispose() {
         if (this.#disposed)
             return;

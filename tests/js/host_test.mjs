@@ -5,7 +5,7 @@
 // environment. Prints one line per test and exits non-zero on a failure.
 import {
     BVH, Box, Camera, EnvironmentMap, Mat4, Material, Mesh, Quad, Renderer, RenderServer, Scene, Sky, Sphere, Texture, Vec2,
-    Vec3,
+    Vec3, encodePng,
 } from "../../raymond_amd/js/index.mjs";
 import { lightChances, toHalf, writeParams, writeFrameParams, sceneTables, deltaObjects, emitters } from "../../raymond_amd/js/frame.mjs";
 
@@ -258,6 +258,33 @@ test("Renderer.create rejects with an Error when no GPU can be opened", async ()
     let threw = null;
     try { await Renderer.create({ width: 4, height: 4 }); } catch (e) { threw = e; }
     ok(threw instanceof Error, "create() should reject");
+});
+
+test("PNG: signature, IHDR, chunk CRCs and the pixels inflate back", async () => {
+    const { inflateSync } = await import("zlib");
+    const w = 3, h = 2, rgba = new Uint8Array(w * h * 4).map((_, i) => (i * 37) & 255);
+    const png = encodePng(w, h, rgba);
+    ok(png.subarray(0, 8).equals(Buffer.from([0x89, 0x50, 0x4e, 0x47, 0x0d, 0x0a, 0x1a, 0x0a])), "signature");
+    ok(png.readUInt32BE(16) == w && png.readUInt32BE(20) == h && png[24] == 8 && png[25] == 6, "IHDR");
+    ok(png.subarray(png.length - 4).equals(Buffer.from([0xae, 0x42, 0x60, 0x82])), "IEND's CRC is AE426082");
+    let at = 8, idat = null;
+    while (at < png.length) {
+        const len = png.readUInt32BE(at), type = png.toString("ascii", at + 4, at + 8);
+        const crc = png.readUInt32BE(at + 8 + len);
+        let c = 0xffffffff;
+        for (const b of png.subarray(at + 4, at + 8 + len)) {
+            c ^= b;
+            for (let k = 0; k < 8; ++k) c = c & 1 ? 0xedb88320 ^ (c >>> 1) : c >>> 1;
+        }
+        ok(((c ^ 0xffffffff) >>> 0) == crc, `${type} CRC`);
+        if (type == "IDAT") idat = png.subarray(at + 8, at + 8 + len);
+        at += 12 + len;
+    }
+    const raw = inflateSync(idat);
+    for (let y = 0; y < h; ++y) {
+        ok(raw[y * (w * 4 + 1)] == 0, "filter byte");
+        for (let i = 0; i < w * 4; ++i) ok(raw[y * (w * 4 + 1) + 1 + i] == rgba[y * w * 4 + i], "pixel");
+    }
 });
 
 test("RenderServer.create rejects with an Error when no GPU can be opened", async () => {

@@ -1,12 +1,13 @@
 // Drives the JS Renderer through the N-API addon on the GPU: renders the
 // Cornell box for a few frames and writes the averaged and denoised pixels to
-// the files given, for the Python side to compare with the C ABI's output.
+// the files given, and the display pass's image as a PNG, for the Python side
+// to compare with the C ABI's output.
 import { writeFileSync } from "fs";
-import { Renderer } from "../../raymond_amd/js/index.mjs";
+import { Renderer, encodePng } from "../../raymond_amd/js/index.mjs";
 import { cornell } from "../../raymond_amd/js/configs.mjs";
 
 async function main() {
-    const [, , outAvg, outDen, frames] = process.argv;
+    const [, , outAvg, outDen, frames, outPng] = process.argv;
     const { scene, camera } = cornell();
     const renderer = await Renderer.create({ width: 64, height: 64 });
     renderer.scene = scene;
@@ -21,12 +22,14 @@ async function main() {
     await renderer.device.queue.onSubmittedWorkDone();
     const avg = await renderer.readPixels();
     const den = await renderer.readPixels({ denoised: true });
+    const shown = await renderer.readDisplay();
     renderer.view = "albedo";
     let bad = false;
     try { await renderer.readPixels({ denoised: true }); } catch (e) { bad = /Only the color view/.test(e.message); }
     if (!bad) throw new Error("denoised readback outside the colour view should throw");
     writeFileSync(outAvg, Buffer.from(avg.buffer));
     writeFileSync(outDen, Buffer.from(den.buffer));
+    writeFileSync(outPng, encodePng(64, 64, shown));
     console.log(JSON.stringify({ samples: renderer.samples }));
     renderer.dispose();
 }

@@ -316,3 +316,37 @@ def test_camera_rays_follow_the_basis(pack):
     assert abs(mid[2] + 1) < 1e-3
     x = d[0, 0, 0] / -d[0, 0, 2]
     assert abs(x - (-(1 - 1 / 64) * t)) < 1e-5
+
+
+def test_display_closed_forms():
+    """DISPLAY_SHADER (src/shaders.mjs:2100-2152) restated: ACES is 0 at 0 and
+    saturates, sRGB meets its linear segment at 0.0031308 and maps 1 to 1,
+    normals map [-1, 1] to [0, 1], depth halves at the focus distance, the
+    accumulation is averaged first, and nothing rendered shows black."""
+    from raymond_amd import scenefile
+    from raymond_amd.scenefile import get_param, set_params
+    p = scenefile.pack("cornell", {"width": 8, "height": 1})
+    acc = np.zeros((1, 8, 4), np.float32)
+    values = [0.0, 0.0031308, 1.0, 1e6]
+    for i, v in enumerate(values):
+        acc[0, i, :3] = v * 4  # four samples
+    view = lambda k, **kw: pyoracle.display(set_params(p.params.copy(), total_samples=4, denoise=0, view=k, **kw), acc)
+    albedo = view(1)[0]
+    assert list(albedo[0, :3]) == [0, 0, 0]
+    # Both sRGB branches give 0.04045 at the threshold: round(0.04045 * 255) = 10.
+    assert list(albedo[1, :3]) == [10, 10, 10]
+    assert list(albedo[2, :3]) == [255, 255, 255]
+    colour = view(0, exposure=1.0)[0]
+    assert list(colour[0, :3]) == [0, 0, 0] and list(colour[3, :3]) == [255, 255, 255]
+    ramp = np.zeros((1, 8, 4), np.float32)
+    ramp[0, :, :3] = np.linspace(0.0, 4.0, 8)[:, None] * 4
+    shades = pyoracle.display(set_params(p.params.copy(), total_samples=4, denoise=0, view=0, exposure=1.0), ramp)
+    assert np.all(np.diff(shades[0, :, 0].astype(int)) >= 0), "ACES then sRGB is monotone"
+    normals = np.zeros((1, 8, 4), np.float32)
+    normals[0, 0, :3] = np.array([-1.0, 0.0, 1.0]) * 4
+    assert list(pyoracle.display(set_params(p.params.copy(), total_samples=4, view=2), normals)[0, 0, :3]) == [0, 128, 255]
+    focus = float(get_param(p.params, "focus_distance"))
+    depth = np.zeros((1, 8, 4), np.float32)
+    depth[0, 0, 0] = focus * 4
+    assert list(pyoracle.display(set_params(p.params.copy(), total_samples=4, view=3), depth)[0, :2, 0]) == [128, 0]
+    assert np.all(pyoracle.display(set_params(p.params.copy(), total_samples=0), acc)[0] == [0, 0, 0, 255])
