@@ -96,6 +96,17 @@ int rm_scene_textures(rm_context* ctx, rm_scene* scene, int normal_maps, const u
 int rm_environment(rm_context* ctx, const uint16_t* rgba_half, uint32_t width, uint32_t height, const float* table,
                    uint32_t table_width, uint32_t table_height);
 
+/* Builds a scene's BVH on the host: the tree BVH.build makes
+ * (src/BVH.mjs:57-234) — binned SAH over 32 centroid bins, leaves of at most
+ * max_leaf_size primitives unless splitting pays — node for node and bit for
+ * bit, on `threads` threads (0: one per core). No device is needed.
+ * mins/maxs: 3 doubles per primitive. nodes: room for max(1, 2·count − 1)
+ * nodes of 32 bytes (f32 min[3], u32 first, f32 max[3], u32 count);
+ * *node_count receives the number written. order: room for `count` u32;
+ * order[i] is the original index of the i-th primitive in leaf order. */
+int rm_bvh_build(const double* mins, const double* maxs, uint32_t count, uint32_t max_leaf_size, uint32_t threads,
+                 void* nodes, uint32_t node_capacity, uint32_t* node_count, uint32_t* order);
+
 /* Sizes the per-pixel buffers for width × height (reused while they fit).
  * Replaces Renderer.#resize (src/Renderer.mjs:766-799). */
 int rm_resize(rm_context* ctx, uint32_t width, uint32_t height);

@@ -366,6 +366,67 @@ static napi_value js_device_info(napi_env env, napi_callback_info info) {
     return obj;
 }
 
+/* A Float64Array's data and length (elements). */
+static int get_f64(napi_env env, napi_value v, const double** data, size_t* length) {
+    bool is_ta = false;
+    napi_typedarray_type type;
+    size_t offset;
+    napi_value ab;
+    void* base;
+    napi_is_typedarray(env, v, &is_ta);
+    if (!is_ta || napi_get_typedarray_info(env, v, &type, length, &base, &ab, &offset) != napi_ok ||
+        type != napi_float64_array) {
+        napi_throw_type_error(env, NULL, "Expected a Float64Array.");
+        return 0;
+    }
+    *data = (const double*)base;
+    return 1;
+}
+
+/* bvhBuild(mins, maxs, maxLeafSize, threads) → { nodes, nodeCount, order }:
+ * BVH.build's result (src/BVH.mjs:57-234) from the native builder. */
+static napi_value js_bvh_build(napi_env env, napi_callback_info info) {
+    napi_value argv[4], obj, nodes_ab, order_ab, order, count_v;
+    const double *mins, *maxs;
+    size_t n_min, n_max;
+    uint32_t max_leaf, threads;
+    if (!args(env, info, 4, argv) || !get_f64(env, argv[0], &mins, &n_min) || !get_f64(env, argv[1], &maxs, &n_max) ||
+        !get_u32(env, argv[2], &max_leaf) || !get_u32(env, argv[3], &threads))
+        return NULL;
+    if (n_min != n_max || n_min % 3 != 0 || n_min / 3 >= (1u << 31)) {
+        napi_throw_range_error(env, NULL, "mins and maxs need 3 values per primitive, as many of each.");
+        return NULL;
+    }
+    uint32_t count = (uint32_t)(n_min / 3), most = count > 1 ? 2 * count - 1 : 1, used = 0;
+    void* order_data;
+    CHECK(napi_create_arraybuffer(env, (size_t)count * 4, &order_data, &order_ab));
+    CHECK(napi_create_typedarray(env, napi_uint32_array, count, order_ab, 0, &order));
+    void* scratch = malloc((size_t)most * 32);
+    if (!scratch) {
+        napi_throw_error(env, NULL, "Out of memory for the BVH's nodes.");
+        return NULL;
+    }
+    int r = rm_bvh_build(mins, maxs, count, max_leaf, threads, scratch, most, &used, (uint32_t*)order_data);
+    if (r != RM_OK) {
+        free(scratch);
+        return throw_rm(env, r);
+    }
+    void* nodes_data;
+    if (napi_create_arraybuffer(env, (size_t)used * 32, &nodes_data, &nodes_ab) != napi_ok) {
+        free(scratch);
+        napi_throw_error(env, NULL, "N-API call failed: napi_create_arraybuffer");
+        return NULL;
+    }
+    memcpy(nodes_data, scratch, (size_t)used * 32);
+    free(scratch);
+    CHECK(napi_create_object(env, &obj));
+    CHECK(napi_create_uint32(env, used, &count_v));
+    CHECK(napi_set_named_property(env, obj, "nodes", nodes_ab));
+    CHECK(napi_set_named_property(env, obj, "nodeCount", count_v));
+    CHECK(napi_set_named_property(env, obj, "order", order));
+    return obj;
+}
+
 /* ---- the job server ---- */
 
 static napi_value js_server_open(napi_env env, napi_callback_info info) {
@@ -473,6 +534,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"display", NULL, js_display, NULL, NULL, NULL, napi_enumerable, NULL},
         {"raycast", NULL, js_raycast, NULL, NULL, NULL, napi_enumerable, NULL},
         {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"bvhBuild", NULL, js_bvh_build, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverOpen", NULL, js_server_open, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverContext", NULL, js_server_context, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverSubmit", NULL, js_server_submit, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -38,6 +38,10 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}", "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["kernels.hip", "api.hip", "server.hip"]
+# Host-only C++ (the BVH builder): double precision as written, no FMA contraction.
+CXX_SOURCES = ["bvh_build.cpp"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+             f"-I{ROOT / 'include'}"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -51,11 +55,14 @@ def build_hip() -> Path:
     LIB.mkdir(parents=True, exist_ok=True)
     objs = [OBJ / (s.replace(".hip", ".o")) for s in SOURCES]
     cmds = [[HIPCC, *HIP_FLAGS, "-c", str(CSRC / s), "-o", str(o)] for s, o in zip(SOURCES, objs)]
+    cxx_objs = [OBJ / (s.replace(".cpp", ".o")) for s in CXX_SOURCES]
+    cmds += [["g++", *CXX_FLAGS, "-c", str(CSRC / s), "-o", str(o)] for s, o in zip(CXX_SOURCES, cxx_objs)]
     with ThreadPoolExecutor(max_workers=len(cmds)) as pool:
         list(pool.map(_run, cmds))
     out = LIB / "libraymond_hip.so"
     tmp = LIB / "libraymond_hip.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs + cxx_objs), "-o",
+          str(tmp)])
     tmp.replace(out)
     return out
 

@@ -33,6 +33,7 @@ SIGNATURES = {
     "rm_read": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rm_synchronize": (C.c_int, [_P]),
     "rm_display": (C.c_int, [_P, _P, C.c_size_t]),
+    "rm_bvh_build": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P]),
     "rm_raycast": (C.c_int, [_P, _P, _P, _U32, C.c_int, _P, _P]),
     "rm_timing": (C.c_int, [_P, C.c_int]),
     "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
@@ -78,6 +79,24 @@ def _ptr(a: np.ndarray | None):
 def _check(code: int, what: str) -> None:
     if code != RM_OK:
         raise RaymondError(f"{what}: {load().rm_last_error().decode()} (code {code})")
+
+
+def bvh_build(mins: np.ndarray, maxs: np.ndarray, max_leaf_size: int = 4, threads: int = 0):
+    """The scene BVH BVH.build makes (src/BVH.mjs:57-234), built natively on
+    the host: (nodes as (N, 8) u32 — f32 min[3], first, f32 max[3], count —
+    and the leaf order)."""
+    lo = np.ascontiguousarray(mins, dtype=np.float64).reshape(-1)
+    hi = np.ascontiguousarray(maxs, dtype=np.float64).reshape(-1)
+    if lo.size != hi.size or lo.size % 3:
+        raise ValueError("mins and maxs need 3 values per primitive, as many of each")
+    n = lo.size // 3
+    most = max(1, 2 * n - 1)
+    nodes = np.empty((most, 8), dtype=np.uint32)
+    order = np.empty(n, dtype=np.uint32)
+    used = C.c_uint32(0)
+    _check(load().rm_bvh_build(_ptr(lo), _ptr(hi), n, max_leaf_size, threads, _ptr(nodes), most, C.byref(used),
+                               _ptr(order)), "rm_bvh_build")
+    return nodes[:used.value].copy(), order
 
 
 class Scene:

@@ -1,0 +1,24 @@
+// Loads raymond.node, the N-API addon over libraymond_hip.so (both built in
+// raymond_amd/lib). Loading needs no GPU: the BVH builder runs on the host,
+// and the device calls fail with an Error when no GPU is visible.
+import { createRequire } from "module";
+import { dirname, join } from "path";
+import { fileURLToPath } from "url";
+
+const here = dirname(fileURLToPath(import.meta.url));
+let addonCache = null;
+
+/** The addon, loaded once; throws an Error naming the path when it can't load. */
+export function loadAddon() {
+    if (addonCache)
+        return addonCache;
+    const require = createRequire(import.meta.url);
+    const path = process.env.RAYMOND_ADDON || join(here, "..", "lib", "raymond.node");
+    try {
+        addonCache = require(path);
+    }
+    catch (error) {
+        throw new Error(`The Raymond HIP addon isn't built or can't load (${path}): ${error.message}`);
+    }
+    return addonCache;
+}

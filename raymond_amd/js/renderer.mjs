@@ -4,35 +4,16 @@
 // the frame itself (trace, light tracing, photon gather, denoise) runs as HIP
 // kernels in libraymond_hip.so, reached through the N-API addon raymond.node.
 // There is no other backend: without the addon or a GPU, create() rejects.
-import { createRequire } from "module";
-import { dirname, join } from "path";
-import { fileURLToPath } from "url";
+import { loadAddon } from "./addon.mjs";
 import {
     IMAGE_KEY_BYTES, PARAMS_BYTES, VIEWS, deltaObjects, emitters, environmentData,
     sceneTables, textureLayers, writeFrameParams, writeParams,
 } from "./frame.mjs";
 
-const here = dirname(fileURLToPath(import.meta.url));
-
 /** Buffers the reader reads back: accumulation, guides, denoised result. */
 export const BUFFERS = Object.freeze({ accum: 0, albedo: 1, normal: 2, denoised: 3 });
 
-let addonCache = null;
-
-/** Loads raymond.node (built beside libraymond_hip.so in raymond_amd/lib). */
-export function loadAddon() {
-    if (addonCache)
-        return addonCache;
-    const require = createRequire(import.meta.url);
-    const path = process.env.RAYMOND_ADDON || join(here, "..", "lib", "raymond.node");
-    try {
-        addonCache = require(path);
-    }
-    catch (error) {
-        throw new Error(`The Raymond HIP addon isn't built or can't load (${path}): ${error.message}`);
-    }
-    return addonCache;
-}
+export { loadAddon };
 
 /**
  * Everything a frame needs from the host, computed without a device: the

@@ -8,6 +8,7 @@ import {
     Vec3, encodePng,
 } from "../../raymond_amd/js/index.mjs";
 import { lightChances, toHalf, writeParams, writeFrameParams, sceneTables, deltaObjects, emitters } from "../../raymond_amd/js/frame.mjs";
+import * as configs from "../../raymond_amd/js/configs.mjs";
 
 let failed = 0;
 const tests = [];
@@ -97,6 +98,95 @@ test("two clusters are separated at the root", () => {
 test("an empty tree is one node of no size", () => {
     const bvh = BVH.build(new Float64Array(0), new Float64Array(0));
     ok(bvh.nodeCount == 1 && new Uint32Array(bvh.nodes)[7] == 0);
+});
+
+// The native builder (rm_bvh_build through the addon) against the JS
+// builder, byte for byte: nodes and leaf order, on one thread and on all.
+function sameTree(mins, maxs, opts = {}) {
+    const js = BVH.build(mins, maxs, { ...opts, builder: "js" });
+    const bytes = (a) => Buffer.from(a.buffer, a.byteOffset, a.byteLength);
+    for (const threads of [1, 0]) {
+        const nb = BVH.build(mins, maxs, { ...opts, builder: "native", threads });
+        ok(nb.nodeCount == js.nodeCount, `${nb.nodeCount} nodes, the JS builder ${js.nodeCount}`);
+        ok(Buffer.from(nb.nodes).equals(Buffer.from(js.nodes)), `nodes differ (threads ${threads})`);
+        ok(bytes(nb.order).equals(bytes(js.order)), `leaf order differs (threads ${threads})`);
+    }
+}
+
+test("native builder: the JS tree on random boxes of every size and leaf size", () => {
+    for (const n of [0, 1, 2, 3, 4, 5, 7, 8, 9, 17, 100, 1000, 8191, 8192, 9000, 20000])
+        sameTree(...Object.values(randomBoxes(n, 100 + n)));
+    for (const maxLeafSize of [1, 2, 8, 16])
+        sameTree(...Object.values(randomBoxes(3000, maxLeafSize)), { maxLeafSize });
+    // Clustered: a few dense blobs far apart, thin slivers, boxes much larger than their spacing.
+    sameTree(...Object.values(randomBoxes(12000, 5, 1, 40)));
+    sameTree(...Object.values(randomBoxes(12000, 6, 1e4, 1e-3)));
+});
+
+test("native builder: the JS tree on 300,000 boxes (big nodes binned by every thread)", () => {
+    sameTree(...Object.values(randomBoxes(300000, 9)));
+});
+
+test("native builder: the JS tree on degenerate input", () => {
+    // Equal centroids (no split plane: median halving), on both paths.
+    for (const n of [40, 9000]) sameTree(new Float64Array(n * 3).fill(1), new Float64Array(n * 3).fill(2));
+    // Collinear, flat and point boxes; values past float range (areas overflow).
+    const r = random(21), n = 5000;
+    const line = [new Float64Array(n * 3), new Float64Array(n * 3)];
+    const flat = [new Float64Array(n * 3), new Float64Array(n * 3)];
+    const points = [new Float64Array(n * 3), new Float64Array(n * 3)];
+    const huge = [new Float64Array(n * 3), new Float64Array(n * 3)];
+    for (let i = 0; i < n; ++i) {
+        const x = Math.floor(r() * 64) * 0.25;
+        line[0][i * 3] = x; line[1][i * 3] = x + 0.5;
+        for (let a = 0; a < 3; ++a) {
+            const v = r() * 10;
+            flat[0][i * 3 + a] = v; flat[1][i * 3 + a] = a == 2 ? v : v + r();
+            points[0][i * 3 + a] = points[1][i * 3 + a] = Math.round(v);
+            huge[0][i * 3 + a] = (r() - 0.5) * 1e200; huge[1][i * 3 + a] = huge[0][i * 3 + a] + r() * 1e199;
+        }
+    }
+    for (const [mins, maxs] of [line, flat, points, huge]) sameTree(mins, maxs);
+});
+
+test("native builder: the JS tree with signed zeros, Float32Array and array input", () => {
+    const r = random(33), n = 6000;
+    const mins = new Float64Array(n * 3), maxs = new Float64Array(n * 3);
+    for (let i = 0; i < n * 3; ++i) {
+        const k = Math.floor(r() * 4);
+        mins[i] = k == 0 ? -0 : k == 1 ? 0 : -r() * 4;
+        maxs[i] = k == 0 ? (r() < 0.5 ? -0 : 0) : k == 1 ? 0 : r() * 4;
+    }
+    sameTree(mins, maxs);
+    const f32 = randomBoxes(4000, 44);
+    sameTree(Float32Array.from(f32.mins), Float32Array.from(f32.maxs));
+    sameTree(Array.from(f32.mins.subarray(0, 300)), Array.from(f32.maxs.subarray(0, 300)));
+});
+
+test("native builder: the JS tree on the configs' scenes", () => {
+    const build = BVH.build, seen = [];
+    BVH.build = (mins, maxs, o) => { seen.push([mins, maxs]); return build.call(BVH, mins, maxs, o); };
+    try {
+        configs.cornell().scene.pack();
+        configs.spheres().scene.pack();
+        configs.mixed().scene.pack();
+        configs.triangles({ grid: 96, extras: 4, envWidth: 16, envHeight: 8 }).scene.pack();
+        configs.caustics({ res: 64 }).scene.pack();
+    }
+    finally {
+        BVH.build = build;
+    }
+    ok(seen.length == 5);
+    for (const [mins, maxs] of seen) sameTree(mins, maxs);
+});
+
+test("native builder: bad arguments throw", () => {
+    let threw = false;
+    try { BVH.build(new Float64Array(3), new Float64Array(3), { builder: "gpu" }); } catch (e) { threw = e instanceof RangeError; }
+    ok(threw, "an unknown builder is a RangeError");
+    threw = false;
+    try { BVH.build(new Float64Array(6), new Float64Array(3), { builder: "native" }); } catch (e) { threw = e instanceof RangeError; }
+    ok(threw, "mismatched bounds are a RangeError");
 });
 
 test("packing: counts, material dedupe, lights and the empty scene", () => {

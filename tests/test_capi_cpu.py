@@ -46,7 +46,7 @@ def test_addon_exports():
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().split(",") == sorted(
         ["open", "close", "uploadScene", "releaseScene", "uploadTextures", "uploadEnvironment", "resize", "frame",
-         "read", "synchronize", "display", "raycast", "deviceInfo", "serverOpen", "serverContext", "serverSubmit", "serverWait",
+         "read", "synchronize", "display", "raycast", "deviceInfo", "bvhBuild", "serverOpen", "serverContext", "serverSubmit", "serverWait",
          "serverClose"])
 
 
@@ -67,3 +67,27 @@ def test_server_rejects_null_arguments_without_touching_a_device():
     assert lib.rm_server_submit(None, None, None) == -1
     assert lib.rm_server_wait(None, 1, None, 0, None) == -1
     assert lib.rm_server_close(None) == -1
+
+
+def test_bvh_build_on_the_host():
+    """rm_bvh_build needs no device: a known two-cluster split, the empty
+    tree, and its argument checks."""
+    import numpy as np
+    from raymond_amd import capi
+    lo = np.array([[0, 0, 0], [0.1, 0, 0], [10, 0, 0], [10.1, 0, 0], [10.2, 0, 0]], np.float64)
+    nodes, order = capi.bvh_build(lo, lo + 0.05, max_leaf_size=1)
+    f = nodes.view(np.float32)
+    assert nodes[0, 7] == 0 and nodes[0, 3] == 1, "the root splits into nodes 1 and 2"
+    left, right = sorted([1, 2], key=lambda k: f[k, 0])
+    assert f[left, 4] < 1 and f[right, 0] >= 10, "the clusters are apart"
+    assert sorted(order) == list(range(5))
+    assert np.allclose(f[0, :3], 0) and np.allclose(f[0, 4:7], [10.25, 0.05, 0.05])
+    nodes, order = capi.bvh_build(np.zeros((0, 3)), np.zeros((0, 3)))
+    assert nodes.shape == (1, 8) and not nodes.any() and order.size == 0
+    lib = capi.load()
+    used = ctypes.c_uint32()
+    assert lib.rm_bvh_build(None, None, 1, 4, 0, None, 1, ctypes.byref(used), None) == -1
+    buf = (ctypes.c_uint32 * 8)()
+    b = (ctypes.c_double * 6)()
+    assert lib.rm_bvh_build(b, b, 2, 4, 0, buf, 1, ctypes.byref(used), (ctypes.c_uint32 * 2)()) == -1
+    assert b"2" in lib.rm_last_error()
