@@ -156,6 +156,7 @@ def bench_frames(args, name: str) -> dict:
     device, cus = ctx.device_info()
     ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULES[args.schedule])
     ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, args.persistent_blocks)
+    ctx.set_option(capi.OPTION_PRECISION, capi.PRECISION_QUANTIZED_BVH if args.quantized_bvh else 0)
     scene = ctx.upload_scene(p)
     ctx.environment(p.env_rgba, p.meta["envWidth"], p.meta["envHeight"], p.env_table, p.meta["envTableWidth"],
                     p.meta["envTableHeight"])
@@ -207,7 +208,8 @@ def bench_frames(args, name: str) -> dict:
                    "rr_from_bounce": 3, "nee_mis": True, "light_tracing": bool(caustics),
                    "photon_gather": bool(caustics), "denoise_passes": 5, "frames_timed": args.steps,
                    "primitives": p.primitive_count, "device": device, "compute_units": cus,
-                   "path": "plain (one job, megakernel, FP32 BVH)", "schedule": args.schedule},
+                   "path": "one job, megakernel, " + ("8-bit quantized BVH" if args.quantized_bvh else "FP32 BVH"),
+                   "schedule": args.schedule},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_summary(wl["traffic"]),
                      "kernel": {"compact": "k_main_compact", "per-pixel": "k_main",
@@ -231,6 +233,7 @@ def bench_raycast(args) -> dict:
     rays = o.camera_rays()
     n = rays.shape[0]
     ctx = capi.Context(0)
+    ctx.set_option(capi.OPTION_PRECISION, capi.PRECISION_QUANTIZED_BVH if args.quantized_bvh else 0)
     device, cus = ctx.device_info()
     scene = ctx.upload_scene(p)
     hits, counters = ctx.raycast(scene, rays, counters=True)
@@ -428,6 +431,8 @@ def main() -> int:
     ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent"], default="per-pixel",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
                          "or lanes regenerating paths from a tile queue")
+    ap.add_argument("--quantized-bvh", action="store_true",
+                    help="traverse the 8-bit child-quantized tree (same hits as the FP32 tree)")
     ap.add_argument("--persistent-blocks", type=int, default=0, help="blocks for the persistent pass (0: fill the GPU)")
     args = ap.parse_args()
     if args.gpus != 1:

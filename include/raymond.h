@@ -51,7 +51,13 @@ extern "C" {
 #define RM_OPTION_SCHEDULE 0           /* how the path-tracing pass maps paths to lanes, RM_SCHEDULE_*;
                                           every schedule gives the same results, bit for bit */
 #define RM_OPTION_PERSISTENT_BLOCKS 1  /* blocks of 256 threads for RM_SCHEDULE_PERSISTENT; 0 (default): fill the GPU */
-#define RM_OPTION_COUNT 2
+#define RM_OPTION_PRECISION 2          /* RM_PRECISION_* flags; 0 (default): the plain FP32 path */
+#define RM_OPTION_COUNT 3
+
+/* Reduced- and extended-precision variants (RM_OPTION_PRECISION flags). */
+#define RM_PRECISION_QUANTIZED_BVH 1   /* traverse the 8-bit child-quantized tree (rm_bvh_quantize) in the
+                                          per-pixel path pass, rm_raycast, light tracing and photon gather;
+                                          decoded boxes enclose the FP32 ones, so no FP32 hit is missed */
 
 #define RM_SCHEDULE_PIXEL 0            /* default: one thread per pixel, each path run to its end */
 #define RM_SCHEDULE_COMPACT 1          /* a block's live paths are packed into its first waves between bounces */
@@ -106,6 +112,17 @@ int rm_environment(rm_context* ctx, const uint16_t* rgba_half, uint32_t width, u
  * order[i] is the original index of the i-th primitive in leaf order. */
 int rm_bvh_build(const double* mins, const double* maxs, uint32_t count, uint32_t max_leaf_size, uint32_t threads,
                  void* nodes, uint32_t node_capacity, uint32_t* node_count, uint32_t* order);
+
+/* The 8-bit child-quantized form of a packed tree (node_count nodes of 32
+ * bytes), written to qnodes (node_count records of 32 bytes). An inner
+ * node's record holds its FP32 min corner, a power-of-two step per axis and
+ * both children's boxes as 8-bit offsets rounded outward, so a decoded box
+ * always encloses the FP32 one; a leaf's holds its (first, count). No
+ * counterpart in the reference, whose nodes are FP32 only (src/BVH.mjs:1-16
+ * describes them); rm_scene_upload builds it for every scene, and
+ * RM_PRECISION_QUANTIZED_BVH traverses it. Fails for boxes that aren't
+ * finite or children outside their parent. No device is needed. */
+int rm_bvh_quantize(const void* nodes, uint32_t node_count, void* qnodes);
 
 /* Sizes the per-pixel buffers for width × height (reused while they fit).
  * Replaces Renderer.#resize (src/Renderer.mjs:766-799). */

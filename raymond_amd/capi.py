@@ -34,6 +34,7 @@ SIGNATURES = {
     "rm_synchronize": (C.c_int, [_P]),
     "rm_display": (C.c_int, [_P, _P, C.c_size_t]),
     "rm_bvh_build": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P]),
+    "rm_bvh_quantize": (C.c_int, [_P, C.c_uint32, _P]),
     "rm_raycast": (C.c_int, [_P, _P, _P, _U32, C.c_int, _P, _P]),
     "rm_timing": (C.c_int, [_P, C.c_int]),
     "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
@@ -46,7 +47,8 @@ SIGNATURES = {
     "rm_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t, _P]),
     "rm_server_close": (C.c_int, [_P]),
 }
-OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS = 0, 1
+OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION = 0, 1, 2
+PRECISION_QUANTIZED_BVH = 1
 SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT = 0, 1, 2
 SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT}
 
@@ -97,6 +99,14 @@ def bvh_build(mins: np.ndarray, maxs: np.ndarray, max_leaf_size: int = 4, thread
     _check(load().rm_bvh_build(_ptr(lo), _ptr(hi), n, max_leaf_size, threads, _ptr(nodes), most, C.byref(used),
                                _ptr(order)), "rm_bvh_build")
     return nodes[:used.value].copy(), order
+
+
+def bvh_quantize(nodes: np.ndarray) -> np.ndarray:
+    """The 8-bit child-quantized records of a packed tree ((N, 8) u32 in, (N, 8) u32 out)."""
+    n = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 8)
+    out = np.empty_like(n)
+    _check(load().rm_bvh_quantize(_ptr(n), n.shape[0], _ptr(out)), "rm_bvh_quantize")
+    return out
 
 
 class Scene:

@@ -84,6 +84,7 @@ rm::DevScene dev_scene(const rm_context* c, const rm_scene* s) {
     rm::DevScene S;
     S.prims = s->prims.as<const float4>();
     S.nodes = s->nodes.as<const float4>();
+    S.qnodes = s->qnodes.as<const float4>();
     S.mats = s->mats.as<const float4>();
     S.attr = s->attr.as<const float4>();
     S.lights = s->lights.as<const uint32_t>();
@@ -102,6 +103,11 @@ rm::DevScene dev_scene(const rm_context* c, const rm_scene* s) {
     S.nmap_h = s->nmap_h;
     S.nmap_layers = s->nmap_layers;
     return S;
+}
+
+// Whether traversals walk the scene's quantized tree: asked for, and built.
+bool quantized(const rm_context* c, const rm_scene* s) {
+    return (c->options[RM_OPTION_PRECISION] & RM_PRECISION_QUANTIZED_BVH) && s->qnodes.ptr;
 }
 
 int upload(rm_context* c, DevBuf& b, const void* src, size_t bytes, const char* what) {
@@ -177,6 +183,8 @@ int rm_set_option(rm_context* c, int option, int64_t value) {
         return fail(RM_ERR_INVALID, "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT or _PERSISTENT.");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
+    if (option == RM_OPTION_PRECISION && (value & ~(int64_t)RM_PRECISION_QUANTIZED_BVH))
+        return fail(RM_ERR_INVALID, "RM_OPTION_PRECISION takes RM_PRECISION_* flags.");
     c->options[option] = value;
     return RM_OK;
 }
@@ -188,7 +196,7 @@ int rm_close(rm_context* c) {
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->post) (void)hipStreamSynchronize(c->post);
     for (rm_scene* s : c->scenes) {
-        for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights, &s->textures, &s->nmaps}) b->release();
+        s->release();
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->albedo_frame, &c->heads,
@@ -248,13 +256,24 @@ int rm_scene_upload(rm_context* c, const void* primitives, uint32_t primitive_sl
     int r = RM_OK;
     if (r == RM_OK) r = upload(c, s->prims, primitives, (size_t)primitive_slots * 64, "uploading primitives");
     if (r == RM_OK) r = upload(c, s->nodes, nodes, (size_t)node_count * 32, "uploading BVH nodes");
+    if (r == RM_OK) {
+        // The quantized tree, for RM_PRECISION_QUANTIZED_BVH; a tree it can't
+        // enclose (boxes that aren't finite) keeps only the FP32 form.
+        std::vector<uint32_t> q((size_t)node_count * 8);
+        const std::string before = rm_last_error();
+        if (rm_bvh_quantize(nodes, node_count, q.data()) == RM_OK)
+            r = upload(c, s->qnodes, q.data(), q.size() * 4, "uploading the quantized BVH");
+        else
+            fail(RM_OK, before);
+        if (r == RM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(RM_ERR_DEVICE, "uploading a scene");
+    }
     if (r == RM_OK) r = upload(c, s->mats, materials, (size_t)material_count * 96, "uploading materials");
     if (r == RM_OK) r = upload(c, s->attr, attributes, (size_t)attribute_vec4s * 16, "uploading attributes");
     if (r == RM_OK) r = upload(c, s->lights, lights, (size_t)lights_len * 4, "uploading the lights table");
     if (r == RM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(RM_ERR_DEVICE, "uploading a scene");
     if (r == RM_OK && lights) s->lights_host.assign(lights, lights + lights_len);
     if (r != RM_OK) {
-        for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights}) b->release();
+        s->release();
         delete s;
         return r;
     }
@@ -269,8 +288,7 @@ int rm_scene_release(rm_context* c, rm_scene* s) {
         if (c->scenes[i] == s) {
             (void)hipSetDevice(c->device);
             (void)hipStreamSynchronize(c->stream);
-            for (DevBuf* b : {&s->prims, &s->nodes, &s->mats, &s->attr, &s->lights, &s->textures, &s->nmaps})
-                b->release();
+            s->release();
             delete s;
             c->scenes.erase(c->scenes.begin() + i);
             return RM_OK;
@@ -364,7 +382,7 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
 
     rm::DevScene S = dev_scene(c, s);
     rm::Features f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
-                      s->dispersive};
+                      s->dispersive, quantized(c, s)};
     // Light tracing and the photon gather depend on the scene and the
     // parameters only, so they run on the side stream while the path pass
     // runs; k_resolve then adds splats and gathered light, after the path
@@ -430,7 +448,7 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
     {
         Timed t(c, RM_STAGE_TRACE);
         rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(),
-                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging};
+                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging, f.quantized};
         rm::launch_main(L, f, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the trace");
@@ -541,7 +559,7 @@ int rm_raycast(rm_context* c, const rm_scene* s, const float* rays, uint32_t n, 
     {
         Timed t(c, RM_STAGE_RAYCAST);
         rm::launch_raycast(dev_scene(c, s), c->rays.as<const float4>(), n, c->hits.as<uint4>(), any_hit != 0,
-                           c->spill.as<uint32_t>(), n, counters ? c->counters.as<uint2>() : nullptr, c->stream);
+                           quantized(c, s), c->spill.as<uint32_t>(), n, counters ? c->counters.as<uint2>() : nullptr, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the raycast");
     RM_HIP(hipMemcpyAsync(hits, c->hits.ptr, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream), "reading hits");

@@ -493,3 +493,101 @@ extern "C" int rm_bvh_build(const double* mins, const double* maxs, uint32_t cou
     *node_count = used;
     return RM_OK;
 }
+
+// ------------------------------------------------------------ quantized tree
+//
+// The 8-bit child-quantized form of a packed tree, one 32-byte record per
+// node, indexed like the nodes:
+//
+//   inner node p, children c and c + 1 (both boxes relative to p's box):
+//     f32 origin[3]   p's min corner
+//     u32 meta        step exponents + 127 (bits 0-7, 8-15, 16-23: the
+//                     step is 2^e per axis), bit 24: child c is a leaf,
+//                     bit 25: child c + 1 is a leaf
+//     u8  box[12]     c's lo xyz, hi xyz, then c + 1's lo xyz, hi xyz
+//     u32 c
+//   leaf: u32 first, u32 count, then zeros.
+//
+// A coordinate decodes as origin + q·step in FP32 (q·step is exact), the
+// very expression the device evaluates; each lo is the largest q whose
+// decoded value is at most the child's FP32 lo, each hi the smallest whose
+// decoded value is at least its hi. Decoded boxes therefore enclose the
+// FP32 ones, and since a slab test's entry distance only falls and its exit
+// only grows as a box grows, every ray that enters a child's FP32 box
+// enters its decoded box: no leaf the FP32 tree reaches is skipped.
+namespace {
+
+inline float decode(float origin, uint32_t q, float step) { return origin + (float)q * step; }
+
+// The smallest exponent in [-126, 127] whose 255 steps reach hi from lo.
+int step_exponent(float lo, float hi) {
+    if (!(hi > lo)) return -126;
+    int e = (int)std::ceil(std::log2(((double)hi - (double)lo) / 255.0)) - 1;
+    e = std::max(-126, std::min(127, e));
+    while (e < 127 && !(decode(lo, 255, std::ldexp(1.0f, e)) >= hi)) ++e;
+    return e;
+}
+
+bool quantize_axis(float origin, float step, float lo, float hi, uint32_t& qlo, uint32_t& qhi) {
+    double g = std::floor(((double)lo - origin) / step);
+    uint32_t a = g < 0.0 ? 0u : g > 255.0 ? 255u : (uint32_t)g;
+    while (a > 0 && decode(origin, a, step) > lo) --a;
+    while (a < 255 && decode(origin, a + 1, step) <= lo) ++a;
+    g = std::ceil(((double)hi - origin) / step);
+    uint32_t b = g < 0.0 ? 0u : g > 255.0 ? 255u : (uint32_t)g;
+    while (b < 255 && decode(origin, b, step) < hi) ++b;
+    while (b > 0 && decode(origin, b - 1, step) >= hi) --b;
+    qlo = a;
+    qhi = b;
+    return decode(origin, a, step) <= lo && decode(origin, b, step) >= hi;
+}
+
+}  // namespace
+
+extern "C" int rm_bvh_quantize(const void* nodes, uint32_t node_count, void* qnodes) {
+    if (!nodes || !qnodes || node_count == 0)
+        return rmi::fail(RM_ERR_INVALID, "rm_bvh_quantize needs nodes and room for as many quantized nodes.");
+    const uint32_t* nu = static_cast<const uint32_t*>(nodes);
+    const float* nf = static_cast<const float*>(nodes);
+    uint32_t* qu = static_cast<uint32_t*>(qnodes);
+    float* qf = static_cast<float*>(qnodes);
+    for (uint32_t p = 0; p < node_count; ++p) {
+        const size_t o = (size_t)p * 8;
+        const uint32_t first = nu[o + 3], count = nu[o + 7];
+        std::memset(qu + o, 0, 32);
+        if (count > 0) {
+            qu[o] = first;
+            qu[o + 1] = count;
+            continue;
+        }
+        if ((uint64_t)first + 1 >= node_count)
+            return rmi::fail(RM_ERR_INVALID, "BVH node " + std::to_string(p) + " points outside the tree.");
+        uint32_t meta = 0, q[2][6];
+        for (int a = 0; a < 3; ++a) {
+            const float lo = nf[o + a], hi = nf[o + 4 + a];
+            if (!std::isfinite(lo) || !std::isfinite(hi))
+                return rmi::fail(RM_ERR_INVALID, "BVH node " + std::to_string(p) + " has a box that isn't finite.");
+            const int e = step_exponent(lo, hi);
+            const float step = std::ldexp(1.0f, e);
+            meta |= (uint32_t)(e + 127) << (8 * a);
+            for (uint32_t k = 0; k < 2; ++k) {
+                const size_t co = (size_t)(first + k) * 8;
+                const float clo = nf[co + a], chi = nf[co + 4 + a];
+                if (!(clo >= lo && chi <= hi) || !quantize_axis(lo, step, clo, chi, q[k][a], q[k][3 + a]))
+                    return rmi::fail(RM_ERR_INVALID,
+                                     "BVH node " + std::to_string(first + k) + " lies outside its parent's box.");
+            }
+        }
+        for (uint32_t k = 0; k < 2; ++k)
+            if (nu[(size_t)(first + k) * 8 + 7] > 0) meta |= 1u << (24 + k);
+        qf[o] = nf[o];
+        qf[o + 1] = nf[o + 1];
+        qf[o + 2] = nf[o + 2];
+        qu[o + 3] = meta;
+        const uint32_t* b = &q[0][0];   // c lo xyz, hi xyz, c + 1 lo xyz, hi xyz
+        for (int w = 0; w < 3; ++w)
+            qu[o + 4 + w] = b[4 * w] | b[4 * w + 1] << 8 | b[4 * w + 2] << 16 | b[4 * w + 3] << 24;
+        qu[o + 7] = first;
+    }
+    return RM_OK;
+}

@@ -52,10 +52,14 @@ struct StageTimer {
 
 struct rm_scene {
     rmi::DevBuf prims, nodes, mats, attr, lights, textures, nmaps;
+    rmi::DevBuf qnodes;   // the 8-bit child-quantized tree; empty when the tree can't be quantized
     uint32_t primitive_slots = 0, primitive_count = 0, node_count = 0, material_count = 0;
     uint32_t tex_w = 0, tex_h = 0, tex_layers = 0, nmap_w = 0, nmap_h = 0, nmap_layers = 0;
     bool dispersive = false;
     std::vector<uint32_t> lights_host;   // the lights table, for checking parameters against
+    void release() {
+        for (rmi::DevBuf* b : {&prims, &nodes, &mats, &attr, &lights, &textures, &nmaps, &qnodes}) b->release();
+    }
 };
 
 struct rm_context {
@@ -80,7 +84,7 @@ struct rm_context {
     rm::Params last_params = {};   // the last frame's, for rm_display
     bool rendered = false;
     int compute_units = 0;
-    int64_t options[RM_OPTION_COUNT] = {0, 0};
+    int64_t options[RM_OPTION_COUNT] = {};
     bool timing = false;
     rmi::StageTimer timer;
     std::vector<rm_scene*> scenes;

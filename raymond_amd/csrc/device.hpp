@@ -121,6 +121,7 @@ struct Mat {
 struct DevScene {
     const float4* prims;       // 4 per primitive
     const float4* nodes;       // 2 per node
+    const float4* qnodes;      // 2 per node: the 8-bit child-quantized tree, or null
     const float4* mats;        // 6 per material
     const float4* attr;        // vec4s
     const uint32_t* lights;    // lights, volumes, cdf bits
@@ -309,8 +310,11 @@ struct Hit { float t; uint32_t prim; float u, v; };
 // child test has already loaded, so a pop costs no global load. Capacity is
 // the reference's 64 entries; a full stack drops the far child exactly as the
 // reference does.
-template <int LDS_DEPTH, int BLOCK>
+// QUANT selects the tree trace() walks: the FP32 nodes, or the quantized
+// records (entries then hold a node's index and whether it is a leaf).
+template <int LDS_DEPTH, int BLOCK, bool QUANT = false>
 struct Stack {
+    static constexpr bool quantized = QUANT;
     uint2* lds;      // &shared[0][threadIdx.x]
     uint2* spill;    // this thread's column of an [entry][slot] spill array
     uint32_t stride; // slots in the spill array
@@ -339,7 +343,7 @@ struct Count {
 // every tie resolves as the reference's does. The current node's (first,
 // count) come along from the child test, so a node record is loaded once.
 template <bool ANY, class STACK, class COUNT>
-RM_D Hit trace(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& cnt) {
+RM_D Hit trace_fp32(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& cnt) {
     Hit hit = {t_max, NONE, 0.0f, 0.0f};
     V3 sd = safe(d);
     V3 inv = {1.0f / sd.x, 1.0f / sd.y, 1.0f / sd.z};
@@ -387,6 +391,103 @@ RM_D Hit trace(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& 
         }
     }
     return hit;
+}
+
+// One axis of a quantized child box: origin + q·step, q·step exact (an
+// 8-bit integer times a power of two), so only the sum rounds — the
+// expression the host rounded outward against.
+RM_D float dequant(float origin, uint32_t q, float step) { return origin + (float)q * step; }
+
+RM_D float step_of(uint32_t biased) { return __uint_as_float(biased << 23); }
+
+// trace() over the 8-bit child-quantized tree (rm_bvh_quantize): the same
+// walk — nearer child first, ties to the left, the farther pushed when both
+// are entered and the stack has room, no re-test on pop — with each inner
+// node's two child boxes decoded from one 32-byte record. Decoded boxes
+// enclose the FP32 ones, so the leaves the FP32 walk tests are all tested
+// here and the nearest hit is the same; only the visiting order of boxes
+// that overlap along the ray can differ, which matters for hits at exactly
+// equal t.
+template <bool ANY, class STACK, class COUNT>
+RM_D Hit trace_q(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& cnt) {
+    Hit hit = {t_max, NONE, 0.0f, 0.0f};
+    V3 sd = safe(d);
+    V3 inv = {1.0f / sd.x, 1.0f / sd.y, 1.0f / sd.z};
+    const float4* Q = S.qnodes;
+    cnt.nodes(1);
+    float4 r0 = S.nodes[0], r1 = S.nodes[1];
+    if (slabs(r0, r1, o, inv, t_max) >= INF) return hit;
+    uint32_t node = 0u, first = __float_as_uint(r0.w), count = __float_as_uint(r1.w);
+    uint32_t sp = 0;
+    for (;;) {
+        bool descend = false;
+        if (count > 0u) {
+            for (uint32_t i = first; i < first + count; i++) {
+                cnt.prims(1);
+                V3 h = intersect(S, i, o, d, hit.t);
+                if (h.x > 0.0f) {
+                    hit = {h.x, i, h.y, h.z};
+                    if (ANY) return hit;
+                }
+            }
+        } else {
+            cnt.nodes(2);
+            const float4 a = Q[(size_t)node * 2], b = Q[(size_t)node * 2 + 1];
+            const uint32_t meta = __float_as_uint(a.w);
+            const float sx = step_of(meta & 0xffu), sy = step_of(meta >> 8 & 0xffu), sz = step_of(meta >> 16 & 0xffu);
+            const uint32_t w0 = __float_as_uint(b.x), w1 = __float_as_uint(b.y), w2 = __float_as_uint(b.z);
+            const float4 l0 = make_float4(dequant(a.x, w0 & 0xffu, sx), dequant(a.y, w0 >> 8 & 0xffu, sy),
+                                          dequant(a.z, w0 >> 16 & 0xffu, sz), 0.0f);
+            const float4 l1 = make_float4(dequant(a.x, w0 >> 24, sx), dequant(a.y, w1 & 0xffu, sy),
+                                          dequant(a.z, w1 >> 8 & 0xffu, sz), 0.0f);
+            const float4 q0 = make_float4(dequant(a.x, w1 >> 16 & 0xffu, sx), dequant(a.y, w1 >> 24, sy),
+                                          dequant(a.z, w2 & 0xffu, sz), 0.0f);
+            const float4 q1 = make_float4(dequant(a.x, w2 >> 8 & 0xffu, sx), dequant(a.y, w2 >> 16 & 0xffu, sy),
+                                          dequant(a.z, w2 >> 24, sz), 0.0f);
+            float t_left = slabs(l0, l1, o, inv, hit.t);
+            float t_right = slabs(q0, q1, o, inv, hit.t);
+            bool left_first = t_left <= t_right;
+            if (fminf(t_left, t_right) < INF) {
+                const uint32_t c = __float_as_uint(b.w);
+                const uint32_t near = left_first ? c : c + 1u;
+                const uint32_t near_leaf = meta >> (left_first ? 24 : 25) & 1u;
+                if (fmaxf(t_left, t_right) < INF && sp < STACK_LIMIT) {
+                    stack.put(sp, make_uint2(left_first ? c + 1u : c, meta >> (left_first ? 25 : 24) & 1u));
+                    sp++;
+                }
+                node = near;
+                count = 0u;
+                if (near_leaf) {
+                    const float4 e = Q[(size_t)near * 2];
+                    first = __float_as_uint(e.x);
+                    count = __float_as_uint(e.y);
+                }
+                descend = true;
+            }
+        }
+        if (!descend) {
+            if (sp == 0u) break;
+            sp--;
+            const uint2 e = stack.get(sp);
+            node = e.x;
+            count = 0u;
+            if (e.y) {
+                const float4 l = Q[(size_t)node * 2];
+                first = __float_as_uint(l.x);
+                count = __float_as_uint(l.y);
+            }
+        }
+    }
+    return hit;
+}
+
+// The walk the stack's type asks for.
+template <bool ANY, class STACK, class COUNT>
+RM_D Hit trace(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& cnt) {
+    if constexpr (STACK::quantized)
+        return trace_q<ANY>(S, o, d, t_max, stack, cnt);
+    else
+        return trace_fp32<ANY>(S, o, d, t_max, stack, cnt);
 }
 
 }  // namespace rm

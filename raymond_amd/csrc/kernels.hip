@@ -52,9 +52,9 @@ RM_D bool stencil_pixel(uint32_t width, uint32_t height, uint32_t& x, uint32_t& 
     return x < width && y < height;
 }
 
-template <int LDS_DEPTH>
+template <int LDS_DEPTH, bool QUANT = false>
 struct BlockStack {
-    using S = Stack<LDS_DEPTH, BLOCK_THREADS>;
+    using S = Stack<LDS_DEPTH, BLOCK_THREADS, QUANT>;
     RM_D static S make(uint2* lds, uint32_t* spill_base, uint32_t slot, uint32_t slots) {
         S s;
         s.lds = lds + threadIdx.x;
@@ -67,14 +67,14 @@ struct BlockStack {
 // ------------------------------------------------------------------ raycast
 // trace() over a ray batch: rays are 8 floats (o, t_max, d, pad); hits are
 // (t, prim, u, v) as 4 words.
-template <bool ANY>
+template <bool ANY, bool QUANT>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_raycast(DevScene S, const float4* __restrict__ rays, uint32_t n,
                                                            uint4* __restrict__ hits, uint32_t* spill, uint32_t slots,
                                                            uint2* counters) {
     __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     if (i >= n) return;
-    auto stack = BlockStack<TRACE_LDS_DEPTH>::make(lds, spill, i, slots);
+    auto stack = BlockStack<TRACE_LDS_DEPTH, QUANT>::make(lds, spill, i, slots);
     float4 a = rays[(size_t)i * 2], b = rays[(size_t)i * 2 + 1];
     Hit h;
     if (counters) {
@@ -368,7 +368,7 @@ struct PixelSums {
 // 112 bytes a lane of rarely touched state (the pixel's sums, the guides) and
 // the extra wave hides enough traversal latency to make the pass about 11%
 // faster on the 1080p sphere scene.
-template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, bool QUANT>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main(
     DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
     float4* __restrict__ guide_normal, uint32_t* spill, uint32_t slots) {
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
     if (x >= P.width || y >= P.height) return;
     const Ctx C{S, &P};
     const uint32_t index = y * P.width + x;
-    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, index, slots);
+    auto stack = BlockStack<MAIN_LDS_DEPTH, QUANT>::make(lds, spill, index, slots);
     PixelSums sums;
     sums.clear();
     for (uint32_t k = 0; k < P.samples_per_frame; k++) {
@@ -661,7 +661,7 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
 // the bounce loop with one counter atomic per wave (ballot + prefix count):
 // photon slots are claimed in a different order than one atomic per lane
 // would give, which the reference leaves unspecified too.
-template <bool TEXTURES, bool DISPERSION>
+template <bool TEXTURES, bool DISPERSION, bool QUANT>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Params P, const float4* __restrict__ E,
                                                                uint32_t* splats, uint32_t* heads,
                                                                float4* __restrict__ photons, uint32_t* spill,
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Param
     const Ctx C{S, &P};
     const uint32_t id = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     const uint32_t photons_n = photon_count(P);
-    auto stack = BlockStack<MAIN_LDS_DEPTH>::make(lds, spill, id, slots);
+    auto stack = BlockStack<MAIN_LDS_DEPTH, QUANT>::make(lds, spill, id, slots);
     bool pending = false;
     V3 dep_p = {0, 0, 0}, dep_d = {0, 0, 0}, dep_beta = {0, 0, 0};
 
@@ -890,9 +890,9 @@ RM_D V3 gather(const Ctx& C, const uint32_t* heads, const float4* photons, const
 
 // src/shaders.mjs:1577-1626. The pixel's gathered light goes to
 // gathered[index] (w = 1), or zeros when it adds none; k_resolve adds it.
-template <bool TEXTURES>
+template <bool TEXTURES, class STACK>
 RM_D float4 gather_pixel(const Ctx& C, const uint32_t* heads, const float4* photons, uint32_t x, uint32_t y,
-                         uint32_t index, Stack<GATHER_LDS_DEPTH, BLOCK_THREADS>& stack) {
+                         uint32_t index, STACK& stack) {
     const Params& P = *C.P;
     const DevScene& S = C.S;
     NoCount nc;
@@ -927,7 +927,7 @@ RM_D float4 gather_pixel(const Ctx& C, const uint32_t* heads, const float4* phot
     return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-template <bool TEXTURES>
+template <bool TEXTURES, bool QUANT>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Params P, const uint32_t* heads,
                                                                  const float4* __restrict__ photons,
                                                                  float4* __restrict__ gathered, uint32_t* spill,
@@ -938,7 +938,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Par
     if (x >= P.width || y >= P.height) return;
     const Ctx C{S, &P};
     const uint32_t index = y * P.width + x;
-    auto stack = BlockStack<GATHER_LDS_DEPTH>::make(lds, spill, index, slots);
+    auto stack = BlockStack<GATHER_LDS_DEPTH, QUANT>::make(lds, spill, index, slots);
     gathered[index] = gather_pixel<TEXTURES>(C, heads, photons, x, y, index, stack);
 }
 
@@ -1162,8 +1162,12 @@ static void launch_main_v(const LaunchMain& L, hipStream_t stream) {
                            L.galb, L.gnorm, L.spill, L.slots, L.staging);
         return;
     }
-    hipLaunchKernelGGL((k_main<V, T, E, D>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum, L.galb,
-                       L.gnorm, L.spill, L.slots);
+    if (L.quantized)
+        hipLaunchKernelGGL((k_main<V, T, E, D, true>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
+                           L.galb, L.gnorm, L.spill, L.slots);
+    else
+        hipLaunchKernelGGL((k_main<V, T, E, D, false>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
+                           L.galb, L.gnorm, L.spill, L.slots);
 }
 
 template <bool V, bool T, bool E, bool D>
@@ -1205,30 +1209,43 @@ void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream) {
     }
 }
 
-void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, uint32_t* spill,
-                    uint32_t slots, uint2* counters, hipStream_t stream) {
+void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, bool quantized,
+                    uint32_t* spill, uint32_t slots, uint2* counters, hipStream_t stream) {
     dim3 grid((n + BLOCK_THREADS - 1) / BLOCK_THREADS);
-    if (any_hit)
-        hipLaunchKernelGGL(k_raycast<true>, grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill, slots, counters);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill, slots, counters);
+    };
+    if (quantized) {
+        if (any_hit) go(k_raycast<true, true>);
+        else go(k_raycast<false, true>);
+    } else {
+        if (any_hit) go(k_raycast<true, false>);
+        else go(k_raycast<false, false>);
+    }
+}
+
+template <bool T, bool D>
+static void launch_light_trace_v(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
+                                 uint32_t* heads, float4* photons, uint32_t* spill, uint32_t slots, dim3 grid,
+                                 bool quantized, hipStream_t stream) {
+    if (quantized)
+        hipLaunchKernelGGL((k_light_trace<T, D, true>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
+                           heads, photons, spill, slots);
     else
-        hipLaunchKernelGGL(k_raycast<false>, grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill, slots, counters);
+        hipLaunchKernelGGL((k_light_trace<T, D, false>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
+                           heads, photons, spill, slots);
 }
 
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
                         float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
                         hipStream_t stream) {
     dim3 grid((count + BLOCK_THREADS - 1) / BLOCK_THREADS);
-    if (f.textures || f.dispersion) {
-        if (f.textures)
-            hipLaunchKernelGGL((k_light_trace<true, true>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
-                               heads, photons, spill, slots);
-        else
-            hipLaunchKernelGGL((k_light_trace<false, true>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
-                               heads, photons, spill, slots);
-    } else {
-        hipLaunchKernelGGL((k_light_trace<false, false>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
-                           heads, photons, spill, slots);
-    }
+    if (f.textures)
+        launch_light_trace_v<true, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.quantized, stream);
+    else if (f.dispersion)
+        launch_light_trace_v<false, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.quantized, stream);
+    else
+        launch_light_trace_v<false, false>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.quantized, stream);
 }
 
 void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, hipStream_t stream) {
@@ -1240,12 +1257,16 @@ void launch_resolve(const Params& P, uint4* splats, const float4* gathered, floa
 void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
                           float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream) {
     dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
-    if (f.textures)
-        hipLaunchKernelGGL(k_photon_gather<true>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons,
-                           gathered, spill, slots);
-    else
-        hipLaunchKernelGGL(k_photon_gather<false>, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons,
-                           gathered, spill, slots);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons, gathered, spill, slots);
+    };
+    if (f.textures) {
+        if (f.quantized) go(k_photon_gather<true, true>);
+        else go(k_photon_gather<true, false>);
+    } else {
+        if (f.quantized) go(k_photon_gather<false, true>);
+        else go(k_photon_gather<false, false>);
+    }
 }
 
 void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStream_t stream) {

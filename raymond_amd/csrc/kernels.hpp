@@ -18,7 +18,11 @@ constexpr int GATHER_LDS_DEPTH = 8;
 // shallowest LDS stack (the compacted path kernel keeps 4 entries in LDS).
 constexpr uint32_t SPILL_ENTRIES = STACK_LIMIT - 4u;
 
-struct Features { bool volumes, textures, environment, dispersion; };
+// What a frame's scene and options need; each combination is its own kernel.
+// quantized: traverse the 8-bit child-quantized tree (the per-pixel path
+// pass, raycast, light tracing and gather; the compact and persistent path
+// passes walk the FP32 tree).
+struct Features { bool volumes, textures, environment, dispersion, quantized; };
 
 struct LaunchMain {
     DevScene S;
@@ -31,13 +35,14 @@ struct LaunchMain {
     uint32_t persistent_blocks;  // > 0: k_main_persistent with this grid
     uint32_t* counter;           // k_main_persistent's tile counter, zeroed before the launch
     float4* staging;             // non-null: k_main_compact, 3 float4 per thread of its grid
+    bool quantized;              // the per-pixel pass walks the quantized tree
 };
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
 // How many 256-thread blocks of the persistent path kernel fit on one CU.
 int main_resident_blocks_per_cu(const Features& f);
-void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, uint32_t* spill,
-                    uint32_t slots, uint2* counters, hipStream_t stream);
+void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, bool quantized,
+                    uint32_t* spill, uint32_t slots, uint2* counters, hipStream_t stream);
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
                         float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
                         hipStream_t stream);

@@ -1,0 +1,118 @@
+"""The quantized tree (RM_PRECISION_QUANTIZED_BVH) on the GPU against the
+oracle's FP32 trace() (src/shaders.mjs:389-447).
+
+Decoded boxes enclose the FP32 ones, so the nearest t is bit-exact on every
+ray and occlusion answers agree. The primitive and barycentrics are
+bit-exact too, except where two primitives are hit at exactly the same t
+(coplanar faces, such as a box standing on a floor, met from below): the
+first one reached wins, and the quantized boxes can order two overlapping
+children differently. There the oracle confirms the GPU's primitive is hit
+at that very t, with those barycentrics."""
+import numpy as np
+import pytest
+
+from helpers import gpu_render
+from oracle.pyoracle import Oracle
+from raymond_amd import capi
+from test_gpu_trace import assert_same_hits, random_rays
+
+pytestmark = pytest.mark.gpu
+
+SCENES = [
+    ("cornell", {}),
+    ("mixed", {}),
+    ("spheres", {}),
+    ("triangles", {"scene": {"grid": 96, "extras": 4, "envWidth": 16, "envHeight": 8}, "width": 64, "height": 36}),
+    ("caustics", {"scene": {"res": 48}, "width": 64, "height": 36}),
+]
+
+
+def assert_same_closest(gpu, cpu, rays, o, what) -> int:
+    """Same nearest t (bits) and hit/miss on every ray; where the primitive
+    differs, a tie: the GPU's primitive is hit at that t. Returns the ties."""
+    np.testing.assert_array_equal(gpu[:, 0], cpu[:, 0], err_msg=f"{what}: t")
+    np.testing.assert_array_equal(gpu[:, 1] == 0xFFFFFFFF, cpu[:, 1] == 0xFFFFFFFF, err_msg=f"{what}: hit or miss")
+    ties = np.nonzero(np.any(gpu[:, 1:] != cpu[:, 1:], axis=1))[0]
+    for i in ties:
+        own = o.intersect(int(gpu[i, 1]), rays[i], float(rays[i, 3])).astype(np.float32).view(np.uint32)
+        assert (own == gpu[i, [0, 2, 3]]).all(), f"{what}: ray {i} names primitive {gpu[i, 1]}, not hit at its t"
+        assert gpu[i, 1] != cpu[i, 1]
+    return len(ties)
+
+
+class quantized:
+    """Turns the quantized tree on for a context, and off again."""
+
+    def __init__(self, ctx, on=True):
+        self.ctx, self.on = ctx, on
+
+    def __enter__(self):
+        self.ctx.set_option(capi.OPTION_PRECISION, capi.PRECISION_QUANTIZED_BVH if self.on else 0)
+
+    def __exit__(self, *exc):
+        self.ctx.set_option(capi.OPTION_PRECISION, 0)
+
+
+@pytest.mark.parametrize("config,opts", SCENES, ids=[s[0] for s in SCENES])
+def test_closest_hits_match_the_fp32_oracle(ctx, pack, config, opts):
+    p = pack(config, **opts)
+    o = Oracle(p)
+    rays = random_rays(200_000, seed=7 + len(config))
+    scene = ctx.upload_scene(p)
+    try:
+        with quantized(ctx):
+            gpu, gcnt = ctx.raycast(scene, rays, counters=True)
+            occluded = ctx.raycast(scene, rays, any_hit=True)[:, 1] != 0xFFFFFFFF
+        _, fcnt = ctx.raycast(scene, rays, counters=True)
+    finally:
+        scene.release()
+    cpu = o.trace(rays)
+    ties = assert_same_closest(gpu, cpu, rays, o, f"{config} quantized closest hits")
+    assert ties <= 0.002 * len(rays)
+    cpu_any = o.trace(rays, any_hit=True)[:, 1] != 0xFFFFFFFF
+    np.testing.assert_array_equal(occluded, cpu_any)
+    # A decoded box encloses its FP32 box: never less work than the FP32 walk.
+    assert (gcnt[:, 1] >= fcnt[:, 1]).mean() > 0.999
+    assert gcnt[:, 0].sum() >= fcnt[:, 0].sum()
+
+
+def test_cornell_primary_rays(ctx, pack):
+    p = pack("cornell")
+    o = Oracle(p)
+    rays = o.camera_rays()
+    scene = ctx.upload_scene(p)
+    try:
+        with quantized(ctx):
+            gpu = ctx.raycast(scene, rays)
+    finally:
+        scene.release()
+    assert_same_closest(gpu, o.trace(rays), rays, o, "cornell primary rays, quantized")
+
+
+@pytest.mark.parametrize("config,opts", SCENES[:4], ids=[s[0] for s in SCENES[:4]])
+def test_frames_match_the_fp32_tree(ctx, pack, config, opts):
+    """Paths see the same hits, so pixels are bit-identical but for the rare
+    path that meets an exact-t tie between coplanar faces."""
+    p = pack(config, **opts)
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0)
+    plain = gpu_render(ctx, p, 2, params)
+    with quantized(ctx):
+        q = gpu_render(ctx, p, 2, params)
+    for a, b, what in zip(q, plain, ("accum", "guide albedo", "guide normal")):
+        same = (a == b).all(axis=-1).mean()
+        assert same >= 0.995, f"{what}: {same:.5f} of pixels bit-identical"
+
+
+def test_caustic_frames_agree_with_the_fp32_tree(ctx, pack):
+    """Light tracing and the gather on the quantized tree: the same frame as
+    the FP32 tree's to the light tests' tolerance (photon deposit order is
+    atomic, so gathered sums differ by rounding)."""
+    from test_gpu_light import CAUSTIC_SCENES
+    from helpers import agreement
+    p = pack(*CAUSTIC_SCENES[0][:1], **CAUSTIC_SCENES[0][1])
+    params = p.with_params(caustics=1, denoise=0, firefly_clamp=0.0)
+    plain = gpu_render(ctx, p, 2, params)[0]
+    with quantized(ctx):
+        q = gpu_render(ctx, p, 2, params)[0]
+    assert agreement(q, plain, rel=1e-3, abs_=1e-4) > 0.99
+    assert abs(q[..., :3].sum() - plain[..., :3].sum()) <= 1e-3 * abs(plain[..., :3].sum())
