@@ -58,6 +58,11 @@ extern "C" {
 #define RM_PRECISION_QUANTIZED_BVH 1   /* traverse the 8-bit child-quantized tree (rm_bvh_quantize) in the
                                           per-pixel path pass, rm_raycast, light tracing and photon gather;
                                           decoded boxes enclose the FP32 ones, so no FP32 hit is missed */
+#define RM_PRECISION_FP16_GUIDES 2     /* the denoiser's per-frame guide (normal, distance) in FP16; the guide
+                                          sums stay FP32 and the passes compute in FP32 */
+#define RM_PRECISION_COMPENSATED 4     /* per-pixel running sums (radiance, guide albedo, guide normal) are
+                                          compensated FP32: each add's rounding error is kept and folded
+                                          back for the denoiser and for rm_read / rm_display */
 
 #define RM_SCHEDULE_PIXEL 0            /* default: one thread per pixel, each path run to its end */
 #define RM_SCHEDULE_COMPACT 1          /* a block's live paths are packed into its first waves between bounces */

@@ -1912,3 +1912,132 @@ void oracle_env_radiance(const OracleScene* s, const uint8_t* params, const floa
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- FP64 denoiser
+// The whole denoiser (src/shaders.mjs:1916-2098: prepare, à-trous at steps
+// 1, 2, 4, 8, 16, compose) in double precision, from the FP32 sums: the
+// error reference for the GPU's FP32 and FP16-guide passes. Same formulas
+// and tap order as the FP32 restatement above, every value a double.
+namespace {
+
+struct D4 { double x, y, z, w; };
+
+struct Den64 {
+    const Params* P;
+    const float* accum;
+    const float* galb;
+    const float* gnorm;
+    double samples() const { return (double)std::max(P->total_samples, 1u); }
+    static double lum(double r, double g, double b) { return r * 0.2126 + g * 0.7152 + b * 0.0722; }
+    D4 albedo(uint32_t i) const {
+        const float* g = galb + (size_t)i * 4;
+        const double n = samples();
+        return {std::max(g[0] / n, 0.01), std::max(g[1] / n, 0.01), std::max(g[2] / n, 0.01), 0.0};
+    }
+    D4 guide(uint32_t i) const {
+        const float* g = gnorm + (size_t)i * 4;
+        const double l2 = (double)g[0] * g[0] + (double)g[1] * g[1] + (double)g[2] * g[2];
+        const double l = std::sqrt(l2);
+        return l2 > 0.0 ? D4{g[0] / l, g[1] / l, g[2] / l, g[3] / samples()} : D4{0.0, 0.0, 0.0, g[3] / samples()};
+    }
+    static double similarity(const D4& p, const D4& q, double step) {
+        if (p.w <= 0.0 || q.w <= 0.0) return (p.w <= 0.0 && q.w <= 0.0) ? 1.0 : 0.0;
+        if (p.x * p.x + p.y * p.y + p.z * p.z == 0.0) return 0.0;
+        const double wn = std::pow(std::max(p.x * q.x + p.y * q.y + p.z * q.z, 0.0), 64.0);
+        const double wz = std::exp(-std::fabs(p.w - q.w) / (p.w * 0.02 * step + 1e-6));
+        return wn * wz;
+    }
+};
+
+const double KERNEL64[3] = {0.375, 0.25, 0.0625};
+
+}  // namespace
+
+extern "C" void oracle_denoise64(const uint8_t* params, const float* accum, const float* guide_albedo,
+                                 const float* guide_normal, double* out) {
+    Den64 D{reinterpret_cast<const Params*>(params), accum, guide_albedo, guide_normal};
+    const Params& P = *D.P;
+    const int32_t W = (int32_t)P.width, H = (int32_t)P.height;
+    const size_t N = (size_t)W * H;
+    const double n = D.samples();
+    std::vector<D4> guides(N), light(N), next(N);
+    for (size_t i = 0; i < N; ++i) guides[i] = D.guide((uint32_t)i);
+    auto light_at = [&](size_t i) {
+        const float* a = accum + i * 4;
+        const D4 al = D.albedo((uint32_t)i);
+        return D4{a[0] / n / al.x, a[1] / n / al.y, a[2] / n / al.z, 0.0};
+    };
+    for (int32_t y = 0; y < H; ++y)
+        for (int32_t x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            D4 l = light_at(i);
+            const D4 al = D.albedo((uint32_t)i);
+            const double a = Den64::lum(al.x, al.y, al.z);
+            if (n >= 4.0) {
+                const float* ac = accum + i * 4;
+                const double mean = Den64::lum(ac[0], ac[1], ac[2]) / n;
+                const double spread = std::max(ac[3] / n - mean * mean, 0.0) * n / (n - 1.0);
+                l.w = spread / n / (a * a);
+            } else {
+                double sum = 0.0, sum2 = 0.0, weight = 0.0;
+                for (int32_t dy = -2; dy <= 2; dy++)
+                    for (int32_t dx = -2; dx <= 2; dx++) {
+                        const int32_t xx = x + dx, yy = y + dy;
+                        if (xx < 0 || yy < 0 || xx >= W || yy >= H) continue;
+                        const size_t j = (size_t)yy * W + xx;
+                        const double w = Den64::similarity(guides[i], guides[j], 1.0);
+                        const D4 lj = light_at(j);
+                        const double v = Den64::lum(lj.x, lj.y, lj.z);
+                        sum += w * v;
+                        sum2 += w * v * v;
+                        weight += w;
+                    }
+                const double mean = sum / std::max(weight, 1e-6);
+                l.w = std::max(sum2 / std::max(weight, 1e-6) - mean * mean, 0.0) + 0.01 * mean * mean;
+            }
+            light[i] = l;
+        }
+    for (int32_t step : {1, 2, 4, 8, 16}) {
+        for (int32_t y = 0; y < H; ++y)
+            for (int32_t x = 0; x < W; ++x) {
+                const size_t i = (size_t)y * W + x;
+                const D4 c = light[i], gp = guides[i];
+                const double lp = Den64::lum(c.x, c.y, c.z);
+                double variance = 0.0, vw = 0.0;
+                for (int32_t dy = -1; dy <= 1; dy++)
+                    for (int32_t dx = -1; dx <= 1; dx++) {
+                        const int32_t xx = x + dx, yy = y + dy;
+                        if (xx < 0 || yy < 0 || xx >= W || yy >= H) continue;
+                        const double w = KERNEL64[std::abs(dx)] * KERNEL64[std::abs(dy)];
+                        variance += w * light[(size_t)yy * W + xx].w;
+                        vw += w;
+                    }
+                const double sigma = 4.0 * std::sqrt(variance / vw) + 1e-4;
+                double sx = 0.0, sy = 0.0, sz = 0.0, sv = 0.0, weight = 0.0;
+                for (int32_t dy = -2; dy <= 2; dy++)
+                    for (int32_t dx = -2; dx <= 2; dx++) {
+                        const int32_t xx = x + dx * step, yy = y + dy * step;
+                        if (xx < 0 || yy < 0 || xx >= W || yy >= H) continue;
+                        const size_t j = (size_t)yy * W + xx;
+                        const D4 q = light[j];
+                        const double w = KERNEL64[std::abs(dx)] * KERNEL64[std::abs(dy)] *
+                                         Den64::similarity(gp, guides[j], (double)step) *
+                                         std::exp(-std::fabs(lp - Den64::lum(q.x, q.y, q.z)) / sigma);
+                        sx += w * q.x;
+                        sy += w * q.y;
+                        sz += w * q.z;
+                        sv += w * w * q.w;
+                        weight += w;
+                    }
+                next[i] = weight > 0.0 ? D4{sx / weight, sy / weight, sz / weight, sv / (weight * weight)} : c;
+            }
+        std::swap(light, next);
+    }
+    for (size_t i = 0; i < N; ++i) {
+        const D4 al = D.albedo((uint32_t)i);
+        out[i * 4] = light[i].x * al.x;
+        out[i * 4 + 1] = light[i].y * al.y;
+        out[i * 4 + 2] = light[i].z * al.z;
+        out[i * 4 + 3] = 0.0;
+    }
+}

@@ -82,6 +82,11 @@ void oracle_atrous(const uint8_t* params, uint32_t step, const float* guide_norm
 void oracle_compose(const uint8_t* params, const float* guide_albedo, const float* light_in,
                     float* out);
 
+/* The whole denoiser (prepare, à-trous 1..16, compose) in FP64 from the FP32
+ * sums: the error reference for the GPU's FP32 and FP16-guide passes. */
+void oracle_denoise64(const uint8_t* params, const float* accum, const float* guide_albedo,
+                      const float* guide_normal, double* out);
+
 /* DISPLAY_SHADER (src/shaders.mjs:2100-2152) as RGBA8 per pixel. */
 void oracle_display(const uint8_t* params, const float* accum, const float* denoised, uint8_t* out);
 

@@ -53,6 +53,7 @@ def load() -> C.CDLL:
             "oracle_prepare": (None, [P, P, P, P, P]),
             "oracle_atrous": (None, [P, U, P, P, P]),
             "oracle_compose": (None, [P, P, P, P]),
+            "oracle_denoise64": (None, [P, P, P, P, P]),
             "oracle_display": (None, [P, P, P, P]),
             "oracle_pcg": (U, [U]),
             "oracle_owen": (U, [U, U]),
@@ -174,6 +175,15 @@ class Oracle:
         lib.oracle_compose(_p(params), _p(galb), _p(src), _p(out))
         passes.append(out)
         return passes
+
+    @staticmethod
+    def denoise64(params, accum, galb, gnorm):
+        """The denoiser in FP64 from the FP32 sums: (H, W, 4) float64."""
+        out = np.zeros(accum.shape, np.float64)
+        load().oracle_denoise64(_p(np.ascontiguousarray(params)), _p(np.ascontiguousarray(accum, np.float32)),
+                                _p(np.ascontiguousarray(galb, np.float32)), _p(np.ascontiguousarray(gnorm, np.float32)),
+                                _p(out))
+        return out
 
     def intersect(self, prim, ray, t_max=1e30):
         out = np.zeros(3, np.float32)

@@ -48,7 +48,7 @@ SIGNATURES = {
     "rm_server_close": (C.c_int, [_P]),
 }
 OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION = 0, 1, 2
-PRECISION_QUANTIZED_BVH = 1
+PRECISION_QUANTIZED_BVH, PRECISION_FP16_GUIDES, PRECISION_COMPENSATED = 1, 2, 4
 SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT = 0, 1, 2
 SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT}
 

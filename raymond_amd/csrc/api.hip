@@ -105,6 +105,29 @@ rm::DevScene dev_scene(const rm_context* c, const rm_scene* s) {
     return S;
 }
 
+}  // namespace
+
+int rmi::current_sums(rm_context* c, hipStream_t stream, const float4* out[3]) {
+    if (!c->comp_live) {
+        out[0] = c->accum.as<const float4>();
+        out[1] = c->galb.as<const float4>();
+        out[2] = c->gnorm.as<const float4>();
+        return RM_OK;
+    }
+    const size_t px = (size_t)c->width * c->height;
+    RM_HIP(c->folded.ensure(px * 48), "allocating the folded sums");
+    rm::Params P = {};
+    P.width = c->width;
+    P.height = c->height;
+    rm::launch_fold(P, c->accum.as<const float4>(), c->galb.as<const float4>(), c->gnorm.as<const float4>(),
+                    c->comp.as<const float4>(), c->folded.as<float4>(), stream);
+    RM_HIP(hipGetLastError(), "launching the fold of compensated sums");
+    for (int k = 0; k < 3; ++k) out[k] = c->folded.as<const float4>() + (size_t)k * px;
+    return RM_OK;
+}
+
+namespace {
+
 // Whether traversals walk the scene's quantized tree: asked for, and built.
 bool quantized(const rm_context* c, const rm_scene* s) {
     return (c->options[RM_OPTION_PRECISION] & RM_PRECISION_QUANTIZED_BVH) && s->qnodes.ptr;
@@ -183,7 +206,8 @@ int rm_set_option(rm_context* c, int option, int64_t value) {
         return fail(RM_ERR_INVALID, "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT or _PERSISTENT.");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
-    if (option == RM_OPTION_PRECISION && (value & ~(int64_t)RM_PRECISION_QUANTIZED_BVH))
+    if (option == RM_OPTION_PRECISION &&
+        (value & ~(int64_t)(RM_PRECISION_QUANTIZED_BVH | RM_PRECISION_FP16_GUIDES | RM_PRECISION_COMPENSATED)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PRECISION takes RM_PRECISION_* flags.");
     c->options[option] = value;
     return RM_OK;
@@ -200,7 +224,7 @@ int rm_close(rm_context* c) {
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->albedo_frame, &c->heads,
-                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging, &c->display})
+                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging, &c->display, &c->comp, &c->folded})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
@@ -343,6 +367,7 @@ int rm_resize(rm_context* c, uint32_t width, uint32_t height) {
         RM_HIP(hipMemsetAsync(c->gnorm.ptr, 0, bytes, c->stream), "clearing the guides");
         RM_HIP(hipMemsetAsync(c->splats.ptr, 0, bytes, c->stream), "clearing the splats");
         c->capacity = px;
+        c->comp_live = false;
     }
     int r = ensure_spill(c, px);
     if (r != RM_OK) return r;
@@ -445,9 +470,20 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         staging = c->staging.as<float4>();
         slots = threads;
     }
+    // Compensated sums: the error terms start at zero with the image (the
+    // path pass writes them on a first frame), or when compensation starts
+    // part-way through an accumulation.
+    const bool compensated = (c->options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0;
+    float4* comp = nullptr;
+    if (compensated) {
+        RM_HIP(c->comp.ensure((size_t)px * 48), "allocating the compensation terms");
+        comp = c->comp.as<float4>();
+        if (!c->comp_live && P.sample_offset != 0u) rm::launch_fill_u32(c->comp.as<uint32_t>(), 0u, (size_t)px * 12, c->stream);
+    }
+    c->comp_live = compensated;
     {
         Timed t(c, RM_STAGE_TRACE);
-        rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(),
+        rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), comp,
                          c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging, f.quantized};
         rm::launch_main(L, f, c->stream);
     }
@@ -457,13 +493,14 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         RM_HIP(hipStreamWaitEvent(c->stream, c->join, 0), "joining the side stream");
         {
             Timed t(c, RM_STAGE_RESOLVE);
-            rm::launch_resolve(P, c->splats.as<uint4>(), c->gathered.as<const float4>(), c->accum.as<float4>(),
+            rm::launch_resolve(P, c->splats.as<uint4>(), c->gathered.as<const float4>(), c->accum.as<float4>(), comp,
                                c->stream);
         }
         RM_HIP(hipGetLastError(), "launching the splat resolve");
     }
 
     if (P.denoise != 0u) {
+        const bool fp16 = (c->options[RM_OPTION_PRECISION] & RM_PRECISION_FP16_GUIDES) != 0;
         float4* a = c->light_a.as<float4>();
         float4* b = c->light_b.as<float4>();
         RM_HIP(c->albedo_frame.ensure((size_t)c->width * c->height * 16), "allocating the denoiser's albedo");
@@ -471,9 +508,11 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "ordering the denoiser");
         {
             Timed t(c, RM_STAGE_PREPARE);
-            rm::launch_guides(P, c->gnorm.as<const float4>(), c->guides.as<float4>(), c->stream);
-            rm::launch_prepare(P, c->accum.as<const float4>(), c->galb.as<const float4>(), c->guides.as<const float4>(),
-                               a, c->albedo_frame.as<float4>(), c->stream);
+            const float4* sums[3];
+            int r = rmi::current_sums(c, c->stream, sums);
+            if (r != RM_OK) return r;
+            rm::launch_guides(P, sums[2], c->guides.ptr, fp16, c->stream);
+            rm::launch_prepare(P, sums[0], sums[1], c->guides.ptr, fp16, a, c->albedo_frame.as<float4>(), c->stream);
         }
         RM_HIP(hipEventRecord(c->prepared_ev, c->stream), "ordering the denoiser");
         RM_HIP(hipStreamWaitEvent(c->post, c->prepared_ev, 0), "ordering the denoiser");
@@ -482,7 +521,7 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         float4* out = b;
         for (int k = 0; k < 5; ++k) {
             Timed t(c, RM_STAGE_ATROUS, c->post);
-            rm::launch_atrous(P, steps[k], c->guides.as<const float4>(), in, out, c->post);
+            rm::launch_atrous(P, steps[k], c->guides.ptr, fp16, in, out, c->post);
             std::swap(in, out);
         }
         {
@@ -501,19 +540,20 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
 int rm_read(rm_context* c, int buffer, float* dst, size_t floats) {
     if (!c || !dst) return fail(RM_ERR_INVALID, "rm_read needs a context and a destination.");
     if (c->width == 0) return fail(RM_ERR_STATE, "Nothing has been rendered yet.");
-    DevBuf* b = nullptr;
-    switch (buffer) {
-    case RM_BUFFER_ACCUM: b = &c->accum; break;
-    case RM_BUFFER_GUIDE_ALBEDO: b = &c->galb; break;
-    case RM_BUFFER_GUIDE_NORMAL: b = &c->gnorm; break;
-    case RM_BUFFER_DENOISED: b = &c->denoised; break;
-    default: return fail(RM_ERR_INVALID, "Unknown buffer " + std::to_string(buffer) + ".");
-    }
+    if (buffer < RM_BUFFER_ACCUM || buffer > RM_BUFFER_DENOISED)
+        return fail(RM_ERR_INVALID, "Unknown buffer " + std::to_string(buffer) + ".");
     size_t have = (size_t)c->width * c->height * 4;
     if (floats > have) return fail(RM_ERR_INVALID, "rm_read asked for more than the image holds.");
     RM_HIP(hipSetDevice(c->device), "selecting the device");
     if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "waiting for the denoiser");
-    RM_HIP(hipMemcpyAsync(dst, b->ptr, floats * 4, hipMemcpyDeviceToHost, c->stream), "reading back pixels");
+    const void* src = c->denoised.ptr;
+    if (buffer != RM_BUFFER_DENOISED) {
+        const float4* sums[3];
+        int r = rmi::current_sums(c, c->stream, sums);
+        if (r != RM_OK) return r;
+        src = sums[buffer == RM_BUFFER_ACCUM ? 0 : buffer == RM_BUFFER_GUIDE_ALBEDO ? 1 : 2];
+    }
+    RM_HIP(hipMemcpyAsync(dst, src, floats * 4, hipMemcpyDeviceToHost, c->stream), "reading back pixels");
     RM_HIP(hipStreamSynchronize(c->stream), "reading back pixels");
     return RM_OK;
 }
@@ -529,8 +569,10 @@ int rm_display(rm_context* c, uint8_t* rgba, size_t bytes) {
     RM_HIP(hipSetDevice(c->device), "selecting the device");
     RM_HIP(c->display.ensure(need), "allocating the display image");
     if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "waiting for the denoiser");
-    rm::launch_display(P, c->accum.as<const float4>(), c->denoised.as<const float4>(), c->display.as<uchar4>(),
-                       c->stream);
+    const float4* sums[3];
+    int r = rmi::current_sums(c, c->stream, sums);
+    if (r != RM_OK) return r;
+    rm::launch_display(P, sums[0], c->denoised.as<const float4>(), c->display.as<uchar4>(), c->stream);
     RM_HIP(hipGetLastError(), "launching the display pass");
     RM_HIP(hipMemcpyAsync(rgba, c->display.ptr, need, hipMemcpyDeviceToHost, c->stream), "reading back the display");
     RM_HIP(hipStreamSynchronize(c->stream), "waiting for the GPU");

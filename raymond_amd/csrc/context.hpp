@@ -81,6 +81,11 @@ struct rm_context {
     rmi::DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
     rmi::DevBuf rays, hits, counters, work, staging, display;
+    // Compensated sums (RM_PRECISION_COMPENSATED): comp holds 3 float4 of
+    // rounding error a pixel for accum, galb and gnorm; comp_live says the
+    // last frame kept it up to date; folded holds sum + error for readers.
+    rmi::DevBuf comp, folded;
+    bool comp_live = false;
     rm::Params last_params = {};   // the last frame's, for rm_display
     bool rendered = false;
     int compute_units = 0;
@@ -89,3 +94,10 @@ struct rm_context {
     rmi::StageTimer timer;
     std::vector<rm_scene*> scenes;
 };
+
+namespace rmi {
+// The per-pixel sums (accum, guide albedo, guide normal) at their best FP32
+// value, ordered on `stream`: the running sums themselves, or when the last
+// frame kept compensated sums, sum + error folded into ctx->folded.
+int current_sums(rm_context* ctx, hipStream_t stream, const float4* out[3]);
+}  // namespace rmi

@@ -9,6 +9,8 @@
 //   k_prepare / k_atrous / k_compose   the à-trous denoiser
 // Each kernel restates the reference entry point cited beside it
 // (src/shaders.mjs). Traversal stacks live in LDS with a global spill.
+#include <hip/hip_fp16.h>
+
 #include "kernels.hpp"
 
 namespace rm {
@@ -318,6 +320,27 @@ RM_D V3 inspect(const Ctx& C, Rng& g, STACK& stack, V2 px) {
 }
 
 // A pixel's sums over the frame's samples (src/shaders.mjs:1846-1894).
+// sum += v with the rounding error added to err: Knuth's TwoSum, exact in
+// IEEE FP32 arithmetic without contraction (this file is compiled with
+// -ffp-contract=off), so sum + err carries the true running sum to about one
+// rounding of the total however many frames are added.
+RM_D float two_sum(float& sum, float v) {
+    const float t = sum + v;
+    const float z = t - sum;
+    const float e = (sum - (t - z)) + (v - z);
+    sum = t;
+    return e;
+}
+RM_D void compensated_add(float4& sum_ref, float4& err_ref, float4 v) {
+    float4 s = sum_ref, e = err_ref;
+    e.x += two_sum(s.x, v.x);
+    e.y += two_sum(s.y, v.y);
+    e.z += two_sum(s.z, v.z);
+    e.w += two_sum(s.w, v.w);
+    sum_ref = s;
+    err_ref = e;
+}
+
 struct PixelSums {
     V3 c, albedo;
     float w;
@@ -344,17 +367,32 @@ struct PixelSums {
                                  normal.w + gd.normal.w);
         }
     }
-    // The first frame (sample offset 0) overwrites; later ones add.
-    RM_D void store(const Params& P, uint32_t index, float4* accum, float4* galb, float4* gnorm) const {
+    // The first frame (sample offset 0) overwrites; later ones add. With
+    // comp (3 float4 a pixel), each add is compensated: comp gathers the
+    // rounding error of every add to the running sums.
+    RM_D void store(const Params& P, uint32_t index, float4* accum, float4* galb, float4* gnorm,
+                    float4* comp) const {
+        const float4 add_a = make_float4(c.x, c.y, c.z, w);
+        const float4 add_g = make_float4(albedo.x, albedo.y, albedo.z, 0.0f);
         if (P.sample_offset == 0u) {
-            accum[index] = make_float4(c.x, c.y, c.z, w);
-            galb[index] = make_float4(albedo.x, albedo.y, albedo.z, 0.0f);
+            accum[index] = add_a;
+            galb[index] = add_g;
             gnorm[index] = normal;
-        } else {
+            if (comp) {
+                const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                comp[(size_t)index * 3] = zero;
+                comp[(size_t)index * 3 + 1] = zero;
+                comp[(size_t)index * 3 + 2] = zero;
+            }
+        } else if (!comp) {
             float4 a = accum[index], ga = galb[index], gn = gnorm[index];
             accum[index] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + w);
             galb[index] = make_float4(ga.x + albedo.x, ga.y + albedo.y, ga.z + albedo.z, ga.w + 0.0f);
             gnorm[index] = make_float4(gn.x + normal.x, gn.y + normal.y, gn.z + normal.z, gn.w + normal.w);
+        } else {
+            compensated_add(accum[index], comp[(size_t)index * 3], add_a);
+            compensated_add(galb[index], comp[(size_t)index * 3 + 1], add_g);
+            compensated_add(gnorm[index], comp[(size_t)index * 3 + 2], normal);
         }
     }
 };
@@ -371,7 +409,7 @@ struct PixelSums {
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, bool QUANT>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main(
     DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
-    float4* __restrict__ guide_normal, uint32_t* spill, uint32_t slots) {
+    float4* __restrict__ guide_normal, float4* __restrict__ comp, uint32_t* spill, uint32_t slots) {
     __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     uint32_t x, y;
     tile_pixel(x, y);
@@ -396,7 +434,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
             sums.add(P, inspect<TEXTURES, ENV>(C, g, stack, {(float)x + 0.5f, (float)y + 0.5f}), gd, false);
         }
     }
-    sums.store(P, index, accum, guide_albedo, guide_normal);
+    sums.store(P, index, accum, guide_albedo, guide_normal, comp);
 }
 
 // The same samples with the block's live paths packed together between
@@ -477,7 +515,8 @@ RM_D void slot_load(const SlotStore& st, uint32_t slot, const Params& P, uint32_
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main_compact(
     DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
-    float4* __restrict__ guide_normal, uint32_t* spill, uint32_t slots, float4* __restrict__ staging) {
+    float4* __restrict__ guide_normal, float4* __restrict__ comp, uint32_t* spill, uint32_t slots,
+    float4* __restrict__ staging) {
     __shared__ uint2 lds[COMPACT_LDS_DEPTH * BLOCK_THREADS];
     __shared__ uint32_t slot_words[SLOT_WORDS * BLOCK_THREADS];
     __shared__ uint32_t wave_live[BLOCK_THREADS / 64];
@@ -538,7 +577,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
         }
         __syncthreads();
     }
-    if (valid) sums.store(P, index, accum, guide_albedo, guide_normal);
+    if (valid) sums.store(P, index, accum, guide_albedo, guide_normal, comp);
 }
 
 // The same samples with lanes kept busy: waves claim 8×8 pixel tiles from a
@@ -552,7 +591,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, Params P, float4* __restrict__ accum,
                                                                    float4* __restrict__ guide_albedo,
-                                                                   float4* __restrict__ guide_normal, uint32_t* spill,
+                                                                   float4* __restrict__ guide_normal,
+                                                                   float4* __restrict__ comp, uint32_t* spill,
                                                                    uint32_t slots, uint32_t* counter) {
     __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     const Ctx C{S, &P};
@@ -609,7 +649,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, P
             if (++k < P.samples_per_frame) {
                 path_begin(P, g, ps, gd, x, y, index, P.sample_offset + k);
             } else {
-                sums.store(P, index, accum, guide_albedo, guide_normal);
+                sums.store(P, index, accum, guide_albedo, guide_normal, comp);
                 alive = false;
             }
         }
@@ -835,7 +875,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Param
 // gather pass leaves its light in `gathered` (w = 1 where it adds any), so it
 // can run while the path-tracing pass still writes the accumulation.
 __global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* splats, const float4* gathered,
-                                                           float4* accum) {
+                                                           float4* accum, float4* comp) {
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     if (i >= P.width * P.height) return;
     uint4 s = splats[i];
@@ -843,15 +883,42 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* spla
     V3 light = v3((float)s.x, (float)s.y, (float)s.z) / SPLAT_SCALE;
     float l = luminance(light);
     float spp = (float)P.samples_per_frame;
-    float4 a = accum[i];
-    a = make_float4(a.x + light.x * spp, a.y + light.y * spp, a.z + light.z * spp, a.w + l * l * spp);
+    const float4 splat = make_float4(light.x * spp, light.y * spp, light.z * spp, l * l * spp);
     float4 g = gathered[i];
+    float4 gather = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (g.w != 0.0f) {
         V3 gl = v3(g.x, g.y, g.z);
         float lg = luminance(gl);
-        a = make_float4(a.x + gl.x * spp, a.y + gl.y * spp, a.z + gl.z * spp, a.w + lg * lg * spp);
+        gather = make_float4(gl.x * spp, gl.y * spp, gl.z * spp, lg * lg * spp);
+    }
+    float4 a = accum[i];
+    if (comp) {
+        float4 e = comp[(size_t)i * 3];
+        compensated_add(a, e, splat);
+        if (g.w != 0.0f) compensated_add(a, e, gather);
+        comp[(size_t)i * 3] = e;
+    } else {
+        a = make_float4(a.x + splat.x, a.y + splat.y, a.z + splat.z, a.w + splat.w);
+        if (g.w != 0.0f) a = make_float4(a.x + gather.x, a.y + gather.y, a.z + gather.z, a.w + gather.w);
     }
     accum[i] = a;
+}
+
+// The compensated sums' best FP32 value, sum + error, for their readers
+// (the denoiser, readback, display): three planes of W·H float4.
+__global__ __launch_bounds__(BLOCK_THREADS) void k_fold(Params P, const float4* __restrict__ accum,
+                                                        const float4* __restrict__ galb,
+                                                        const float4* __restrict__ gnorm,
+                                                        const float4* __restrict__ comp, float4* __restrict__ out) {
+    const uint32_t n = P.width * P.height;
+    const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const float4* src[3] = {accum, galb, gnorm};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float4 s = src[k][i], e = comp[(size_t)i * 3 + k];
+        out[(size_t)k * n + i] = make_float4(s.x + e.x, s.y + e.y, s.z + e.z, s.w + e.w);
+    }
 }
 
 // The photons within the radius of a surface, times the BSDF, over the
@@ -970,21 +1037,45 @@ RM_D float pow64(float x) {
     return x;
 }
 
+// How the per-frame guide (unit normal, mean distance) is stored for the
+// 25-tap passes: FP32, or FP16 (half the bytes every tap reads). The sums
+// it comes from stay FP32, and the passes compute in FP32 either way.
+struct GuidesF32 {
+    using T = float4;
+    RM_D static float4 get(const T* g, uint32_t i) { return g[i]; }
+    RM_D static T put(float4 v) { return v; }
+};
+struct GuidesF16 {
+    using T = ushort4;
+    RM_D static float wide(unsigned short h) { return __half2float(__ushort_as_half(h)); }
+    // Round to nearest; distances past FP16's range clamp to its largest finite value.
+    RM_D static unsigned short narrow(float f) {
+        return __half_as_ushort(__float2half_rn(clampf(f, -65504.0f, 65504.0f)));
+    }
+    RM_D static float4 get(const T* g, uint32_t i) {
+        const ushort4 u = g[i];
+        return make_float4(wide(u.x), wide(u.y), wide(u.z), wide(u.w));
+    }
+    RM_D static T put(float4 v) { return make_ushort4(narrow(v.x), narrow(v.y), narrow(v.z), narrow(v.w)); }
+};
+
 // Each pixel's normal_at() once per frame, so the 25-tap passes read it
 // rather than renormalising every neighbour's guide.
+template <class G>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_guides(Params P, const float4* __restrict__ gnorm,
-                                                          float4* __restrict__ guides) {
+                                                          typename G::T* __restrict__ guides) {
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     if (i >= P.width * P.height) return;
-    guides[i] = normal_at(gnorm, i, (float)max(P.total_samples, 1u));
+    guides[i] = G::put(normal_at(gnorm, i, (float)max(P.total_samples, 1u)));
 }
 
 // src/shaders.mjs:1974-2021. The pixel's albedo (albedo_at) also goes to
 // albedo_out for k_compose, so that the passes after this one no longer read
 // the accumulation buffers the next frame writes.
+template <class G>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float4* __restrict__ accum,
                                                            const float4* __restrict__ galb,
-                                                           const float4* __restrict__ guides, float4* light_out,
+                                                           const typename G::T* __restrict__ guides, float4* light_out,
                                                            float4* __restrict__ albedo_out) {
     uint32_t x, y;
     if (!stencil_pixel(P.width, P.height, x, y)) return;
@@ -994,7 +1085,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
     const V3 own_albedo = albedo_at(galb, i, n);
     albedo_out[i] = make_float4(own_albedo.x, own_albedo.y, own_albedo.z, 0.0f);
     V3 light = (v3(ac.x, ac.y, ac.z) / n) / own_albedo;
-    float4 gp = guides[i];
+    float4 gp = G::get(guides, i);
     float a = luminance(own_albedo);
     float variance;
     if (n >= 4.0f) {
@@ -1008,7 +1099,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
                 int xx = (int)x + dx, yy = (int)y + dy;
                 if (xx < 0 || yy < 0 || xx >= (int)P.width || yy >= (int)P.height) continue;
                 uint32_t j = (uint32_t)yy * P.width + (uint32_t)xx;
-                float w = similarity(gp, guides[j], 1.0f);
+                float w = similarity(gp, G::get(guides, j), 1.0f);
                 float4 aj = accum[j];
                 float l = luminance((v3(aj.x, aj.y, aj.z) / n) / albedo_at(galb, j, n));
                 sum += w * l;
@@ -1026,14 +1117,15 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
 // the power by six squarings, the two exponentials merged into one exp2,
 // and the centre's divisions hoisted to reciprocals: a few ulps from the
 // reference's per-tap pow/exp/divide, several times fewer instructions.
-__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const float4* __restrict__ guides,
+template <class G>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const typename G::T* __restrict__ guides,
                                                           const float4* __restrict__ light_in,
                                                           float4* __restrict__ light_out) {
     uint32_t x, y;
     if (!stencil_pixel(P.width, P.height, x, y)) return;
     const uint32_t i = y * P.width + x;
     const float4 centre = light_in[i];
-    const float4 gp = guides[i];
+    const float4 gp = G::get(guides, i);
     const float lp = luminance(v3(centre.x, centre.y, centre.z));
     const int W = (int)P.width, H = (int)P.height;
     float variance = 0.0f, variance_weight = 0.0f;
@@ -1064,7 +1156,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, co
             if (xx < 0 || xx >= W) continue;
             uint32_t j = (uint32_t)yy * P.width + (uint32_t)xx;
             float4 q = light_in[j];
-            float4 g = guides[j];
+            float4 g = G::get(guides, j);
             float e = -fabsf(lp - luminance(v3(q.x, q.y, q.z))) * k_l;
             float s;
             if (sky || g.w <= 0.0f) {
@@ -1153,21 +1245,21 @@ template <bool V, bool T, bool E, bool D>
 static void launch_main_v(const LaunchMain& L, hipStream_t stream) {
     if (L.persistent_blocks > 0 && L.P.view == VIEW_COLOR) {
         hipLaunchKernelGGL((k_main_persistent<V, T, E, D>), dim3(L.persistent_blocks), dim3(BLOCK_THREADS), 0, stream,
-                           L.S, L.P, L.accum, L.galb, L.gnorm, L.spill, L.slots, L.counter);
+                           L.S, L.P, L.accum, L.galb, L.gnorm, L.comp, L.spill, L.slots, L.counter);
         return;
     }
     dim3 grid((L.P.width + 15) / 16, (L.P.height + 15) / 16);
     if (L.staging && L.P.view == VIEW_COLOR) {
         hipLaunchKernelGGL((k_main_compact<V, T, E, D>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
-                           L.galb, L.gnorm, L.spill, L.slots, L.staging);
+                           L.galb, L.gnorm, L.comp, L.spill, L.slots, L.staging);
         return;
     }
     if (L.quantized)
         hipLaunchKernelGGL((k_main<V, T, E, D, true>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
-                           L.galb, L.gnorm, L.spill, L.slots);
+                           L.galb, L.gnorm, L.comp, L.spill, L.slots);
     else
         hipLaunchKernelGGL((k_main<V, T, E, D, false>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
-                           L.galb, L.gnorm, L.spill, L.slots);
+                           L.galb, L.gnorm, L.comp, L.spill, L.slots);
 }
 
 template <bool V, bool T, bool E, bool D>
@@ -1248,10 +1340,18 @@ void launch_light_trace(const DevScene& S, const Params& P, const float4* emitte
         launch_light_trace_v<false, false>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.quantized, stream);
 }
 
-void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, hipStream_t stream) {
+void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, float4* comp,
+                    hipStream_t stream) {
     uint32_t n = P.width * P.height;
     hipLaunchKernelGGL(k_resolve, dim3((n + BLOCK_THREADS - 1) / BLOCK_THREADS), dim3(BLOCK_THREADS), 0, stream, P,
-                       splats, gathered, accum);
+                       splats, gathered, accum, comp);
+}
+
+void launch_fold(const Params& P, const float4* accum, const float4* galb, const float4* gnorm, const float4* comp,
+                 float4* out, hipStream_t stream) {
+    uint32_t n = P.width * P.height;
+    hipLaunchKernelGGL(k_fold, dim3((n + BLOCK_THREADS - 1) / BLOCK_THREADS), dim3(BLOCK_THREADS), 0, stream, P,
+                       accum, galb, gnorm, comp, out);
 }
 
 void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
@@ -1269,21 +1369,37 @@ void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* he
     }
 }
 
-void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStream_t stream) {
+void launch_guides(const Params& P, const float4* gnorm, void* guides, bool fp16, hipStream_t stream) {
     uint32_t n = P.width * P.height;
-    hipLaunchKernelGGL(k_guides, dim3((n + BLOCK_THREADS - 1) / BLOCK_THREADS), dim3(BLOCK_THREADS), 0, stream, P,
-                       gnorm, guides);
+    dim3 grid((n + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    if (fp16)
+        hipLaunchKernelGGL(k_guides<GuidesF16>, grid, dim3(BLOCK_THREADS), 0, stream, P, gnorm,
+                           static_cast<ushort4*>(guides));
+    else
+        hipLaunchKernelGGL(k_guides<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, gnorm,
+                           static_cast<float4*>(guides));
 }
 
-void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* guides, float4* out,
-                    float4* albedo_out, hipStream_t stream) {
+void launch_prepare(const Params& P, const float4* accum, const float4* galb, const void* guides, bool fp16,
+                    float4* out, float4* albedo_out, hipStream_t stream) {
     dim3 grid(stencil_blocks(P.width, P.height));
-    hipLaunchKernelGGL(k_prepare, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb, guides, out, albedo_out);
+    if (fp16)
+        hipLaunchKernelGGL(k_prepare<GuidesF16>, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb,
+                           static_cast<const ushort4*>(guides), out, albedo_out);
+    else
+        hipLaunchKernelGGL(k_prepare<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, accum, galb,
+                           static_cast<const float4*>(guides), out, albedo_out);
 }
 
-void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream) {
+void launch_atrous(const Params& P, int step, const void* guides, bool fp16, const float4* in, float4* out,
+                   hipStream_t stream) {
     dim3 grid(stencil_blocks(P.width, P.height));
-    hipLaunchKernelGGL(k_atrous, grid, dim3(BLOCK_THREADS), 0, stream, P, step, guides, in, out);
+    if (fp16)
+        hipLaunchKernelGGL(k_atrous<GuidesF16>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
+                           static_cast<const ushort4*>(guides), in, out);
+    else
+        hipLaunchKernelGGL(k_atrous<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
+                           static_cast<const float4*>(guides), in, out);
 }
 
 void launch_display(const Params& P, const float4* accum, const float4* denoised, uchar4* out, hipStream_t stream) {

@@ -30,6 +30,7 @@ struct LaunchMain {
     float4* accum;
     float4* galb;
     float4* gnorm;
+    float4* comp;                // non-null: compensated sums, 3 float4 of rounding error a pixel
     uint32_t* spill;
     uint32_t slots;
     uint32_t persistent_blocks;  // > 0: k_main_persistent with this grid
@@ -46,13 +47,19 @@ void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hi
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
                         float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
                         hipStream_t stream);
-void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, hipStream_t stream);
+void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, float4* comp,
+                    hipStream_t stream);
+// sum + error of the compensated sums, into out: planes accum, galb, gnorm of W·H float4.
+void launch_fold(const Params& P, const float4* accum, const float4* galb, const float4* gnorm, const float4* comp,
+                 float4* out, hipStream_t stream);
 void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
                           float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream);
-void launch_guides(const Params& P, const float4* gnorm, float4* guides, hipStream_t stream);
-void launch_prepare(const Params& P, const float4* accum, const float4* galb, const float4* guides, float4* out,
-                    float4* albedo_out, hipStream_t stream);
-void launch_atrous(const Params& P, int step, const float4* guides, const float4* in, float4* out, hipStream_t stream);
+// fp16: the guide buffer holds FP16 (8 bytes a pixel) rather than FP32.
+void launch_guides(const Params& P, const float4* gnorm, void* guides, bool fp16, hipStream_t stream);
+void launch_prepare(const Params& P, const float4* accum, const float4* galb, const void* guides, bool fp16,
+                    float4* out, float4* albedo_out, hipStream_t stream);
+void launch_atrous(const Params& P, int step, const void* guides, bool fp16, const float4* in, float4* out,
+                   hipStream_t stream);
 void launch_compose(const Params& P, const float4* albedo, const float4* in, float4* out, hipStream_t stream);
 void launch_display(const Params& P, const float4* accum, const float4* denoised, uchar4* out, hipStream_t stream);
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);

@@ -225,7 +225,15 @@ bool rm_server::progress(Slot& s) {
             s.pinned_floats = floats;
         }
         hipStream_t last = j.denoise ? s.ctx->post : s.ctx->stream;
-        const void* src = j.denoise ? s.ctx->denoised.ptr : s.ctx->accum.ptr;
+        const void* src = s.ctx->denoised.ptr;
+        if (!j.denoise) {
+            const float4* sums[3];
+            if (rmi::current_sums(s.ctx, last, sums) != RM_OK) {
+                finish(s, RM_ERR_DEVICE, std::string("reading back an image: ") + rm_last_error());
+                return true;
+            }
+            src = sums[0];
+        }
         hipError_t e = hipMemcpyAsync(s.pinned, src, floats * 4, hipMemcpyDeviceToHost, last);
         if (e == hipSuccess) e = hipEventRecord(s.readback_ev, last);
         if (e != hipSuccess) {

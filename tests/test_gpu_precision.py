@@ -1,0 +1,137 @@
+"""FP16 denoiser guides and compensated FP32 sums (RM_OPTION_PRECISION),
+each with its error against an FP64 CPU oracle.
+
+* FP16 guides: the denoised image from the GPU's own sums, with FP32 and
+  with FP16 guides, against oracle_denoise64 on the same sums.
+* Compensated sums: a pixel that sees an emitter head-on adds the same
+  FP32 value every frame, so after N frames the exact sum is N·v (FP64);
+  the compensated sum rounds to it, the plain one drifts. On the Cornell
+  box, both against FP64 sums of the oracle's per-frame values."""
+import numpy as np
+import pytest
+
+from helpers import frame_params, gpu_render
+from oracle.pyoracle import Oracle
+from raymond_amd import capi
+from raymond_amd.scenefile import set_params
+
+pytestmark = pytest.mark.gpu
+
+
+class precision:
+    def __init__(self, ctx, flags):
+        self.ctx, self.flags = ctx, flags
+
+    def __enter__(self):
+        self.ctx.set_option(capi.OPTION_PRECISION, self.flags)
+
+    def __exit__(self, *exc):
+        self.ctx.set_option(capi.OPTION_PRECISION, 0)
+
+
+def rel_error(a, ref):
+    a = a[..., :3].astype(np.float64)
+    ref = ref[..., :3]
+    return np.abs(a - ref) / np.maximum(np.abs(ref), 1e-3)
+
+
+@pytest.mark.parametrize("frames", [2, 6])
+def test_fp16_guides_against_the_fp64_denoiser(ctx, pack, frames):
+    p = pack("cornell", width=96, height=96)
+    params = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
+    acc, galb, gnorm, d32 = gpu_render(ctx, p, frames, params, denoised=True)
+    with precision(ctx, capi.PRECISION_FP16_GUIDES):
+        acc16, galb16, gnorm16, d16 = gpu_render(ctx, p, frames, params, denoised=True)
+    # The sums don't depend on the guides' storage.
+    np.testing.assert_array_equal(acc16, acc)
+    np.testing.assert_array_equal(gnorm16, gnorm)
+    ref = Oracle.denoise64(frame_params(p, frames - 1, params), acc, galb, gnorm)
+    e32, e16 = rel_error(d32, ref), rel_error(d16, ref)
+    print(f"\n{frames} frames: FP32 guides rel. error mean {e32.mean():.2e} p99 {np.percentile(e32, 99):.2e} "
+          f"max {e32.max():.2e}; FP16 guides mean {e16.mean():.2e} p99 {np.percentile(e16, 99):.2e} max {e16.max():.2e}")
+    assert e32.mean() < 1e-5 and np.percentile(e32, 99) < 1e-4
+    assert e16.mean() < 5e-3 and np.percentile(e16, 99) < 5e-2
+    assert (d16 != d32).any(), "FP16 guides should change some pixels"
+
+
+def emitter_wall(pack, frames_total):
+    """A 64×64 view filled by an emitter facing the camera, radiance (0.17, 0.51, 1.19)."""
+    objects = [{"type": "quad", "corner": [-20, -20, 0], "u": [40, 0, 0], "v": [0, 40, 0],
+                "material": {"type": "light", "emission": [0.17, 0.51, 1.19]}}]
+    return pack("custom", scene=dict(objects=objects, sky="black", maxBounces=1), width=64, height=64)
+
+
+def test_compensated_sums_round_to_the_exact_sum(ctx, pack):
+    frames = 1536
+    p = emitter_wall(pack, frames)
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0)
+    plain = gpu_render(ctx, p, frames, params)[0]
+    with precision(ctx, capi.PRECISION_COMPENSATED):
+        comp = gpu_render(ctx, p, frames, params)[0]
+    one = gpu_render(ctx, p, 1, params)[0]
+    v = one[32, 32].astype(np.float64)
+    assert (one == one[32, 32]).all() and v[0] > 0, "every pixel sees the emitter, the same value each frame"
+    exact = frames * v
+    ulp = np.spacing(exact.astype(np.float32)).astype(np.float64)
+    err_comp = np.abs(comp[32, 32] - exact) / ulp
+    err_plain = np.abs(plain[32, 32] - exact) / ulp
+    print(f"\n{frames} frames of {v}: plain off by {err_plain} ulp, compensated by {err_comp} ulp")
+    assert (err_comp <= 0.5).all(), "the compensated sum is the float nearest the exact sum"
+    assert err_plain.max() >= 4, "the plain sum drifts"
+    assert (comp == comp[32, 32]).all()
+
+
+def test_compensated_cornell_against_fp64_sums(ctx, pack):
+    frames = 64
+    p = pack("cornell", width=48, height=48)
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0)
+    plain = gpu_render(ctx, p, frames, params)
+    with precision(ctx, capi.PRECISION_COMPENSATED):
+        comp = gpu_render(ctx, p, frames, params)
+    o = Oracle(p)
+    exact = [np.zeros((48, 48, 4), np.float64) for _ in range(3)]
+    for k in range(frames):
+        bufs = [np.zeros((48, 48, 4), np.float32) for _ in range(3)]
+        fp = frame_params(p, k, params)
+        if k > 0:
+            fp = set_params(fp.copy(), sample_offset=k)
+        o.main(fp, *bufs)
+        for e, b in zip(exact, bufs):
+            e += b
+    for name, i in (("radiance", 0), ("guide albedo", 1), ("guide normal", 2)):
+        ep, ec = rel_error(plain[i], exact[i]), rel_error(comp[i], exact[i])
+        print(f"\n{name}: plain median {np.median(ep):.2e}, compensated median {np.median(ec):.2e}")
+        assert np.median(ec) <= np.median(ep)
+    assert np.median(rel_error(comp[0], exact[0])) < 1e-7
+
+
+def test_compensated_schedules_agree_and_start_mid_run(ctx, pack):
+    """The three path-pass schedules keep the same compensated sums; turning
+    compensation on part-way starts its error terms from zero."""
+    p = pack("mixed", width=64, height=64)
+    params = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
+    outs = []
+    for schedule in (capi.SCHEDULE_PIXEL, capi.SCHEDULE_COMPACT, capi.SCHEDULE_PERSISTENT):
+        ctx.set_option(capi.OPTION_SCHEDULE, schedule)
+        try:
+            with precision(ctx, capi.PRECISION_COMPENSATED):
+                outs.append(gpu_render(ctx, p, 3, params, denoised=True))
+        finally:
+            ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            np.testing.assert_array_equal(a, b)
+    plain = gpu_render(ctx, p, 3, params)[0]
+    scene = ctx.upload_scene(p)
+    try:
+        ctx.upload_environment(p)
+        ctx.resize(p.width, p.height)
+        for k in range(3):
+            if k == 1:
+                ctx.set_option(capi.OPTION_PRECISION, capi.PRECISION_COMPENSATED)
+            ctx.frame(scene, frame_params(p, k, params))
+        late = ctx.read(capi.BUFFER_ACCUM)
+    finally:
+        ctx.set_option(capi.OPTION_PRECISION, 0)
+        scene.release()
+    np.testing.assert_allclose(late, plain, rtol=1e-6, atol=1e-6)
