@@ -299,11 +299,23 @@ def serving_trace(rate: float, duration: float, spp_lo: int, spp_hi: int, seed: 
                      int(rng.integers(spp_lo, spp_hi + 1)), int(rng.integers(4, 13))))
 
 
-def run_trace(packs, trace, slots: int, budget: int) -> dict:
+PRECISION_FLAGS = {"quantized-bvh": 1, "fp16-guides": 2, "compensated": 4}
+
+
+def precision_flags(names: str) -> int:
+    flags = 0
+    for n in filter(None, names.split(",")):
+        flags |= PRECISION_FLAGS[n]
+    return flags
+
+
+def run_trace(packs, trace, slots: int, budget: int, precision: int = 0) -> dict:
     """Submits the trace's jobs at their arrival times to a server with `slots`
-    slots; returns throughput and latency percentiles."""
+    slots and RM_OPTION_PRECISION flags `precision`; returns throughput and
+    latency percentiles."""
     from raymond_amd import capi, scenefile
     with capi.Server(slots=slots, memory_budget=budget) as server:
+        server.set_option(capi.OPTION_PRECISION, precision)
         scenes = [server.context.upload_scene(p) for p in packs]
         jobs = []
         for arrival, k, (w, h), spp, bounces in trace:
@@ -324,7 +336,7 @@ def run_trace(packs, trace, slots: int, budget: int) -> dict:
     first = np.array([s["first_frame_ms"] for s in stats])
     done = np.array([s["done_ms"] for s in stats])
     return {"Msamples_per_s": round(samples / elapsed / 1e6, 3), "wall_s": round(elapsed, 3), "jobs": len(jobs),
-            "samples": samples, "slots": slots,
+            "samples": samples, "slots": slots, "precision": precision,
             "first_frame_ms_p50": round(float(np.percentile(first, 50)), 2),
             "first_frame_ms_p99": round(float(np.percentile(first, 99)), 2),
             "done_ms_p50": round(float(np.percentile(done, 50)), 2),
@@ -381,7 +393,9 @@ def bench_serving(args) -> dict:
                                     caustics=True)) for c, o in SERVING_POOL]
     trace = serving_trace(args.rate, args.duration, args.spp_min, args.spp_max)
     budget = 200 << 30
-    served = run_trace(packs, trace, args.slots, budget)
+    flags = precision_flags(args.served_precision)
+    served = run_trace(packs, trace, args.slots, budget, flags)
+    served_fp32 = run_trace(packs, trace, args.slots, budget) if flags else served
     one_slot = run_trace(packs, trace, 1, budget)
     plain = run_plain(packs, trace)
     with capi.Context(0) as ctx:
@@ -405,9 +419,11 @@ def bench_serving(args) -> dict:
                    "sizes": [f"{w}x{h}" for w, h in SERVING_SIZES],
                    "scene_primitives": [p.primitive_count for p in packs], "slots": args.slots,
                    "memory_budget_bytes": budget, "device": device, "compute_units": cus,
-                   "path": "served: the plain FP32 path in concurrent slots, denoising a job's first and last "
-                           "frames; plain: one context, one job at a time, every frame denoised"},
+                   "served_precision": args.served_precision,
+                   "path": "served: concurrent slots, a job's first and last frames denoised, with the precision "
+                           "variants named; plain: one context, one job at a time, FP32, every frame denoised"},
         "served": served,
+        "served_fp32": served_fp32,
         "served_one_slot": one_slot,
         "plain": plain,
         "speedup_vs_plain": round(served["Msamples_per_s"] / max(plain["Msamples_per_s"], 1e-9), 3),
@@ -428,6 +444,9 @@ def main() -> int:
     ap.add_argument("--spp-min", type=int, default=4, help="serving: smallest target spp")
     ap.add_argument("--spp-max", type=int, default=64, help="serving: largest target spp")
     ap.add_argument("--slots", type=int, default=8, help="serving: concurrent job slots")
+    ap.add_argument("--served-precision", default="quantized-bvh,fp16-guides,compensated",
+                    help="serving: precision variants of the served run, comma-separated "
+                         "(quantized-bvh, fp16-guides, compensated; empty for plain FP32)")
     ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent"], default="per-pixel",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
                          "or lanes regenerating paths from a tile queue")

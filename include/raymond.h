@@ -222,6 +222,9 @@ int rm_server_submit(rm_server* server, const rm_job* job, uint64_t* id);
 /* Waits for a job and copies its image: the denoised buffer when the job's
  * params.denoise is set, else the accumulation (W·H·4 floats; dst may be NULL). */
 int rm_server_wait(rm_server* server, uint64_t id, float* dst, size_t floats, rm_job_stats* stats);
+/* Sets an RM_OPTION_* (as rm_set_option does for a context) for the jobs
+ * submitted from now on; each job keeps the options it was submitted with. */
+int rm_server_set_option(rm_server* server, int option, int64_t value);
 /* Waits for every job, then frees the server and its scenes. */
 int rm_server_close(rm_server* server);
 

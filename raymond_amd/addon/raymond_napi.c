@@ -519,6 +519,26 @@ static napi_value js_server_close(napi_env env, napi_callback_info info) {
     return undefined(env);
 }
 
+/* setOption(ctx, option, value) and serverSetOption(server, option, value):
+ * rm_set_option / rm_server_set_option. */
+static napi_value set_option(napi_env env, napi_callback_info info, int server) {
+    napi_value argv[3];
+    void* h;
+    uint32_t option;
+    int64_t value;
+    if (!args(env, info, 3, argv) || !get_ptr(env, argv[0], &h) || !get_u32(env, argv[1], &option)) return NULL;
+    if (napi_get_value_int64(env, argv[2], &value) != napi_ok) {
+        napi_throw_type_error(env, NULL, "Expected a number.");
+        return NULL;
+    }
+    int r = server ? rm_server_set_option((rm_server*)h, (int)option, value)
+                   : rm_set_option((rm_context*)h, (int)option, value);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
+}
+static napi_value js_set_option(napi_env env, napi_callback_info info) { return set_option(env, info, 0); }
+static napi_value js_server_set_option(napi_env env, napi_callback_info info) { return set_option(env, info, 1); }
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor props[] = {
         {"open", NULL, js_open, NULL, NULL, NULL, napi_enumerable, NULL},
@@ -535,6 +555,8 @@ static napi_value init(napi_env env, napi_value exports) {
         {"raycast", NULL, js_raycast, NULL, NULL, NULL, napi_enumerable, NULL},
         {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_enumerable, NULL},
         {"bvhBuild", NULL, js_bvh_build, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"setOption", NULL, js_set_option, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"serverSetOption", NULL, js_server_set_option, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverOpen", NULL, js_server_open, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverContext", NULL, js_server_context, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverSubmit", NULL, js_server_submit, NULL, NULL, NULL, napi_enumerable, NULL},

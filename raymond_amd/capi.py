@@ -46,6 +46,7 @@ SIGNATURES = {
     "rm_server_submit": (C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
     "rm_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t, _P]),
     "rm_server_close": (C.c_int, [_P]),
+    "rm_server_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
 OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION = 0, 1, 2
 PRECISION_QUANTIZED_BVH, PRECISION_FP16_GUIDES, PRECISION_COMPENSATED = 1, 2, 4
@@ -278,6 +279,10 @@ class Server:
         words = p.view(np.uint32)
         self.sizes[job_id.value] = (int(words[41]), int(words[40]))
         return job_id.value
+
+    def set_option(self, option: int, value: int) -> None:
+        """An OPTION_* for the jobs submitted from now on."""
+        _check(load().rm_server_set_option(self.handle, option, value), "rm_server_set_option")
 
     def wait(self, job_id: int):
         """(image as (H, W, 4) float32, timing dict) of a job, once it is done."""

@@ -42,6 +42,7 @@ struct Job {
     bool denoise = false, caustics = false;
     float radius0 = 0.0f;
     uint64_t bytes = 0;
+    int64_t options[RM_OPTION_COUNT] = {};   // the server's options when the job was submitted
     // progress, owned by the scheduler thread until `finished`
     uint32_t frames_enqueued = 0, frames_done = 0;
     Clock::time_point t_submit, t_admit, t_first, t_done;
@@ -62,9 +63,10 @@ struct Slot {
 };
 
 // Device memory a job's context needs, by the buffers rm_frame sizes for it.
-uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics) {
+uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, bool compensated) {
     const uint64_t px = (uint64_t)w * h;
     uint64_t b = px * (10 * 16 + rm::SPILL_ENTRIES * 8);
+    if (compensated) b += px * 96;   // error terms and their folded sums
     if (caustics) {
         const uint64_t photons = rm::PHOTON_CAPACITY;
         b += (px > photons ? px : photons) * rm::SPILL_ENTRIES * 8;
@@ -78,6 +80,7 @@ uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics) {
 struct rm_server {
     int device = 0;
     uint64_t budget = 0, used = 0;
+    int64_t options[RM_OPTION_COUNT] = {};   // given to jobs submitted from now on
     rm_context* home = nullptr;
     std::vector<Slot> slots;
     std::mutex m;
@@ -155,7 +158,9 @@ bool rm_server::admit() {
         }
         j->t_admit = Clock::now();
         any = true;
-        int r = rm_resize(s.ctx, j->width, j->height);
+        int r = RM_OK;
+        for (int k = 0; k < RM_OPTION_COUNT && r == RM_OK; ++k) r = rm_set_option(s.ctx, k, j->options[k]);
+        if (r == RM_OK) r = rm_resize(s.ctx, j->width, j->height);
         if (r == RM_OK) {
             if (!j->env_rgba.empty())
                 r = rm_environment(s.ctx, j->env_rgba.data(), j->env_w, j->env_h, j->env_table.data(), j->table_w,
@@ -345,7 +350,12 @@ int rm_server_submit(rm_server* sv, const rm_job* job, uint64_t* id) {
         j->table_w = job->table_width;
         j->table_h = job->table_height;
     }
-    j->bytes = job_bytes(j->width, j->height, j->caustics);
+    {
+        std::lock_guard<std::mutex> lock(sv->m);
+        std::memcpy(j->options, sv->options, sizeof j->options);
+    }
+    j->bytes = job_bytes(j->width, j->height, j->caustics,
+                         (j->options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0);
     // Scene uploads on the home context must have landed before any slot reads them.
     RM_HIP(hipSetDevice(sv->device), "selecting the device");
     RM_HIP(hipStreamSynchronize(sv->home->stream), "waiting for scene uploads");
@@ -385,6 +395,16 @@ int rm_server_wait(rm_server* sv, uint64_t id, float* dst, size_t floats, rm_job
         if (floats > j->image.size()) return rmi::fail(RM_ERR_INVALID, "rm_server_wait asked for more than the image holds.");
         std::memcpy(dst, j->image.data(), floats * 4);
     }
+    return RM_OK;
+}
+
+int rm_server_set_option(rm_server* sv, int option, int64_t value) {
+    if (!sv) return rmi::fail(RM_ERR_INVALID, "rm_server_set_option needs a server.");
+    // The home context checks the value exactly as rm_set_option would.
+    int r = rm_set_option(sv->home, option, value);
+    if (r != RM_OK) return r;
+    std::lock_guard<std::mutex> lock(sv->m);
+    sv->options[option] = value;
     return RM_OK;
 }
 

@@ -22,3 +22,14 @@ export function loadAddon() {
     }
     return addonCache;
 }
+
+/** RM_OPTION_PRECISION. */
+export const OPTION_PRECISION = 2;
+
+/**
+ * RM_PRECISION_* flags from { quantizedBvh, fp16Guides, compensated }: the
+ * 8-bit child-quantized BVH, FP16 denoiser guides, compensated FP32 sums.
+ */
+export function precisionFlags({ quantizedBvh = false, fp16Guides = false, compensated = false } = {}) {
+    return (quantizedBvh ? 1 : 0) | (fp16Guides ? 2 : 0) | (compensated ? 4 : 0);
+}

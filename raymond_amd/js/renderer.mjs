@@ -4,7 +4,7 @@
 // the frame itself (trace, light tracing, photon gather, denoise) runs as HIP
 // kernels in libraymond_hip.so, reached through the N-API addon raymond.node.
 // There is no other backend: without the addon or a GPU, create() rejects.
-import { loadAddon } from "./addon.mjs";
+import { OPTION_PRECISION, loadAddon, precisionFlags } from "./addon.mjs";
 import {
     IMAGE_KEY_BYTES, PARAMS_BYTES, VIEWS, deltaObjects, emitters, environmentData,
     sceneTables, textureLayers, writeFrameParams, writeParams,
@@ -48,7 +48,7 @@ export class Renderer {
     #addon; #ctx; #canvas;
     #scene = null; #camera = null;
     #maxBounces = 8; #samplesPerFrame = 1; #exposure = 1.0; #fireflyClamp = 20.0;
-    #view = "color"; #denoise = true; #caustics = true;
+    #view = "color"; #denoise = true; #caustics = true; #precision = {};
     #causticsActive = false;
     #samples = 0;
     #params = new ArrayBuffer(PARAMS_BYTES);
@@ -106,6 +106,15 @@ export class Renderer {
     set denoise(v) { this.#denoise = v; }
     get caustics() { return this.#caustics; }
     set caustics(v) { this.#caustics = v; }
+    /**
+     * Precision variants for the frames from now on, none by default:
+     * { quantizedBvh, fp16Guides, compensated } (RM_OPTION_PRECISION).
+     */
+    get precision() { return Object.assign({}, this.#precision); }
+    set precision(v) {
+        this.#addon.setOption(this.#ctx, OPTION_PRECISION, precisionFlags(v));
+        this.#precision = Object.assign({}, v);
+    }
     get samples() { return this.#samples; }
 
     reset() { this.#samples = 0; }

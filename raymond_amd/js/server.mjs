@@ -4,6 +4,7 @@
 // pixel; jobs submitted while others run are admitted as slots and memory
 // free up, and their frames overlap on the device. Scenes are packed and
 // uploaded once and stay resident while the server is open.
+import { OPTION_PRECISION, precisionFlags } from "./addon.mjs";
 import { loadAddon, packForDevice } from "./renderer.mjs";
 
 export class RenderServer {
@@ -13,10 +14,22 @@ export class RenderServer {
     #sizes = new Map();    // job id → { width, height, samples, denoise }
     #closed = false;
 
-    /** Opens a server on `device` with `slots` concurrent jobs within `memoryBudget` bytes. */
-    static async create({ device = 0, slots = 8, memoryBudget = 200 * 2 ** 30 } = {}) {
+    /**
+     * Opens a server on `device` with `slots` concurrent jobs within
+     * `memoryBudget` bytes; `precision` ({ quantizedBvh, fp16Guides,
+     * compensated }) applies to every job.
+     */
+    static async create({ device = 0, slots = 8, memoryBudget = 200 * 2 ** 30, precision = {} } = {}) {
         const addon = loadAddon();
-        return new RenderServer(addon, addon.serverOpen(device, slots, memoryBudget));
+        const server = addon.serverOpen(device, slots, memoryBudget);
+        try {
+            addon.serverSetOption(server, OPTION_PRECISION, precisionFlags(precision));
+        }
+        catch (error) {
+            addon.serverClose(server);
+            throw error;
+        }
+        return new RenderServer(addon, server);
     }
 
     constructor(addon, server) {

@@ -46,7 +46,7 @@ def test_addon_exports():
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().split(",") == sorted(
         ["open", "close", "uploadScene", "releaseScene", "uploadTextures", "uploadEnvironment", "resize", "frame",
-         "read", "synchronize", "display", "raycast", "deviceInfo", "bvhBuild", "serverOpen", "serverContext", "serverSubmit", "serverWait",
+         "read", "synchronize", "display", "raycast", "deviceInfo", "bvhBuild", "setOption", "serverSetOption", "serverOpen", "serverContext", "serverSubmit", "serverWait",
          "serverClose"])
 
 

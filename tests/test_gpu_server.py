@@ -69,3 +69,34 @@ def test_budget_serialises_jobs(pack):
     # Admitted one after another: each later job waited in the queue.
     queued = [s["queued_ms"] for _, s in results]
     assert queued == sorted(queued)
+
+
+def test_precision_options_reach_the_jobs(ctx, pack):
+    """rm_server_set_option's precision flags render each job as a context
+    with the same flags renders it alone, bit for bit; jobs submitted before
+    the call keep the plain path."""
+    flags = capi.PRECISION_QUANTIZED_BVH | capi.PRECISION_FP16_GUIDES | capi.PRECISION_COMPENSATED
+    chosen = [j for j in JOBS if not j[2].get("caustics")]
+    packs = [pack(c, **o) for c, o, _, _ in chosen]
+    params = [p.with_params(firefly_clamp=0.0, **f) for p, (_, _, f, _) in zip(packs, chosen)]
+    frames = [n for *_, n in chosen]
+    plain = [alone(ctx, p, q, n) for p, q, n in zip(packs, params, frames)]
+    ctx.set_option(capi.OPTION_PRECISION, flags)
+    try:
+        varied = [alone(ctx, p, q, n) for p, q, n in zip(packs, params, frames)]
+    finally:
+        ctx.set_option(capi.OPTION_PRECISION, 0)
+    with capi.Server(slots=3, memory_budget=8 << 30) as server:
+        scenes = [server.context.upload_scene(p) for p in packs]
+        spp = [int(q.view(np.uint32)[43]) for q in params]
+        before = [server.submit(s, q, n * k, packed=p) for p, q, n, k, s in zip(packs, params, frames, spp, scenes)]
+        server.set_option(capi.OPTION_PRECISION, flags)
+        after = [server.submit(s, q, n * k, packed=p) for p, q, n, k, s in zip(packs, params, frames, spp, scenes)]
+        with pytest.raises(capi.RaymondError):
+            server.set_option(capi.OPTION_PRECISION, 64)
+        got_before = [server.wait(i)[0] for i in before]
+        got_after = [server.wait(i)[0] for i in after]
+    for name, a, b, c, d in zip([j[0] for j in chosen], got_before, plain, got_after, varied):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+        assert np.array_equal(c.view(np.uint32), d.view(np.uint32)), name
+        assert not np.array_equal(c, a), name
