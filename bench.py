@@ -154,13 +154,16 @@ def bench_frames(args, name: str) -> dict:
 
     ctx = capi.Context(0)
     device, cus = ctx.device_info()
-    ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULES[args.schedule])
-    ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, args.persistent_blocks)
+    tuned = args.schedule == "tuned"
+    if not tuned:
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULES[args.schedule])
+        ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, args.persistent_blocks)
     ctx.set_option(capi.OPTION_PRECISION, capi.PRECISION_QUANTIZED_BVH if args.quantized_bvh else 0)
     scene = ctx.upload_scene(p)
     ctx.environment(p.env_rgba, p.meta["envWidth"], p.meta["envHeight"], p.env_table, p.meta["envTableWidth"],
                     p.meta["envTableHeight"])
     ctx.resize(w, h)
+    tuning = ctx.tune(scene, frame_params(0), frames=2) if tuned else None
     emitters = p.emitters if caustics else None
 
     k = 0
@@ -209,11 +212,13 @@ def bench_frames(args, name: str) -> dict:
                    "photon_gather": bool(caustics), "denoise_passes": 5, "frames_timed": args.steps,
                    "primitives": p.primitive_count, "device": device, "compute_units": cus,
                    "path": "one job, megakernel, " + ("8-bit quantized BVH" if args.quantized_bvh else "FP32 BVH"),
-                   "schedule": args.schedule},
+                   "schedule": args.schedule,
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_summary(wl["traffic"]),
                      "kernel": {"compact": "k_main_compact", "per-pixel": "k_main",
-                                "persistent": "k_main_persistent"}[args.schedule],
+                                "persistent": "k_main_persistent", "wavefront": "the wavefront stages (k_wf_*)"}[
+                         tuning["best"][0] if tuned else args.schedule],
                      "launch_ms": round(trace_avg_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": round(bytes_per_launch),
                      "bytes_per_sample": round(alg["per_sample"], 1),
@@ -309,13 +314,14 @@ def precision_flags(names: str) -> int:
     return flags
 
 
-def run_trace(packs, trace, slots: int, budget: int, precision: int = 0) -> dict:
+def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotune: int = 0) -> dict:
     """Submits the trace's jobs at their arrival times to a server with `slots`
     slots and RM_OPTION_PRECISION flags `precision`; returns throughput and
     latency percentiles."""
     from raymond_amd import capi, scenefile
     with capi.Server(slots=slots, memory_budget=budget) as server:
         server.set_option(capi.OPTION_PRECISION, precision)
+        server.set_option(capi.OPTION_AUTOTUNE, autotune)
         scenes = [server.context.upload_scene(p) for p in packs]
         jobs = []
         for arrival, k, (w, h), spp, bounces in trace:
@@ -447,7 +453,8 @@ def main() -> int:
     ap.add_argument("--served-precision", default="quantized-bvh,fp16-guides,compensated",
                     help="serving: precision variants of the served run, comma-separated "
                          "(quantized-bvh, fp16-guides, compensated; empty for plain FP32)")
-    ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent"], default="per-pixel",
+    ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent", "wavefront", "tuned"],
+                    default="per-pixel",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
                          "or lanes regenerating paths from a tile queue")
     ap.add_argument("--quantized-bvh", action="store_true",

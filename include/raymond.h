@@ -52,7 +52,9 @@ extern "C" {
                                           every schedule gives the same results, bit for bit */
 #define RM_OPTION_PERSISTENT_BLOCKS 1  /* blocks of 256 threads for RM_SCHEDULE_PERSISTENT; 0 (default): fill the GPU */
 #define RM_OPTION_PRECISION 2          /* RM_PRECISION_* flags; 0 (default): the plain FP32 path */
-#define RM_OPTION_COUNT 3
+#define RM_OPTION_AUTOTUNE 3           /* job servers: frames per candidate with which rm_tune picks the schedule
+                                          for each new scene and image size; 0 (default): no tuning */
+#define RM_OPTION_COUNT 4
 
 /* Reduced- and extended-precision variants (RM_OPTION_PRECISION flags). */
 #define RM_PRECISION_QUANTIZED_BVH 1   /* traverse the 8-bit child-quantized tree (rm_bvh_quantize) in the
@@ -67,6 +69,8 @@ extern "C" {
 #define RM_SCHEDULE_PIXEL 0            /* default: one thread per pixel, each path run to its end */
 #define RM_SCHEDULE_COMPACT 1          /* a block's live paths are packed into its first waves between bounces */
 #define RM_SCHEDULE_PERSISTENT 2       /* persistent lanes that regenerate finished paths from a tile queue */
+#define RM_SCHEDULE_WAVEFRONT 3        /* one kernel per stage (extend, shade, shadow, finish) over queues of
+                                          live paths, the paths' state in device memory between stages */
 
 typedef struct rm_context rm_context;
 typedef struct rm_scene rm_scene;
@@ -129,6 +133,18 @@ int rm_bvh_build(const double* mins, const double* maxs, uint32_t count, uint32_
  * finite or children outside their parent. No device is needed. */
 int rm_bvh_quantize(const void* nodes, uint32_t node_count, void* qnodes);
 
+/* The 4-wide quantized form of a packed tree: every other level folded
+ * into its parent, so a wide node has two to four children, their boxes
+ * quantized against the node's box as rm_bvh_quantize does, leaf ranges
+ * inline; 64 bytes per wide node, written to `wide` (room for `capacity`;
+ * node_count always suffices), *wide_count receiving how many (0 when the
+ * root is a leaf). Fails, as rm_bvh_quantize does, for boxes that aren't
+ * finite, and for trees where a walk could push more than the 64-entry stack
+ * holds. No counterpart in the reference; rm_scene_upload builds it, and
+ * RM_PRECISION_QUANTIZED_BVH walks it when it exists. No device is needed. */
+int rm_bvh_quantize_wide(const void* nodes, uint32_t node_count, void* wide, uint32_t capacity,
+                         uint32_t* wide_count);
+
 /* Sizes the per-pixel buffers for width × height (reused while they fit).
  * Replaces Renderer.#resize (src/Renderer.mjs:766-799). */
 int rm_resize(rm_context* ctx, uint32_t width, uint32_t height);
@@ -173,6 +189,24 @@ int rm_reset_stage_times(rm_context* ctx);
 /* Sets an RM_OPTION_* for later frames. No counterpart in the reference:
  * these choose among equivalent kernel schedules (and let a tuner pick). */
 int rm_set_option(rm_context* ctx, int option, int64_t value);
+
+/* The autotuner. Times the path pass alone (denoiser and light tracing off)
+ * for `frames` frames after one warm-up under each candidate schedule — one
+ * thread per pixel, compacted, persistent at two grid sizes, wavefront — and
+ * sets the fastest as the context's RM_OPTION_SCHEDULE (and
+ * RM_OPTION_PERSISTENT_BLOCKS). The schedules give the same sums bit for bit,
+ * so tuning changes speed only. The frames render into the context's
+ * buffers: begin the image again (sample offset 0) afterwards. No
+ * counterpart in the reference. */
+#define RM_TUNE_CANDIDATES 5
+typedef struct rm_tune_result {
+    int32_t schedule[RM_TUNE_CANDIDATES];            /* each candidate's RM_SCHEDULE_* */
+    uint32_t persistent_blocks[RM_TUNE_CANDIDATES];  /* and its persistent grid (0: fill the GPU) */
+    double ms[RM_TUNE_CANDIDATES];                   /* mean ms of its path pass per frame */
+    uint32_t count;                                  /* candidates tried */
+    uint32_t best;                                   /* the one now set */
+} rm_tune_result;
+int rm_tune(rm_context* ctx, const rm_scene* scene, const void* params, uint32_t frames, rm_tune_result* out);
 
 /* The device's name and compute-unit count. */
 int rm_device_info(rm_context* ctx, char* name, size_t name_len, int* compute_units);

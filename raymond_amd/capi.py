@@ -35,11 +35,13 @@ SIGNATURES = {
     "rm_display": (C.c_int, [_P, _P, C.c_size_t]),
     "rm_bvh_build": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P]),
     "rm_bvh_quantize": (C.c_int, [_P, C.c_uint32, _P]),
+    "rm_bvh_quantize_wide": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P]),
     "rm_raycast": (C.c_int, [_P, _P, _P, _U32, C.c_int, _P, _P]),
     "rm_timing": (C.c_int, [_P, C.c_int]),
     "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
     "rm_reset_stage_times": (C.c_int, [_P]),
     "rm_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
+    "rm_tune": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
     "rm_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
     "rm_server_open": (C.c_int, [C.c_int, _U32, C.c_uint64, C.POINTER(_P)]),
     "rm_server_context": (_P, [_P]),
@@ -48,12 +50,21 @@ SIGNATURES = {
     "rm_server_close": (C.c_int, [_P]),
     "rm_server_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
-OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION = 0, 1, 2
+OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE = 0, 1, 2, 3
 PRECISION_QUANTIZED_BVH, PRECISION_FP16_GUIDES, PRECISION_COMPENSATED = 1, 2, 4
-SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT = 0, 1, 2
-SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT}
+SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT, SCHEDULE_WAVEFRONT = 0, 1, 2, 3
+SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT,
+             "wavefront": SCHEDULE_WAVEFRONT}
 
 _lib = None
+
+
+TUNE_CANDIDATES = 5
+
+
+class _TuneResult(C.Structure):
+    _fields_ = [("schedule", C.c_int32 * TUNE_CANDIDATES), ("persistent_blocks", C.c_uint32 * TUNE_CANDIDATES),
+                ("ms", C.c_double * TUNE_CANDIDATES), ("count", C.c_uint32), ("best", C.c_uint32)]
 
 
 class RaymondError(RuntimeError):
@@ -108,6 +119,16 @@ def bvh_quantize(nodes: np.ndarray) -> np.ndarray:
     out = np.empty_like(n)
     _check(load().rm_bvh_quantize(_ptr(n), n.shape[0], _ptr(out)), "rm_bvh_quantize")
     return out
+
+
+def bvh_quantize_wide(nodes: np.ndarray) -> np.ndarray:
+    """The 4-wide quantized records of a packed tree ((N, 8) u32 in, (W, 16) u32 out)."""
+    n = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 8)
+    out = np.zeros((n.shape[0], 16), np.uint32)
+    used = C.c_uint32(0)
+    _check(load().rm_bvh_quantize_wide(_ptr(n), n.shape[0], _ptr(out), n.shape[0], C.byref(used)),
+           "rm_bvh_quantize_wide")
+    return out[:used.value].copy()
 
 
 class Scene:
@@ -211,6 +232,17 @@ class Context:
         _check(load().rm_raycast(self.handle, scene.handle, _ptr(r), n, int(any_hit), _ptr(hits), _ptr(cnt)),
                "rm_raycast")
         return (hits, cnt) if counters else hits
+
+    def tune(self, scene: Scene, params: np.ndarray, frames: int = 2) -> dict:
+        """rm_tune: times each candidate schedule's path pass and keeps the
+        fastest; returns {"best": (schedule name, blocks), "candidates":
+        [(schedule name, blocks, ms), ...]}."""
+        res = _TuneResult()
+        _check(load().rm_tune(self.handle, scene.handle, _ptr(np.ascontiguousarray(params)), frames, C.byref(res)),
+               "rm_tune")
+        names = {v: k for k, v in SCHEDULES.items()}
+        cands = [(names[res.schedule[i]], res.persistent_blocks[i], res.ms[i]) for i in range(res.count)]
+        return {"best": cands[res.best][:2], "candidates": cands}
 
     def set_option(self, option: int, value: int) -> None:
         _check(load().rm_set_option(self.handle, option, value), "rm_set_option")

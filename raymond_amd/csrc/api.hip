@@ -85,6 +85,7 @@ rm::DevScene dev_scene(const rm_context* c, const rm_scene* s) {
     S.prims = s->prims.as<const float4>();
     S.nodes = s->nodes.as<const float4>();
     S.qnodes = s->qnodes.as<const float4>();
+    S.wnodes = s->wnodes.as<const float4>();
     S.mats = s->mats.as<const float4>();
     S.attr = s->attr.as<const float4>();
     S.lights = s->lights.as<const uint32_t>();
@@ -128,9 +129,13 @@ int rmi::current_sums(rm_context* c, hipStream_t stream, const float4* out[3]) {
 
 namespace {
 
-// Whether traversals walk the scene's quantized tree: asked for, and built.
-bool quantized(const rm_context* c, const rm_scene* s) {
-    return (c->options[RM_OPTION_PRECISION] & RM_PRECISION_QUANTIZED_BVH) && s->qnodes.ptr;
+// The tree traversals walk: the FP32 nodes, or when quantized nodes are
+// asked for, the 4-wide quantized tree, else the binary quantized one, else
+// (a tree that can't be quantized) the FP32 nodes.
+int tree_of(const rm_context* c, const rm_scene* s) {
+    if (!(c->options[RM_OPTION_PRECISION] & RM_PRECISION_QUANTIZED_BVH)) return rm::TREE_FP32;
+    if (s->wnodes.ptr) return rm::TREE_WIDE;
+    return s->qnodes.ptr ? rm::TREE_QUANTIZED : rm::TREE_FP32;
 }
 
 int upload(rm_context* c, DevBuf& b, const void* src, size_t bytes, const char* what) {
@@ -202,10 +207,13 @@ int rm_open(int device, rm_context** out) {
 int rm_set_option(rm_context* c, int option, int64_t value) {
     if (!c) return fail(RM_ERR_INVALID, "rm_set_option needs a context.");
     if (option < 0 || option >= RM_OPTION_COUNT) return fail(RM_ERR_INVALID, "Unknown option " + std::to_string(option) + ".");
-    if (option == RM_OPTION_SCHEDULE && (value < RM_SCHEDULE_PIXEL || value > RM_SCHEDULE_PERSISTENT))
-        return fail(RM_ERR_INVALID, "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT or _PERSISTENT.");
+    if (option == RM_OPTION_SCHEDULE && (value < RM_SCHEDULE_PIXEL || value > RM_SCHEDULE_WAVEFRONT))
+        return fail(RM_ERR_INVALID,
+                    "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT, _PERSISTENT or _WAVEFRONT.");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
+    if (option == RM_OPTION_AUTOTUNE && (value < 0 || value > 64))
+        return fail(RM_ERR_INVALID, "RM_OPTION_AUTOTUNE is 0 (off) to 64 frames per candidate.");
     if (option == RM_OPTION_PRECISION &&
         (value & ~(int64_t)(RM_PRECISION_QUANTIZED_BVH | RM_PRECISION_FP16_GUIDES | RM_PRECISION_COMPENSATED)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PRECISION takes RM_PRECISION_* flags.");
@@ -224,7 +232,7 @@ int rm_close(rm_context* c) {
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->albedo_frame, &c->heads,
-                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging, &c->display, &c->comp, &c->folded})
+                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging, &c->display, &c->comp, &c->folded, &c->wavefront})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
@@ -281,14 +289,18 @@ int rm_scene_upload(rm_context* c, const void* primitives, uint32_t primitive_sl
     if (r == RM_OK) r = upload(c, s->prims, primitives, (size_t)primitive_slots * 64, "uploading primitives");
     if (r == RM_OK) r = upload(c, s->nodes, nodes, (size_t)node_count * 32, "uploading BVH nodes");
     if (r == RM_OK) {
-        // The quantized tree, for RM_PRECISION_QUANTIZED_BVH; a tree it can't
-        // enclose (boxes that aren't finite) keeps only the FP32 form.
+        // The quantized trees (binary and 4-wide), for RM_PRECISION_QUANTIZED_BVH;
+        // a tree they can't enclose (boxes that aren't finite) keeps only the
+        // FP32 form, without an error.
         std::vector<uint32_t> q((size_t)node_count * 8);
         const std::string before = rm_last_error();
         if (rm_bvh_quantize(nodes, node_count, q.data()) == RM_OK)
             r = upload(c, s->qnodes, q.data(), q.size() * 4, "uploading the quantized BVH");
-        else
-            fail(RM_OK, before);
+        std::vector<uint32_t> w((size_t)node_count * 16);
+        uint32_t wide = 0;
+        if (r == RM_OK && rm_bvh_quantize_wide(nodes, node_count, w.data(), node_count, &wide) == RM_OK && wide > 0)
+            r = upload(c, s->wnodes, w.data(), (size_t)wide * 64, "uploading the wide quantized BVH");
+        if (r == RM_OK) fail(RM_OK, before);
         if (r == RM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(RM_ERR_DEVICE, "uploading a scene");
     }
     if (r == RM_OK) r = upload(c, s->mats, materials, (size_t)material_count * 96, "uploading materials");
@@ -407,7 +419,7 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
 
     rm::DevScene S = dev_scene(c, s);
     rm::Features f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
-                      s->dispersive, quantized(c, s)};
+                      s->dispersive, tree_of(c, s)};
     // Light tracing and the photon gather depend on the scene and the
     // parameters only, so they run on the side stream while the path pass
     // runs; k_resolve then adds splats and gathered light, after the path
@@ -481,10 +493,36 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         if (!c->comp_live && P.sample_offset != 0u) rm::launch_fill_u32(c->comp.as<uint32_t>(), 0u, (size_t)px * 12, c->stream);
     }
     c->comp_live = compensated;
-    {
+    if (schedule == RM_SCHEDULE_WAVEFRONT && P.view == 0u) {
+        // Every stage's grid covers all paths, a thread per queue entry.
+        const uint32_t blocks = (px + rm::BLOCK_THREADS - 1) / rm::BLOCK_THREADS;
+        const uint32_t threads = blocks * rm::BLOCK_THREADS;
+        int r = ensure_spill(c, threads);
+        if (r != RM_OK) return r;
+        RM_HIP(c->wavefront.ensure(rm::wavefront_bytes(px)), "allocating the wavefront path state");
+        rm::Wavefront W;
+        char* base = c->wavefront.as<char>();
+        auto carve = [&](size_t bytes) {
+            char* p = base;
+            base += (bytes + 255) & ~(size_t)255;
+            return p;
+        };
+        W.n = px;
+        W.state[0] = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_STATE_WORDS * 4));
+        W.state[1] = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_STATE_WORDS * 4));
+        W.hits = reinterpret_cast<uint4*>(carve((size_t)px * 16));
+        W.nee = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_NEE_WORDS * 4));
+        W.done = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_DONE_WORDS * 4));
+        W.sums = reinterpret_cast<float4*>(carve((size_t)px * 48));
+        W.counts = reinterpret_cast<uint32_t*>(carve(16));
+        Timed t(c, RM_STAGE_TRACE);
+        rm::LaunchWavefront L{S, P, W, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), comp,
+                              c->spill.as<uint32_t>(), threads, blocks, f.tree};
+        rm::launch_wavefront(L, f, c->stream);
+    } else {
         Timed t(c, RM_STAGE_TRACE);
         rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), comp,
-                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging, f.quantized};
+                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging, f.tree};
         rm::launch_main(L, f, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the trace");
@@ -601,7 +639,7 @@ int rm_raycast(rm_context* c, const rm_scene* s, const float* rays, uint32_t n, 
     {
         Timed t(c, RM_STAGE_RAYCAST);
         rm::launch_raycast(dev_scene(c, s), c->rays.as<const float4>(), n, c->hits.as<uint4>(), any_hit != 0,
-                           quantized(c, s), c->spill.as<uint32_t>(), n, counters ? c->counters.as<uint2>() : nullptr, c->stream);
+                           tree_of(c, s), c->spill.as<uint32_t>(), n, counters ? c->counters.as<uint2>() : nullptr, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the raycast");
     RM_HIP(hipMemcpyAsync(hits, c->hits.ptr, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream), "reading hits");
@@ -638,6 +676,53 @@ int rm_reset_stage_times(rm_context* c) {
         c->timer.ms[s] = 0.0;
         c->timer.launches[s] = 0;
     }
+    return RM_OK;
+}
+
+int rm_tune(rm_context* c, const rm_scene* s, const void* params, uint32_t frames, rm_tune_result* out) {
+    if (!c || !s || !params || frames == 0 || frames > 64)
+        return fail(RM_ERR_INVALID, "rm_tune needs a context, a scene, parameters and 1 to 64 frames.");
+    rm::Params P;
+    std::memcpy(&P, params, sizeof P);
+    P.denoise = 0u;
+    P.caustics = 0u;
+    const uint32_t spp = P.samples_per_frame > 0 ? P.samples_per_frame : 1u;
+    const uint32_t cus = (uint32_t)(c->compute_units > 0 ? c->compute_units : 1);
+    const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_PIXEL, RM_SCHEDULE_COMPACT, RM_SCHEDULE_PERSISTENT,
+                                                  RM_SCHEDULE_PERSISTENT, RM_SCHEDULE_WAVEFRONT};
+    const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 2u * cus, 0u};
+    rm_tune_result res = {};
+    const bool timing = c->timing;
+    c->timing = true;
+    double best_ms = 0.0;
+    int r = RM_OK;
+    for (uint32_t k = 0; k < RM_TUNE_CANDIDATES && r == RM_OK; ++k) {
+        c->options[RM_OPTION_SCHEDULE] = schedules[k];
+        c->options[RM_OPTION_PERSISTENT_BLOCKS] = blocks[k];
+        for (uint32_t f = 0; f <= frames && r == RM_OK; ++f) {
+            if (f == 1) r = rm_reset_stage_times(c);
+            P.sample_offset = f * spp;
+            P.total_samples = (f + 1) * spp;
+            if (r == RM_OK) r = rm_frame(c, s, &P, nullptr, 0);
+        }
+        double ms[RM_STAGE_COUNT];
+        uint32_t launches[RM_STAGE_COUNT];
+        if (r == RM_OK) r = rm_stage_times(c, ms, launches, RM_STAGE_COUNT);
+        if (r != RM_OK) break;
+        res.schedule[k] = schedules[k];
+        res.persistent_blocks[k] = blocks[k];
+        res.ms[k] = ms[RM_STAGE_TRACE] / frames;
+        res.count = k + 1;
+        if (k == 0 || res.ms[k] < best_ms) {
+            best_ms = res.ms[k];
+            res.best = k;
+        }
+    }
+    c->timing = timing;
+    if (r != RM_OK) return r;
+    c->options[RM_OPTION_SCHEDULE] = res.schedule[res.best];
+    c->options[RM_OPTION_PERSISTENT_BLOCKS] = res.persistent_blocks[res.best];
+    if (out) *out = res;
     return RM_OK;
 }
 

@@ -591,3 +591,102 @@ extern "C" int rm_bvh_quantize(const void* nodes, uint32_t node_count, void* qno
     }
     return RM_OK;
 }
+
+// ------------------------------------------------------ 4-wide quantized tree
+//
+// Every other level of the binary tree folded into its parent: a wide node
+// made from binary node p has p's children as its children, each inner one
+// replaced by its own two children — two to four in all. One 64-byte record
+// per wide node (node 0 from the root), the children's boxes quantized
+// against p's box exactly as rm_bvh_quantize quantizes them:
+//
+//   f32 origin[3]    p's min corner
+//   u32 meta         step exponents + 127 (bits 0-7, 8-15, 16-23),
+//                    bits 24-27: child k is a leaf, bits 28-30: children
+//   u8  box[24]      child k's lo xyz, hi xyz at bytes 6k .. 6k + 5
+//   u32 ref[4]       a leaf's first primitive, or an inner child's wide node
+//   u8  count[4]     a leaf child's primitive count (at most 255)
+//   u32 zero
+//
+// A walk pushes up to three children per wide node; the producer refuses a
+// tree where that could pass the 64-entry traversal stack on any path.
+extern "C" int rm_bvh_quantize_wide(const void* nodes, uint32_t node_count, void* wide, uint32_t capacity,
+                                    uint32_t* wide_count) {
+    if (!nodes || !wide || !wide_count || node_count == 0)
+        return rmi::fail(RM_ERR_INVALID, "rm_bvh_quantize_wide needs nodes and room for the wide nodes.");
+    const uint32_t* nu = static_cast<const uint32_t*>(nodes);
+    const float* nf = static_cast<const float*>(nodes);
+    uint32_t* wu = static_cast<uint32_t*>(wide);
+    float* wf = static_cast<float*>(wide);
+    *wide_count = 0;
+    if (nu[7] > 0) return RM_OK;   // the root is a leaf: nothing to fold
+    struct Item { uint32_t binary, wide, pushes; };
+    std::vector<Item> todo{{0u, 0u, 0u}};
+    uint32_t used = 1;
+    while (!todo.empty()) {
+        const Item it = todo.back();
+        todo.pop_back();
+        const uint32_t p = it.binary;
+        const size_t po = (size_t)p * 8;
+        const uint32_t c = nu[po + 3];
+        if ((uint64_t)c + 1 >= node_count)
+            return rmi::fail(RM_ERR_INVALID, "BVH node " + std::to_string(p) + " points outside the tree.");
+        uint32_t kids[4], n = 0;
+        for (uint32_t k = c; k < c + 2; ++k) {
+            if (nu[(size_t)k * 8 + 7] > 0) {
+                kids[n++] = k;
+            } else {
+                const uint32_t g = nu[(size_t)k * 8 + 3];
+                if ((uint64_t)g + 1 >= node_count)
+                    return rmi::fail(RM_ERR_INVALID, "BVH node " + std::to_string(k) + " points outside the tree.");
+                kids[n++] = g;
+                kids[n++] = g + 1;
+            }
+        }
+        const uint32_t pushes = it.pushes + (n - 1);
+        if (pushes > 64)
+            return rmi::fail(RM_ERR_INVALID, "The tree is too deep for the wide walk's 64-entry stack.");
+        uint32_t* r = wu + (size_t)it.wide * 16;
+        std::memset(r, 0, 64);
+        uint32_t meta = (n << 28);
+        uint8_t* box = reinterpret_cast<uint8_t*>(r + 4);
+        uint8_t* counts = reinterpret_cast<uint8_t*>(r + 14);
+        for (int a = 0; a < 3; ++a) {
+            const float lo = nf[po + a], hi = nf[po + 4 + a];
+            if (!std::isfinite(lo) || !std::isfinite(hi))
+                return rmi::fail(RM_ERR_INVALID, "BVH node " + std::to_string(p) + " has a box that isn't finite.");
+            const int e = step_exponent(lo, hi);
+            const float step = std::ldexp(1.0f, e);
+            meta |= (uint32_t)(e + 127) << (8 * a);
+            wf[(size_t)it.wide * 16 + a] = lo;
+            for (uint32_t k = 0; k < n; ++k) {
+                const size_t ko = (size_t)kids[k] * 8;
+                uint32_t qlo, qhi;
+                if (!(nf[ko + a] >= lo && nf[ko + 4 + a] <= hi) ||
+                    !quantize_axis(lo, step, nf[ko + a], nf[ko + 4 + a], qlo, qhi))
+                    return rmi::fail(RM_ERR_INVALID,
+                                     "BVH node " + std::to_string(kids[k]) + " lies outside its parent's box.");
+                box[6 * k + a] = (uint8_t)qlo;
+                box[6 * k + 3 + a] = (uint8_t)qhi;
+            }
+        }
+        for (uint32_t k = 0; k < n; ++k) {
+            const size_t ko = (size_t)kids[k] * 8;
+            if (nu[ko + 7] > 0) {
+                if (nu[ko + 7] > 255)
+                    return rmi::fail(RM_ERR_INVALID, "A leaf of more than 255 primitives can't be folded.");
+                meta |= 1u << (24 + k);
+                r[10 + k] = nu[ko + 3];
+                counts[k] = (uint8_t)nu[ko + 7];
+            } else {
+                if (used >= capacity) return rmi::fail(RM_ERR_INVALID, "rm_bvh_quantize_wide ran out of room.");
+                r[10 + k] = used;
+                todo.push_back({kids[k], used, pushes});
+                ++used;
+            }
+        }
+        r[3] = meta;
+    }
+    *wide_count = used;
+    return RM_OK;
+}

@@ -53,12 +53,14 @@ struct StageTimer {
 struct rm_scene {
     rmi::DevBuf prims, nodes, mats, attr, lights, textures, nmaps;
     rmi::DevBuf qnodes;   // the 8-bit child-quantized tree; empty when the tree can't be quantized
+    rmi::DevBuf wnodes;   // the 4-wide quantized tree; empty when the tree can't be folded
     uint32_t primitive_slots = 0, primitive_count = 0, node_count = 0, material_count = 0;
     uint32_t tex_w = 0, tex_h = 0, tex_layers = 0, nmap_w = 0, nmap_h = 0, nmap_layers = 0;
     bool dispersive = false;
     std::vector<uint32_t> lights_host;   // the lights table, for checking parameters against
     void release() {
-        for (rmi::DevBuf* b : {&prims, &nodes, &mats, &attr, &lights, &textures, &nmaps, &qnodes}) b->release();
+        for (rmi::DevBuf* b : {&prims, &nodes, &mats, &attr, &lights, &textures, &nmaps, &qnodes, &wnodes})
+            b->release();
     }
 };
 
@@ -85,6 +87,7 @@ struct rm_context {
     // rounding error a pixel for accum, galb and gnorm; comp_live says the
     // last frame kept it up to date; folded holds sum + error for readers.
     rmi::DevBuf comp, folded;
+    rmi::DevBuf wavefront;   // RM_SCHEDULE_WAVEFRONT's path state and queues
     bool comp_live = false;
     rm::Params last_params = {};   // the last frame's, for rm_display
     bool rendered = false;

@@ -122,6 +122,7 @@ struct DevScene {
     const float4* prims;       // 4 per primitive
     const float4* nodes;       // 2 per node
     const float4* qnodes;      // 2 per node: the 8-bit child-quantized tree, or null
+    const float4* wnodes;      // 4 per wide node: the 4-wide quantized tree, or null
     const float4* mats;        // 6 per material
     const float4* attr;        // vec4s
     const uint32_t* lights;    // lights, volumes, cdf bits
@@ -310,11 +311,15 @@ struct Hit { float t; uint32_t prim; float u, v; };
 // child test has already loaded, so a pop costs no global load. Capacity is
 // the reference's 64 entries; a full stack drops the far child exactly as the
 // reference does.
-// QUANT selects the tree trace() walks: the FP32 nodes, or the quantized
-// records (entries then hold a node's index and whether it is a leaf).
-template <int LDS_DEPTH, int BLOCK, bool QUANT = false>
+// The trees trace() can walk.
+constexpr int TREE_FP32 = 0;        // the packed FP32 nodes
+constexpr int TREE_QUANTIZED = 1;   // the 8-bit child-quantized binary records
+constexpr int TREE_WIDE = 2;        // the 4-wide 8-bit quantized records
+
+// TREE selects the tree trace() walks; an entry is what that walk pushes.
+template <int LDS_DEPTH, int BLOCK, int TREE = TREE_FP32>
 struct Stack {
-    static constexpr bool quantized = QUANT;
+    static constexpr int tree = TREE;
     uint2* lds;      // &shared[0][threadIdx.x]
     uint2* spill;    // this thread's column of an [entry][slot] spill array
     uint32_t stride; // slots in the spill array
@@ -481,10 +486,107 @@ RM_D Hit trace_q(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT
     return hit;
 }
 
+// Byte i (a constant once unrolled) of six packed words.
+RM_D uint32_t byte_of(const uint32_t (&w)[6], int i) { return w[i >> 2] >> (8 * (i & 3)) & 0xffu; }
+
+// trace() over the 4-wide quantized tree (rm_bvh_quantize_wide): each step
+// decodes two to four child boxes from one 64-byte record, visits the
+// nearest child entered and pushes the others, nearer ones on top. Leaf
+// ranges sit in the parent's record, so a leaf costs no load of its own.
+// Decoded boxes enclose the FP32 ones, so every leaf the FP32 walk reaches
+// is reached and the nearest t is the same; the order differs, which matters
+// only for hits at exactly equal t. The producer bounds the pushes on any
+// path by the 64-entry stack, so none is ever dropped.
+template <bool ANY, class STACK, class COUNT>
+RM_D Hit trace_w(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& cnt) {
+    Hit hit = {t_max, NONE, 0.0f, 0.0f};
+    V3 sd = safe(d);
+    V3 inv = {1.0f / sd.x, 1.0f / sd.y, 1.0f / sd.z};
+    cnt.nodes(1);
+    float4 r0 = S.nodes[0], r1 = S.nodes[1];
+    if (slabs(r0, r1, o, inv, t_max) >= INF) return hit;
+    // count > 0: a leaf's primitives [first, first + count); 0: wide node `first`.
+    uint32_t first = __float_as_uint(r1.w) > 0u ? __float_as_uint(r0.w) : 0u, count = __float_as_uint(r1.w);
+    uint32_t sp = 0;
+    for (;;) {
+        bool descend = false;
+        if (count > 0u) {
+            for (uint32_t i = first; i < first + count; i++) {
+                cnt.prims(1);
+                V3 h = intersect(S, i, o, d, hit.t);
+                if (h.x > 0.0f) {
+                    hit = {h.x, i, h.y, h.z};
+                    if (ANY) return hit;
+                }
+            }
+        } else {
+            const float4* W = S.wnodes + (size_t)first * 4;
+            const float4 a = W[0], b = W[1], c = W[2], e = W[3];
+            const uint32_t meta = __float_as_uint(a.w);
+            const uint32_t n = meta >> 28;
+            cnt.nodes(n);
+            const float sx = step_of(meta & 0xffu), sy = step_of(meta >> 8 & 0xffu), sz = step_of(meta >> 16 & 0xffu);
+            const uint32_t bw[6] = {__float_as_uint(b.x), __float_as_uint(b.y), __float_as_uint(b.z),
+                                    __float_as_uint(b.w), __float_as_uint(c.x), __float_as_uint(c.y)};
+            const uint32_t ref[4] = {__float_as_uint(c.z), __float_as_uint(c.w), __float_as_uint(e.x),
+                                     __float_as_uint(e.y)};
+            const uint32_t counts = __float_as_uint(e.z);
+            float t[4];
+            uint32_t id[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                id[k] = (uint32_t)k;
+                t[k] = INF;
+                if ((uint32_t)k < n) {
+                    const float4 lo = make_float4(dequant(a.x, byte_of(bw, 6 * k), sx),
+                                                  dequant(a.y, byte_of(bw, 6 * k + 1), sy),
+                                                  dequant(a.z, byte_of(bw, 6 * k + 2), sz), 0.0f);
+                    const float4 hi = make_float4(dequant(a.x, byte_of(bw, 6 * k + 3), sx),
+                                                  dequant(a.y, byte_of(bw, 6 * k + 4), sy),
+                                                  dequant(a.z, byte_of(bw, 6 * k + 5), sz), 0.0f);
+                    t[k] = slabs(lo, hi, o, inv, hit.t);
+                }
+            }
+            // Nearest first; equal distances in slot order.
+            auto order = [&](int i, int j) {
+                if (t[j] < t[i] || (t[j] == t[i] && id[j] < id[i])) {
+                    const float tt = t[i]; t[i] = t[j]; t[j] = tt;
+                    const uint32_t ii = id[i]; id[i] = id[j]; id[j] = ii;
+                }
+            };
+            order(0, 1); order(2, 3); order(0, 2); order(1, 3); order(1, 2);
+            if (t[0] < INF) {
+#pragma unroll
+                for (int k = 3; k >= 1; --k) {
+                    if (t[k] < INF && sp < STACK_LIMIT) {
+                        const uint32_t leaf = meta >> (24 + id[k]) & 1u;
+                        stack.put(sp, make_uint2(ref[id[k]], leaf ? counts >> (8 * id[k]) & 0xffu : 0u));
+                        sp++;
+                    }
+                }
+                const uint32_t leaf = meta >> (24 + id[0]) & 1u;
+                first = ref[id[0]];
+                count = leaf ? counts >> (8 * id[0]) & 0xffu : 0u;
+                descend = true;
+            }
+        }
+        if (!descend) {
+            if (sp == 0u) break;
+            sp--;
+            const uint2 en = stack.get(sp);
+            first = en.x;
+            count = en.y;
+        }
+    }
+    return hit;
+}
+
 // The walk the stack's type asks for.
 template <bool ANY, class STACK, class COUNT>
 RM_D Hit trace(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT& cnt) {
-    if constexpr (STACK::quantized)
+    if constexpr (STACK::tree == TREE_WIDE)
+        return trace_w<ANY>(S, o, d, t_max, stack, cnt);
+    else if constexpr (STACK::tree == TREE_QUANTIZED)
         return trace_q<ANY>(S, o, d, t_max, stack, cnt);
     else
         return trace_fp32<ANY>(S, o, d, t_max, stack, cnt);

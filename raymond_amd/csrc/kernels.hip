@@ -11,6 +11,8 @@
 // (src/shaders.mjs). Traversal stacks live in LDS with a global spill.
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace rm {
@@ -54,9 +56,9 @@ RM_D bool stencil_pixel(uint32_t width, uint32_t height, uint32_t& x, uint32_t& 
     return x < width && y < height;
 }
 
-template <int LDS_DEPTH, bool QUANT = false>
+template <int LDS_DEPTH, int TREE = TREE_FP32>
 struct BlockStack {
-    using S = Stack<LDS_DEPTH, BLOCK_THREADS, QUANT>;
+    using S = Stack<LDS_DEPTH, BLOCK_THREADS, TREE>;
     RM_D static S make(uint2* lds, uint32_t* spill_base, uint32_t slot, uint32_t slots) {
         S s;
         s.lds = lds + threadIdx.x;
@@ -69,14 +71,14 @@ struct BlockStack {
 // ------------------------------------------------------------------ raycast
 // trace() over a ray batch: rays are 8 floats (o, t_max, d, pad); hits are
 // (t, prim, u, v) as 4 words.
-template <bool ANY, bool QUANT>
+template <bool ANY, int TREE>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_raycast(DevScene S, const float4* __restrict__ rays, uint32_t n,
                                                            uint4* __restrict__ hits, uint32_t* spill, uint32_t slots,
                                                            uint2* counters) {
     __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     if (i >= n) return;
-    auto stack = BlockStack<TRACE_LDS_DEPTH, QUANT>::make(lds, spill, i, slots);
+    auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, spill, i, slots);
     float4 a = rays[(size_t)i * 2], b = rays[(size_t)i * 2 + 1];
     Hit h;
     if (counters) {
@@ -406,7 +408,7 @@ struct PixelSums {
 // 112 bytes a lane of rarely touched state (the pixel's sums, the guides) and
 // the extra wave hides enough traversal latency to make the pass about 11%
 // faster on the 1080p sphere scene.
-template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, bool QUANT>
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, int TREE>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main(
     DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
     float4* __restrict__ guide_normal, float4* __restrict__ comp, uint32_t* spill, uint32_t slots) {
@@ -416,7 +418,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
     if (x >= P.width || y >= P.height) return;
     const Ctx C{S, &P};
     const uint32_t index = y * P.width + x;
-    auto stack = BlockStack<MAIN_LDS_DEPTH, QUANT>::make(lds, spill, index, slots);
+    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, index, slots);
     PixelSums sums;
     sums.clear();
     for (uint32_t k = 0; k < P.samples_per_frame; k++) {
@@ -452,11 +454,12 @@ constexpr int COMPACT_LDS_DEPTH = 4;
 constexpr int SLOT_WORDS = 27;
 
 struct SlotStore {
-    uint32_t* w;  // [SLOT_WORDS][BLOCK_THREADS]
-    RM_D void put(uint32_t slot, int k, float v) { w[k * BLOCK_THREADS + slot] = __float_as_uint(v); }
-    RM_D void putu(uint32_t slot, int k, uint32_t v) { w[k * BLOCK_THREADS + slot] = v; }
-    RM_D float get(uint32_t slot, int k) const { return __uint_as_float(w[k * BLOCK_THREADS + slot]); }
-    RM_D uint32_t getu(uint32_t slot, int k) const { return w[k * BLOCK_THREADS + slot]; }
+    uint32_t* w;      // [SLOT_WORDS][stride]
+    uint32_t stride;  // slots: BLOCK_THREADS in LDS, the path count in global memory
+    RM_D void put(uint32_t slot, int k, float v) { w[(size_t)k * stride + slot] = __float_as_uint(v); }
+    RM_D void putu(uint32_t slot, int k, uint32_t v) { w[(size_t)k * stride + slot] = v; }
+    RM_D float get(uint32_t slot, int k) const { return __uint_as_float(w[(size_t)k * stride + slot]); }
+    RM_D uint32_t getu(uint32_t slot, int k) const { return w[(size_t)k * stride + slot]; }
     RM_D void put3(uint32_t slot, int k, V3 v) { put(slot, k, v.x); put(slot, k + 1, v.y); put(slot, k + 2, v.z); }
     RM_D V3 get3(uint32_t slot, int k) const { return {get(slot, k), get(slot, k + 1), get(slot, k + 2)}; }
 };
@@ -482,8 +485,18 @@ RM_D void slot_save(SlotStore& st, uint32_t slot, const PathState& ps, const Gui
     st.put(slot, 25, gd.normal.w);
     st.putu(slot, 26, pix);
 }
-RM_D void slot_load(const SlotStore& st, uint32_t slot, const Params& P, uint32_t sample, PathState& ps, Guides& gd,
-                    Rng& g, uint32_t& pix) {
+// The RNG and spectrum offset of pixel `index`'s sample, as path_begin seeds
+// them, with the dimension back at 0 for the next bounce.
+RM_D void reseed(Rng& g, PathState& ps, uint32_t index, uint32_t sample) {
+    g.state = pcg(index ^ pcg(sample));
+    g.qmc = true;
+    g.seed = pcg(index + 0x9e3779b9u);
+    g.index = sample;
+    g.dim = 0u;
+    ps.spectrum_u = (float)((pcg(index * 7919u + 1u) + sample * 2654435769u) >> 8u) * (1.0f / 16777216.0f);
+}
+// The fields slot_save wrote.
+RM_D void slot_fields(const SlotStore& st, uint32_t slot, PathState& ps, Guides& gd, uint32_t& pix) {
     ps.o = st.get3(slot, 0);
     ps.d = st.get3(slot, 3);
     ps.throughput = st.get3(slot, 6);
@@ -501,15 +514,19 @@ RM_D void slot_load(const SlotStore& st, uint32_t slot, const Params& P, uint32_
     gd.albedo = st.get3(slot, 19);
     gd.normal = make_float4(st.get(slot, 22), st.get(slot, 23), st.get(slot, 24), st.get(slot, 25));
     pix = st.getu(slot, 26);
+}
+RM_D void slot_load(const SlotStore& st, uint32_t slot, const Params& P, uint32_t sample, PathState& ps, Guides& gd,
+                    Rng& g, uint32_t& pix) {
+    slot_fields(st, slot, ps, gd, pix);
     uint32_t x, y;
     tile_pixel_of(pix, x, y);
-    const uint32_t index = y * P.width + x;
-    g.state = pcg(index ^ pcg(sample));
-    g.qmc = true;
-    g.seed = pcg(index + 0x9e3779b9u);
-    g.index = sample;
-    g.dim = 0u;
-    ps.spectrum_u = (float)((pcg(index * 7919u + 1u) + sample * 2654435769u) >> 8u) * (1.0f / 16777216.0f);
+    reseed(g, ps, y * P.width + x, sample);
+}
+// A wavefront slot: its word 26 is the pixel's index itself.
+RM_D void slot_load_wf(const SlotStore& st, uint32_t slot, uint32_t sample, PathState& ps, Guides& gd, Rng& g,
+                       uint32_t& pix) {
+    slot_fields(st, slot, ps, gd, pix);
+    reseed(g, ps, pix, sample);
 }
 
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
@@ -528,7 +545,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
     const bool valid = x < P.width && y < P.height;
     const uint32_t index = valid ? y * P.width + x : 0u;
     auto stack = BlockStack<COMPACT_LDS_DEPTH>::make(lds, spill, gid, slots);
-    SlotStore st{slot_words};
+    SlotStore st{slot_words, BLOCK_THREADS};
     float4* const stage = staging + (size_t)(gid - t) * 3;  // this block's 256 records of 3 float4
     PixelSums sums;
     sums.clear();
@@ -656,6 +673,211 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_main_persistent(DevScene S, P
     }
 }
 
+// ------------------------------------------------------------------ wavefront
+// RM_SCHEDULE_WAVEFRONT: a frame's camera paths advanced one bounce at a time
+// for every pixel, each stage a kernel of its own, a thread per queue entry:
+//
+//   begin   path_begin for every pixel's sample: the first live queue;
+//   extend  the closest hit of each live path (a lean traversal kernel);
+//   shade   path_shade at that hit; the path goes on to the next live queue
+//           or, ended, to the done queue, its shadow ray to the shadow queue;
+//   shadow  the any-hit test of each shadow ray, then apply_nee to its
+//           path's result wherever that path went;
+//   finish  PixelSums::add for the paths that ended;
+//   store   PixelSums::store once the frame's samples are all in.
+//
+// A path's state lives at its queue slot, double-buffered between bounces,
+// so every stage reads and writes its queue in order; slots are handed out
+// one atomic per wave, so a wave's writes are contiguous. The same functions
+// in the same order per path as k_main, each pixel's samples added in sample
+// order: the accumulation is the same, bit for bit.
+constexpr int NEE_WORDS = 14;    // o, d, vec, reach, phase, mis, pdf, medium
+constexpr int DONE_WORDS = 11;   // result, guide albedo, guide normal, pixel
+static_assert(WF_STATE_WORDS == SLOT_WORDS && WF_NEE_WORDS == NEE_WORDS + 1 && WF_DONE_WORDS == DONE_WORDS,
+              "the host sizes these buffers");
+
+// A slot for each lane with pred, from `count`, one atomic per wave; lanes
+// without pred get NONE.
+RM_D uint32_t wave_slot(uint32_t* count, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0ull) return NONE;
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : NONE;
+}
+
+RM_D SlotStore wf_state(const Wavefront& W, uint32_t which) { return SlotStore{W.state[which], W.n}; }
+
+RM_D PixelSums wf_sums(const Wavefront& W, uint32_t i) {
+    const float4 a = W.sums[i], b = W.sums[(size_t)W.n + i], c = W.sums[(size_t)2 * W.n + i];
+    PixelSums s;
+    s.c = {a.x, a.y, a.z};
+    s.w = a.w;
+    s.albedo = {b.x, b.y, b.z};
+    s.normal = c;
+    return s;
+}
+
+RM_D void wf_put_sums(const Wavefront& W, uint32_t i, const PixelSums& s) {
+    W.sums[i] = make_float4(s.c.x, s.c.y, s.c.z, s.w);
+    W.sums[(size_t)W.n + i] = make_float4(s.albedo.x, s.albedo.y, s.albedo.z, 0.0f);
+    W.sums[(size_t)2 * W.n + i] = s.normal;
+}
+
+// One thread per queue entry: every stage's grid covers all n paths, and the
+// threads past the queue's length (read on the device) return at once, so
+// finished waves give way to new ones as in a kernel launched per ray.
+#define WF_EACH(i, count) if (const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x; i < (count))
+
+// Every pixel's sample `sample`: its camera path, in pixel order in live
+// state 0; the first sample of the frame also clears the pixel's sums.
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_begin(Params P, Wavefront W, uint32_t sample, uint32_t first) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        W.counts[0] = W.n;
+        W.counts[1] = W.counts[2] = W.counts[3] = 0u;
+    }
+    WF_EACH(i, W.n) {
+        Rng g;
+        PathState ps;
+        Guides gd;
+        path_begin(P, g, ps, gd, i % P.width, i / P.width, i, sample);
+        SlotStore st = wf_state(W, 0);
+        slot_save(st, i, ps, gd, i);
+        if (first) {
+            PixelSums s;
+            s.clear();
+            wf_put_sums(W, i, s);
+        }
+    }
+}
+
+template <int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_extend(DevScene S, Wavefront W, uint32_t cur, uint32_t* spill,
+                                                             uint32_t slots) {
+    __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
+    // Nothing reads these until the shade kernel after this one.
+    if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[cur ^ 1u] = W.counts[2] = W.counts[3] = 0u;
+    WF_EACH(i, W.counts[cur]) {
+        auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, spill, i, slots);
+        const SlotStore st = wf_state(W, cur);
+        NoCount nc;
+        const Hit h = trace<false>(S, st.get3(i, 0), st.get3(i, 3), INF, stack, nc);
+        W.hits[i] = make_uint4(__float_as_uint(h.t), h.prim, __float_as_uint(h.u), __float_as_uint(h.v));
+    }
+}
+
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_shade(DevScene S, Params P, Wavefront W, uint32_t cur,
+                                                            uint32_t sample) {
+    const Ctx C{S, &P};
+    const uint32_t next = cur ^ 1u;
+    const uint32_t count = W.counts[cur];
+    // Whole waves reach the slot atomics together, so their ballots see every lane.
+    if (blockIdx.x * BLOCK_THREADS >= count) return;
+    const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    const bool live = i < count;
+    bool ended = false;
+    PendingNee pn;
+    pn.active = false;
+    Rng g;
+    PathState ps;
+    Guides gd;
+    uint32_t pix = 0;
+    if (live) {
+        const SlotStore st = wf_state(W, cur);
+        slot_load_wf(st, i, sample, ps, gd, g, pix);
+        const uint4 h = W.hits[i];
+        const Hit hit = {__uint_as_float(h.x), h.y, __uint_as_float(h.z), __uint_as_float(h.w)};
+        ended = path_shade<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, ps, gd, hit, pn);
+    }
+    const uint32_t go = wave_slot(W.counts + next, live && !ended);
+    const uint32_t stop = wave_slot(W.counts + 3, live && ended);
+    const uint32_t shadow = wave_slot(W.counts + 2, live && pn.active);
+    if (!live) return;
+    if (ended) {
+        SlotStore dn{W.done, W.n};
+        dn.put3(stop, 0, ps.result);
+        dn.put3(stop, 3, gd.albedo);
+        dn.put(stop, 6, gd.normal.x);
+        dn.put(stop, 7, gd.normal.y);
+        dn.put(stop, 8, gd.normal.z);
+        dn.put(stop, 9, gd.normal.w);
+        dn.putu(stop, 10, pix);
+    } else {
+        SlotStore st = wf_state(W, next);
+        slot_save(st, go, ps, gd, pix);
+    }
+    if (pn.active) {
+        SlotStore nz{W.nee, W.n};
+        nz.put3(shadow, 0, pn.o);
+        nz.put3(shadow, 3, pn.d);
+        nz.put3(shadow, 6, pn.vec);
+        nz.put(shadow, 9, pn.reach);
+        nz.put(shadow, 10, pn.phase);
+        nz.put(shadow, 11, pn.mis);
+        nz.put(shadow, 12, pn.pdf);
+        nz.putu(shadow, 13, pn.medium ? 1u : 0u);
+        // Where the path's result went: its done slot (top bit) or its next live slot.
+        nz.putu(shadow, 14, ended ? (stop | 0x80000000u) : go);
+    }
+}
+
+template <bool VOLUMES, int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_shadow(DevScene S, Params P, Wavefront W, uint32_t next,
+                                                             uint32_t* spill, uint32_t slots) {
+    __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
+    const Ctx C{S, &P};
+    WF_EACH(i, W.counts[2]) {
+        auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, spill, i, slots);
+        const SlotStore nz{W.nee, W.n};
+        PendingNee pn;
+        pn.o = nz.get3(i, 0);
+        pn.d = nz.get3(i, 3);
+        pn.vec = nz.get3(i, 6);
+        pn.reach = nz.get(i, 9);
+        pn.phase = nz.get(i, 10);
+        pn.mis = nz.get(i, 11);
+        pn.pdf = nz.get(i, 12);
+        pn.medium = nz.getu(i, 13) != 0u;
+        pn.active = true;
+        NoCount nc;
+        if (trace<true>(S, pn.o, pn.d, pn.reach, stack, nc).prim == NONE) {
+            const uint32_t to = nz.getu(i, 14);
+            SlotStore dst = (to & 0x80000000u) ? SlotStore{W.done, W.n} : wf_state(W, next);
+            const uint32_t slot = to & 0x7fffffffu;
+            const int word = (to & 0x80000000u) ? 0 : 9;   // the result's first word
+            PathState ps = {};
+            ps.result = dst.get3(slot, word);
+            apply_nee<VOLUMES>(C, ps, pn);
+            dst.put3(slot, word, ps.result);
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_finish(Params P, Wavefront W) {
+    WF_EACH(i, W.counts[3]) {
+        const SlotStore dn{W.done, W.n};
+        Guides gd;
+        gd.albedo = dn.get3(i, 3);
+        gd.normal = make_float4(dn.get(i, 6), dn.get(i, 7), dn.get(i, 8), dn.get(i, 9));
+        const uint32_t pix = dn.getu(i, 10);
+        PixelSums s = wf_sums(W, pix);
+        s.add(P, dn.get3(i, 0), gd, true);
+        wf_put_sums(W, pix, s);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_store(Params P, Wavefront W, float4* __restrict__ accum,
+                                                            float4* __restrict__ galb, float4* __restrict__ gnorm,
+                                                            float4* __restrict__ comp) {
+    WF_EACH(i, W.n) wf_sums(W, i).store(P, i, accum, galb, gnorm, comp);
+}
+
+#undef WF_EACH
+
 // ------------------------------------------------------------------ light tracing
 RM_D uint32_t photon_count(const Params& P) {
     return min(P.width * P.height * P.samples_per_frame, PHOTON_CAPACITY);
@@ -701,7 +923,7 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
 // the bounce loop with one counter atomic per wave (ballot + prefix count):
 // photon slots are claimed in a different order than one atomic per lane
 // would give, which the reference leaves unspecified too.
-template <bool TEXTURES, bool DISPERSION, bool QUANT>
+template <bool TEXTURES, bool DISPERSION, int TREE>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Params P, const float4* __restrict__ E,
                                                                uint32_t* splats, uint32_t* heads,
                                                                float4* __restrict__ photons, uint32_t* spill,
@@ -710,7 +932,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Param
     const Ctx C{S, &P};
     const uint32_t id = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     const uint32_t photons_n = photon_count(P);
-    auto stack = BlockStack<MAIN_LDS_DEPTH, QUANT>::make(lds, spill, id, slots);
+    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, id, slots);
     bool pending = false;
     V3 dep_p = {0, 0, 0}, dep_d = {0, 0, 0}, dep_beta = {0, 0, 0};
 
@@ -994,7 +1216,7 @@ RM_D float4 gather_pixel(const Ctx& C, const uint32_t* heads, const float4* phot
     return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-template <bool TEXTURES, bool QUANT>
+template <bool TEXTURES, int TREE>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Params P, const uint32_t* heads,
                                                                  const float4* __restrict__ photons,
                                                                  float4* __restrict__ gathered, uint32_t* spill,
@@ -1005,7 +1227,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Par
     if (x >= P.width || y >= P.height) return;
     const Ctx C{S, &P};
     const uint32_t index = y * P.width + x;
-    auto stack = BlockStack<GATHER_LDS_DEPTH, QUANT>::make(lds, spill, index, slots);
+    auto stack = BlockStack<GATHER_LDS_DEPTH, TREE>::make(lds, spill, index, slots);
     gathered[index] = gather_pixel<TEXTURES>(C, heads, photons, x, y, index, stack);
 }
 
@@ -1241,6 +1463,16 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t value, size_t n) {
 // ------------------------------------------------------------------ launchers
 namespace rm {
 
+// Calls f with the tree kind as a compile-time constant.
+template <class F>
+static void with_tree(int tree, F&& f) {
+    switch (tree) {
+    case TREE_WIDE: f(std::integral_constant<int, TREE_WIDE>{}); break;
+    case TREE_QUANTIZED: f(std::integral_constant<int, TREE_QUANTIZED>{}); break;
+    default: f(std::integral_constant<int, TREE_FP32>{}); break;
+    }
+}
+
 template <bool V, bool T, bool E, bool D>
 static void launch_main_v(const LaunchMain& L, hipStream_t stream) {
     if (L.persistent_blocks > 0 && L.P.view == VIEW_COLOR) {
@@ -1254,12 +1486,10 @@ static void launch_main_v(const LaunchMain& L, hipStream_t stream) {
                            L.galb, L.gnorm, L.comp, L.spill, L.slots, L.staging);
         return;
     }
-    if (L.quantized)
-        hipLaunchKernelGGL((k_main<V, T, E, D, true>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
-                           L.galb, L.gnorm, L.comp, L.spill, L.slots);
-    else
-        hipLaunchKernelGGL((k_main<V, T, E, D, false>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
-                           L.galb, L.gnorm, L.comp, L.spill, L.slots);
+    with_tree(L.tree, [&](auto tree) {
+        hipLaunchKernelGGL((k_main<V, T, E, D, decltype(tree)::value>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P,
+                           L.accum, L.galb, L.gnorm, L.comp, L.spill, L.slots);
+    });
 }
 
 template <bool V, bool T, bool E, bool D>
@@ -1301,31 +1531,65 @@ void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream) {
     }
 }
 
-void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, bool quantized,
+template <bool V, bool T, bool E, bool D>
+static void launch_wavefront_v(const LaunchWavefront& L, hipStream_t stream) {
+    const dim3 grid(L.blocks), block(BLOCK_THREADS);
+    for (uint32_t k = 0; k < L.P.samples_per_frame; ++k) {
+        const uint32_t sample = L.P.sample_offset + k;
+        hipLaunchKernelGGL(k_wf_begin, grid, block, 0, stream, L.P, L.W, sample, k == 0 ? 1u : 0u);
+        // Every path ends within max_bounces + 1 bounces.
+        for (uint32_t b = 0; b <= L.P.max_bounces; ++b) {
+            const uint32_t cur = b & 1u;
+            with_tree(L.tree, [&](auto t) {
+                hipLaunchKernelGGL((k_wf_extend<decltype(t)::value>), grid, block, 0, stream, L.S, L.W, cur, L.spill,
+                                   L.slots);
+            });
+            hipLaunchKernelGGL((k_wf_shade<V, T, E, D>), grid, block, 0, stream, L.S, L.P, L.W, cur, sample);
+            with_tree(L.tree, [&](auto t) {
+                hipLaunchKernelGGL((k_wf_shadow<V, decltype(t)::value>), grid, block, 0, stream, L.S, L.P, L.W,
+                                   cur ^ 1u, L.spill, L.slots);
+            });
+            hipLaunchKernelGGL(k_wf_finish, grid, block, 0, stream, L.P, L.W);
+        }
+    }
+    hipLaunchKernelGGL(k_wf_store, grid, block, 0, stream, L.P, L.W, L.accum, L.galb, L.gnorm, L.comp);
+}
+
+void launch_wavefront(const LaunchWavefront& L, const Features& f, hipStream_t stream) {
+    int key = (f.volumes ? 8 : 0) | (f.textures ? 4 : 0) | (f.environment ? 2 : 0) | (f.dispersion ? 1 : 0);
+    switch (key) {
+    case 0: launch_wavefront_v<false, false, false, false>(L, stream); break;
+    case 1: launch_wavefront_v<false, false, false, true>(L, stream); break;
+    case 2: launch_wavefront_v<false, false, true, false>(L, stream); break;
+    case 3: launch_wavefront_v<false, false, true, true>(L, stream); break;
+    case 8: launch_wavefront_v<true, false, false, false>(L, stream); break;
+    case 10: launch_wavefront_v<true, false, true, false>(L, stream); break;
+    default: launch_wavefront_v<true, true, true, true>(L, stream); break;
+    }
+}
+
+void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, int tree,
                     uint32_t* spill, uint32_t slots, uint2* counters, hipStream_t stream) {
     dim3 grid((n + BLOCK_THREADS - 1) / BLOCK_THREADS);
-    auto go = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill, slots, counters);
-    };
-    if (quantized) {
-        if (any_hit) go(k_raycast<true, true>);
-        else go(k_raycast<false, true>);
-    } else {
-        if (any_hit) go(k_raycast<true, false>);
-        else go(k_raycast<false, false>);
-    }
+    with_tree(tree, [&](auto t) {
+        constexpr int TREE = decltype(t)::value;
+        if (any_hit)
+            hipLaunchKernelGGL((k_raycast<true, TREE>), grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill,
+                               slots, counters);
+        else
+            hipLaunchKernelGGL((k_raycast<false, TREE>), grid, dim3(BLOCK_THREADS), 0, stream, S, rays, n, hits, spill,
+                               slots, counters);
+    });
 }
 
 template <bool T, bool D>
 static void launch_light_trace_v(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
                                  uint32_t* heads, float4* photons, uint32_t* spill, uint32_t slots, dim3 grid,
-                                 bool quantized, hipStream_t stream) {
-    if (quantized)
-        hipLaunchKernelGGL((k_light_trace<T, D, true>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
-                           heads, photons, spill, slots);
-    else
-        hipLaunchKernelGGL((k_light_trace<T, D, false>), grid, dim3(BLOCK_THREADS), 0, stream, S, P, emitters, splats,
-                           heads, photons, spill, slots);
+                                 int tree, hipStream_t stream) {
+    with_tree(tree, [&](auto t) {
+        hipLaunchKernelGGL((k_light_trace<T, D, decltype(t)::value>), grid, dim3(BLOCK_THREADS), 0, stream, S, P,
+                           emitters, splats, heads, photons, spill, slots);
+    });
 }
 
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
@@ -1333,11 +1597,11 @@ void launch_light_trace(const DevScene& S, const Params& P, const float4* emitte
                         hipStream_t stream) {
     dim3 grid((count + BLOCK_THREADS - 1) / BLOCK_THREADS);
     if (f.textures)
-        launch_light_trace_v<true, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.quantized, stream);
+        launch_light_trace_v<true, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.tree, stream);
     else if (f.dispersion)
-        launch_light_trace_v<false, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.quantized, stream);
+        launch_light_trace_v<false, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.tree, stream);
     else
-        launch_light_trace_v<false, false>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.quantized, stream);
+        launch_light_trace_v<false, false>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.tree, stream);
 }
 
 void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, float4* comp,
@@ -1360,13 +1624,11 @@ void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* he
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons, gathered, spill, slots);
     };
-    if (f.textures) {
-        if (f.quantized) go(k_photon_gather<true, true>);
-        else go(k_photon_gather<true, false>);
-    } else {
-        if (f.quantized) go(k_photon_gather<false, true>);
-        else go(k_photon_gather<false, false>);
-    }
+    with_tree(f.tree, [&](auto t) {
+        constexpr int TREE = decltype(t)::value;
+        if (f.textures) go(k_photon_gather<true, TREE>);
+        else go(k_photon_gather<false, TREE>);
+    });
 }
 
 void launch_guides(const Params& P, const float4* gnorm, void* guides, bool fp16, hipStream_t stream) {

@@ -19,10 +19,10 @@ constexpr int GATHER_LDS_DEPTH = 8;
 constexpr uint32_t SPILL_ENTRIES = STACK_LIMIT - 4u;
 
 // What a frame's scene and options need; each combination is its own kernel.
-// quantized: traverse the 8-bit child-quantized tree (the per-pixel path
-// pass, raycast, light tracing and gather; the compact and persistent path
-// passes walk the FP32 tree).
-struct Features { bool volumes, textures, environment, dispersion, quantized; };
+// tree: the tree the traversals walk (TREE_FP32, TREE_QUANTIZED, TREE_WIDE) in
+// the per-pixel path pass, raycast, light tracing and gather; the compact and
+// persistent path passes walk the FP32 tree.
+struct Features { bool volumes, textures, environment, dispersion; int tree; };
 
 struct LaunchMain {
     DevScene S;
@@ -36,13 +36,50 @@ struct LaunchMain {
     uint32_t persistent_blocks;  // > 0: k_main_persistent with this grid
     uint32_t* counter;           // k_main_persistent's tile counter, zeroed before the launch
     float4* staging;             // non-null: k_main_compact, 3 float4 per thread of its grid
-    bool quantized;              // the per-pixel pass walks the quantized tree
+    int tree;                    // the tree the per-pixel pass walks
 };
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
+
+// The wavefront path pass's buffers (RM_SCHEDULE_WAVEFRONT), for n paths (the
+// frame's pixels): path state, closest hits, pending shadow rays, each pixel's
+// sums, the live queues of this bounce and the next, the shadow and done
+// queues, and their four counters.
+constexpr int WF_STATE_WORDS = 27;
+constexpr int WF_NEE_WORDS = 15;
+constexpr int WF_DONE_WORDS = 11;
+struct Wavefront {
+    uint32_t* state[2];   // [WF_STATE_WORDS][n] each: the live paths of this bounce and the next, in queue order
+    uint4* hits;          // [n] the live paths' closest hits, in queue order
+    uint32_t* nee;        // [WF_NEE_WORDS][n] shadow rays and where their path's result went
+    uint32_t* done;       // [WF_DONE_WORDS][n] the paths that ended this bounce
+    float4* sums;         // [3][n] each pixel's sums
+    uint32_t* counts;     // [4]: live now, live next, shadow rays, done
+    uint32_t n;
+};
+// Bytes of device memory a Wavefront of n paths needs, each of its seven
+// buffers starting on a 256-byte boundary.
+inline size_t wavefront_bytes(size_t n) {
+    return n * ((2 * WF_STATE_WORDS + WF_NEE_WORDS + WF_DONE_WORDS) * 4 + 16 + 48) + 7 * 256;
+}
+
+struct LaunchWavefront {
+    DevScene S;
+    Params P;
+    Wavefront W;
+    float4* accum;
+    float4* galb;
+    float4* gnorm;
+    float4* comp;       // non-null: compensated sums
+    uint32_t* spill;    // traversal spill for blocks × BLOCK_THREADS slots
+    uint32_t slots;
+    uint32_t blocks;    // the grid of every stage: a thread per path
+    int tree;
+};
+void launch_wavefront(const LaunchWavefront& L, const Features& f, hipStream_t stream);
 // How many 256-thread blocks of the persistent path kernel fit on one CU.
 int main_resident_blocks_per_cu(const Features& f);
-void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, bool quantized,
+void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, int tree,
                     uint32_t* spill, uint32_t slots, uint2* counters, hipStream_t stream);
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
                         float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,

@@ -12,8 +12,10 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 
 #include "context.hpp"
@@ -81,6 +83,9 @@ struct rm_server {
     int device = 0;
     uint64_t budget = 0, used = 0;
     int64_t options[RM_OPTION_COUNT] = {};   // given to jobs submitted from now on
+    // The schedule rm_tune picked for each (scene, width, height, samples per
+    // frame), with RM_OPTION_AUTOTUNE; used by the scheduler thread only.
+    std::map<std::tuple<const rm_scene*, uint32_t, uint32_t, uint32_t>, std::pair<int32_t, uint32_t>> tuned;
     rm_context* home = nullptr;
     std::vector<Slot> slots;
     std::mutex m;
@@ -170,6 +175,21 @@ bool rm_server::admit() {
                 const float table[1] = {0.0f};
                 r = rm_environment(s.ctx, black, 1, 1, table, 1, 0);
             }
+        }
+        if (r == RM_OK && j->options[RM_OPTION_AUTOTUNE] > 0) {
+            // Every schedule gives the same image, so the choice changes speed only.
+            const auto key = std::make_tuple(j->scene, j->width, j->height, j->spp);
+            auto it = tuned.find(key);
+            if (it == tuned.end()) {
+                uint32_t p[RM_PARAMS_BYTES / 4];
+                frame_params(*j, 0, p);
+                rm_tune_result res;
+                r = rm_tune(s.ctx, j->scene, p, (uint32_t)j->options[RM_OPTION_AUTOTUNE], &res);
+                if (r == RM_OK)
+                    it = tuned.emplace(key, std::make_pair(res.schedule[res.best], res.persistent_blocks[res.best])).first;
+            }
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_SCHEDULE, it->second.first);
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_PERSISTENT_BLOCKS, it->second.second);
         }
         if (r != RM_OK) finish(s, r, rm_last_error());
     }

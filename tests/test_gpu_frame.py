@@ -87,16 +87,17 @@ def test_firefly_clamp(ctx, pack):
                                               (CASES[5][0], CASES[5][1], 2)],
                          ids=["spheres", "spheres-3spp", "triangles-fog-env", "dispersive", "textured"])
 def test_schedules_are_bit_identical(ctx, pack, config, opts, spp):
-    """The compacting and lane-regenerating schedules compute the same samples,
-    each pixel's in the same order, as one thread per pixel: accumulation and
-    guides equal bit for bit."""
+    """The compacting, lane-regenerating and wavefront schedules compute the
+    same samples, each pixel's in the same order, as one thread per pixel:
+    accumulation and guides equal bit for bit."""
     from raymond_amd import capi
     p = pack(config, **opts)
     params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0, samples_per_frame=spp)
     try:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         plain = gpu_render(ctx, p, 3, params)
-        runs = [(capi.SCHEDULE_COMPACT, 0), (capi.SCHEDULE_PERSISTENT, 0), (capi.SCHEDULE_PERSISTENT, 7)]
+        runs = [(capi.SCHEDULE_COMPACT, 0), (capi.SCHEDULE_PERSISTENT, 0), (capi.SCHEDULE_PERSISTENT, 7),
+                (capi.SCHEDULE_WAVEFRONT, 0)]
         for schedule, blocks in runs:
             ctx.set_option(capi.OPTION_SCHEDULE, schedule)
             ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, blocks)

@@ -100,3 +100,47 @@ def test_precision_options_reach_the_jobs(ctx, pack):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
         assert np.array_equal(c.view(np.uint32), d.view(np.uint32)), name
         assert not np.array_equal(c, a), name
+
+
+def test_autotuned_jobs_match_their_frames_rendered_alone(ctx, pack):
+    """With RM_OPTION_AUTOTUNE the server times the schedules on a job's scene
+    first; every schedule gives the same sums, so images stay bit-identical."""
+    chosen = [j for j in JOBS if not j[2].get("caustics")]
+    packs = [pack(c, **o) for c, o, _, _ in chosen]
+    params = [p.with_params(firefly_clamp=0.0, **f) for p, (_, _, f, _) in zip(packs, chosen)]
+    frames = [n for *_, n in chosen]
+    expected = [alone(ctx, p, q, n) for p, q, n in zip(packs, params, frames)]
+    with capi.Server(slots=2, memory_budget=8 << 30) as server:
+        server.set_option(capi.OPTION_AUTOTUNE, 1)
+        scenes = [server.context.upload_scene(p) for p in packs]
+        spp = [int(q.view(np.uint32)[43]) for q in params]
+        ids = [server.submit(s, q, n * k, packed=p) for p, q, n, k, s in zip(packs, params, frames, spp, scenes)]
+        ids += [server.submit(s, q, n * k, packed=p) for p, q, n, k, s in zip(packs, params, frames, spp, scenes)]
+        got = [server.wait(i)[0] for i in ids]
+    for k, image in enumerate(got):
+        j = k % len(chosen)
+        assert np.array_equal(image.view(np.uint32), expected[j].view(np.uint32)), chosen[j][0]
+
+
+def test_tune_picks_a_schedule_and_keeps_the_image(ctx, pack):
+    p = pack("spheres", scene={"width": 96, "height": 54}, width=96, height=54)
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0)
+    plain = alone(ctx, p, params, 3)
+    scene = ctx.upload_scene(p)
+    try:
+        ctx.upload_environment(p)
+        ctx.resize(p.width, p.height)
+        result = ctx.tune(scene, params, frames=2)
+    finally:
+        scene.release()
+    names = [c[0] for c in result["candidates"]]
+    assert names == ["per-pixel", "compact", "persistent", "persistent", "wavefront"]
+    assert all(ms > 0 for *_, ms in result["candidates"])
+    best = min(result["candidates"], key=lambda c: c[2])
+    assert result["best"] == best[:2]
+    try:
+        tuned = alone(ctx, p, params, 3)
+    finally:
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+        ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
+    assert np.array_equal(tuned.view(np.uint32), plain.view(np.uint32))

@@ -77,3 +77,54 @@ def test_refuses_what_it_cannot_enclose():
     f[0, 4] = 1.0
     with pytest.raises(capi.RaymondError, match="outside the tree"):
         capi.bvh_quantize(nodes)
+
+
+def decode_wide(w):
+    """Wide records: child boxes (W, 4, 2, 3) float32, meta, refs, counts."""
+    f = w.view(np.float32)
+    origin = f[:, :3]
+    meta = w[:, 3]
+    step = np.stack([((meta >> (8 * a)) & 0xFF).astype(np.uint32) << 23 for a in range(3)], 1).view(np.float32)
+    codes = w[:, 4:10].copy().view(np.uint8).reshape(-1, 4, 2, 3).astype(np.float32)
+    boxes = origin[:, None, None, :] + codes * step[:, None, None, :]
+    counts = w[:, 14].copy().view(np.uint8).reshape(-1, 4)
+    return boxes, meta, w[:, 10:14], counts
+
+
+@pytest.mark.parametrize("config,opts", SCENES, ids=[s[0] for s in SCENES])
+def test_wide_tree_covers_every_leaf_and_encloses_its_boxes(pack, config, opts):
+    """Walking the wide records from the root reaches every leaf of the binary
+    tree exactly once, each decoded box enclosing the FP32 box of the binary
+    node it stands for."""
+    nodes = pack(config, **opts).nodes.reshape(-1, 8)
+    f = nodes.view(np.float32)
+    w = capi.bvh_quantize_wide(nodes)
+    if nodes[0, 7] > 0:
+        assert w.shape[0] == 0
+        return
+    boxes, meta, refs, counts = decode_wide(w)
+    # The binary nodes each wide child stands for, by the fold's rule.
+    def kids(p):
+        out = []
+        for k in (nodes[p, 3], nodes[p, 3] + 1):
+            out += [k] if nodes[k, 7] > 0 else [nodes[k, 3], nodes[k, 3] + 1]
+        return out
+    seen_leaves, todo, visited = [], [(0, 0)], 0
+    while todo:
+        wi, p = todo.pop()
+        visited += 1
+        ks = kids(p)
+        assert meta[wi] >> 28 == len(ks)
+        for slot, k in enumerate(ks):
+            lo, hi = f[k, :3], f[k, 4:7]
+            assert (boxes[wi, slot, 0] <= lo).all() and (boxes[wi, slot, 1] >= hi).all()
+            leaf = bool(meta[wi] >> (24 + slot) & 1)
+            assert leaf == (nodes[k, 7] > 0)
+            if leaf:
+                assert refs[wi, slot] == nodes[k, 3] and counts[wi, slot] == nodes[k, 7]
+                seen_leaves.append(k)
+            else:
+                todo.append((refs[wi, slot], k))
+    assert visited == w.shape[0]
+    leaves = np.nonzero(nodes[:, 7] > 0)[0]
+    assert sorted(seen_leaves) == sorted(leaves.tolist())
