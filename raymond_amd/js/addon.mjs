@@ -33,3 +33,7 @@ export const OPTION_PRECISION = 2;
 export function precisionFlags({ quantizedBvh = false, fp16Guides = false, compensated = false } = {}) {
     return (quantizedBvh ? 1 : 0) | (fp16Guides ? 2 : 0) | (compensated ? 4 : 0);
 }
+
+/** RM_OPTION_SCHEDULE and its values. */
+export const OPTION_SCHEDULE = 0;
+export const SCHEDULES = Object.freeze({ "per-pixel": 0, compact: 1, persistent: 2, wavefront: 3 });

@@ -4,7 +4,7 @@
 // the frame itself (trace, light tracing, photon gather, denoise) runs as HIP
 // kernels in libraymond_hip.so, reached through the N-API addon raymond.node.
 // There is no other backend: without the addon or a GPU, create() rejects.
-import { OPTION_PRECISION, loadAddon, precisionFlags } from "./addon.mjs";
+import { OPTION_PRECISION, OPTION_SCHEDULE, SCHEDULES, loadAddon, precisionFlags } from "./addon.mjs";
 import {
     IMAGE_KEY_BYTES, PARAMS_BYTES, VIEWS, deltaObjects, emitters, environmentData,
     sceneTables, textureLayers, writeFrameParams, writeParams,
@@ -48,7 +48,7 @@ export class Renderer {
     #addon; #ctx; #canvas;
     #scene = null; #camera = null;
     #maxBounces = 8; #samplesPerFrame = 1; #exposure = 1.0; #fireflyClamp = 20.0;
-    #view = "color"; #denoise = true; #caustics = true; #precision = {};
+    #view = "color"; #denoise = true; #caustics = true; #precision = {}; #schedule = "per-pixel";
     #causticsActive = false;
     #samples = 0;
     #params = new ArrayBuffer(PARAMS_BYTES);
@@ -110,6 +110,17 @@ export class Renderer {
      * Precision variants for the frames from now on, none by default:
      * { quantizedBvh, fp16Guides, compensated } (RM_OPTION_PRECISION).
      */
+    /**
+     * How the path-tracing pass maps paths to lanes: "per-pixel" (default),
+     * "compact", "persistent" or "wavefront" — the same image, bit for bit.
+     */
+    get schedule() { return this.#schedule; }
+    set schedule(v) {
+        if (!Object.prototype.hasOwnProperty.call(SCHEDULES, v))
+            throw new RangeError(`schedule is one of ${Object.keys(SCHEDULES).join(", ")}.`);
+        this.#addon.setOption(this.#ctx, OPTION_SCHEDULE, SCHEDULES[v]);
+        this.#schedule = v;
+    }
     get precision() { return Object.assign({}, this.#precision); }
     set precision(v) {
         this.#addon.setOption(this.#ctx, OPTION_PRECISION, precisionFlags(v));

@@ -31,6 +31,17 @@ async function main() {
     writeFileSync(outDen, Buffer.from(den.buffer));
     writeFileSync(outPng, encodePng(64, 64, shown));
     console.log(JSON.stringify({ samples: renderer.samples }));
+    // The other schedules and the precision variants run through the same calls.
+    let rejected = false;
+    try { renderer.schedule = "diagonal"; } catch (e) { rejected = e instanceof RangeError; }
+    if (!rejected) throw new Error("an unknown schedule should be a RangeError");
+    renderer.view = "color";
+    renderer.schedule = "wavefront";
+    renderer.precision = { quantizedBvh: true, fp16Guides: true, compensated: true };
+    renderer.render();
+    renderer.render();
+    const varied = await renderer.readPixels({ denoised: true });
+    if (!varied.every(Number.isFinite)) throw new Error("the varied frames should be finite");
     renderer.dispose();
 }
 

@@ -91,3 +91,16 @@ def test_bvh_build_on_the_host():
     b = (ctypes.c_double * 6)()
     assert lib.rm_bvh_build(b, b, 2, 4, 0, buf, 1, ctypes.byref(used), (ctypes.c_uint32 * 2)()) == -1
     assert b"2" in lib.rm_last_error()
+
+
+def test_new_entry_points_check_their_arguments():
+    """The autotuner, server options and wide quantizer refuse bad arguments
+    without touching a device."""
+    from raymond_amd import capi
+    lib = capi.load()
+    assert lib.rm_tune(None, None, None, 2, None) == -1
+    assert b"rm_tune" in lib.rm_last_error()
+    assert lib.rm_server_set_option(None, 0, 0) == -1
+    used = ctypes.c_uint32()
+    assert lib.rm_bvh_quantize_wide(None, 0, None, 0, ctypes.byref(used)) == -1
+    assert lib.rm_set_option(None, capi.OPTION_AUTOTUNE, 1) == -1

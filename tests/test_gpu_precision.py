@@ -106,12 +106,12 @@ def test_compensated_cornell_against_fp64_sums(ctx, pack):
 
 
 def test_compensated_schedules_agree_and_start_mid_run(ctx, pack):
-    """The three path-pass schedules keep the same compensated sums; turning
+    """The four path-pass schedules keep the same compensated sums; turning
     compensation on part-way starts its error terms from zero."""
     p = pack("mixed", width=64, height=64)
     params = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
     outs = []
-    for schedule in (capi.SCHEDULE_PIXEL, capi.SCHEDULE_COMPACT, capi.SCHEDULE_PERSISTENT):
+    for schedule in (capi.SCHEDULE_PIXEL, capi.SCHEDULE_COMPACT, capi.SCHEDULE_PERSISTENT, capi.SCHEDULE_WAVEFRONT):
         ctx.set_option(capi.OPTION_SCHEDULE, schedule)
         try:
             with precision(ctx, capi.PRECISION_COMPENSATED):

@@ -97,9 +97,17 @@ def test_frames_match_the_fp32_tree(ctx, pack, config, opts):
     plain = gpu_render(ctx, p, 2, params)
     with quantized(ctx):
         q = gpu_render(ctx, p, 2, params)
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_WAVEFRONT)
+        try:
+            qw = gpu_render(ctx, p, 2, params)
+        finally:
+            ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
     for a, b, what in zip(q, plain, ("accum", "guide albedo", "guide normal")):
         same = (a == b).all(axis=-1).mean()
         assert same >= 0.995, f"{what}: {same:.5f} of pixels bit-identical"
+    # The wavefront stages walk the same quantized tree in the same order per ray.
+    for a, b in zip(qw, q):
+        np.testing.assert_array_equal(a, b)
 
 
 def test_caustic_frames_agree_with_the_fp32_tree(ctx, pack):
