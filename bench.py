@@ -314,7 +314,7 @@ def precision_flags(names: str) -> int:
     return flags
 
 
-def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotune: int = 0) -> dict:
+def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotune: int = 0, batch: bool = False) -> dict:
     """Submits the trace's jobs at their arrival times to a server with `slots`
     slots and RM_OPTION_PRECISION flags `precision`; returns throughput and
     latency percentiles."""
@@ -322,6 +322,7 @@ def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotun
     with capi.Server(slots=slots, memory_budget=budget) as server:
         server.set_option(capi.OPTION_PRECISION, precision)
         server.set_option(capi.OPTION_AUTOTUNE, autotune)
+        server.set_option(capi.OPTION_BATCH, int(batch))
         scenes = [server.context.upload_scene(p) for p in packs]
         jobs = []
         for arrival, k, (w, h), spp, bounces in trace:
@@ -342,7 +343,7 @@ def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotun
     first = np.array([s["first_frame_ms"] for s in stats])
     done = np.array([s["done_ms"] for s in stats])
     return {"Msamples_per_s": round(samples / elapsed / 1e6, 3), "wall_s": round(elapsed, 3), "jobs": len(jobs),
-            "samples": samples, "slots": slots, "precision": precision,
+            "samples": samples, "slots": slots, "precision": precision, "batch": batch,
             "first_frame_ms_p50": round(float(np.percentile(first, 50)), 2),
             "first_frame_ms_p99": round(float(np.percentile(first, 99)), 2),
             "done_ms_p50": round(float(np.percentile(done, 50)), 2),
@@ -400,8 +401,9 @@ def bench_serving(args) -> dict:
     trace = serving_trace(args.rate, args.duration, args.spp_min, args.spp_max)
     budget = 200 << 30
     flags = precision_flags(args.served_precision)
-    served = run_trace(packs, trace, args.slots, budget, flags)
-    served_fp32 = run_trace(packs, trace, args.slots, budget) if flags else served
+    served = run_trace(packs, trace, args.slots, budget)
+    served_batched = run_trace(packs, trace, args.slots, budget, batch=True)
+    served_described = run_trace(packs, trace, args.slots, budget, flags, batch=True)
     one_slot = run_trace(packs, trace, 1, budget)
     plain = run_plain(packs, trace)
     with capi.Context(0) as ctx:
@@ -426,10 +428,12 @@ def bench_serving(args) -> dict:
                    "scene_primitives": [p.primitive_count for p in packs], "slots": args.slots,
                    "memory_budget_bytes": budget, "device": device, "compute_units": cus,
                    "served_precision": args.served_precision,
-                   "path": "served: concurrent slots, a job's first and last frames denoised, with the precision "
-                           "variants named; plain: one context, one job at a time, FP32, every frame denoised"},
+                   "path": "served: concurrent slots, FP32, a job's first and last frames denoised (also reported: "
+                           "the running jobs' path passes batched into one launch, and batched with the precision "
+                           "variants named); plain: one context, one job at a time, FP32, every frame denoised"},
         "served": served,
-        "served_fp32": served_fp32,
+        "served_batched": served_batched,
+        "served_batched_with_precision_variants": served_described,
         "served_one_slot": one_slot,
         "plain": plain,
         "speedup_vs_plain": round(served["Msamples_per_s"] / max(plain["Msamples_per_s"], 1e-9), 3),
@@ -451,8 +455,8 @@ def main() -> int:
     ap.add_argument("--spp-max", type=int, default=64, help="serving: largest target spp")
     ap.add_argument("--slots", type=int, default=8, help="serving: concurrent job slots")
     ap.add_argument("--served-precision", default="quantized-bvh,fp16-guides,compensated",
-                    help="serving: precision variants of the served run, comma-separated "
-                         "(quantized-bvh, fp16-guides, compensated; empty for plain FP32)")
+                    help="serving: precision variants of the batched variant run, comma-separated "
+                         "(quantized-bvh, fp16-guides, compensated)")
     ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent", "wavefront", "tuned"],
                     default="per-pixel",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "

@@ -54,7 +54,9 @@ extern "C" {
 #define RM_OPTION_PRECISION 2          /* RM_PRECISION_* flags; 0 (default): the plain FP32 path */
 #define RM_OPTION_AUTOTUNE 3           /* job servers: frames per candidate with which rm_tune picks the schedule
                                           for each new scene and image size; 0 (default): no tuning */
-#define RM_OPTION_COUNT 4
+#define RM_OPTION_BATCH 4              /* job servers: 1 gives the frames of the jobs running at once one
+                                          path-pass launch (rm_frame_batch); 0 (default): a launch per job */
+#define RM_OPTION_COUNT 5
 
 /* Reduced- and extended-precision variants (RM_OPTION_PRECISION flags). */
 #define RM_PRECISION_QUANTIZED_BVH 1   /* traverse the 8-bit child-quantized tree (rm_bvh_quantize) in the
@@ -156,6 +158,18 @@ int rm_resize(rm_context* ctx, uint32_t width, uint32_t height);
  * Replaces the compute pass of Renderer.render (src/Renderer.mjs:441-489). */
 int rm_frame(rm_context* ctx, const rm_scene* scene, const void* params, const float* emitters,
              uint32_t emitter_vec4s);
+
+/* One frame on each of n contexts (at most 8, all on one device) with their
+ * path passes in a single launch: each context's frame as rm_frame enqueues
+ * it, except that the per-pixel path kernel runs once for all of them, the
+ * grid's tiles split among the jobs, whatever their schedules. Every job
+ * renders exactly as it would alone. The frames must walk the same kind of
+ * tree (RM_PRECISION_QUANTIZED_BVH set alike). emitters and emitter_vec4s
+ * may be NULL when no frame traces light. No counterpart in the reference,
+ * which renders one image per Renderer; the job server uses it under
+ * RM_OPTION_BATCH. */
+int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const void* const* params,
+                   const float* const* emitters, const uint32_t* emitter_vec4s, uint32_t n);
 
 /* Waits for the queued work and copies `floats` floats of a per-pixel buffer.
  * Replaces readPixels' copy and map (src/Renderer.mjs:502-528). */

@@ -29,6 +29,7 @@ SIGNATURES = {
     "rm_scene_textures": (C.c_int, [_P, _P, C.c_int, _P, _U32, _U32, _U32]),
     "rm_environment": (C.c_int, [_P, _P, _U32, _U32, _P, _U32, _U32]),
     "rm_resize": (C.c_int, [_P, _U32, _U32]),
+    "rm_frame_batch": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32]),
     "rm_frame": (C.c_int, [_P, _P, _P, _P, _U32]),
     "rm_read": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rm_synchronize": (C.c_int, [_P]),
@@ -50,7 +51,7 @@ SIGNATURES = {
     "rm_server_close": (C.c_int, [_P]),
     "rm_server_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
-OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE = 0, 1, 2, 3
+OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE, OPTION_BATCH = 0, 1, 2, 3, 4
 PRECISION_QUANTIZED_BVH, PRECISION_FP16_GUIDES, PRECISION_COMPENSATED = 1, 2, 4
 SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT, SCHEDULE_WAVEFRONT = 0, 1, 2, 3
 SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT,
@@ -93,6 +94,21 @@ def _ptr(a: np.ndarray | None):
 def _check(code: int, what: str) -> None:
     if code != RM_OK:
         raise RaymondError(f"{what}: {load().rm_last_error().decode()} (code {code})")
+
+
+def frame_batch(contexts, scenes, params, emitters=None) -> None:
+    """rm_frame_batch: one frame on each context, their path passes in one launch."""
+    n = len(contexts)
+    ctxs = (C.c_void_p * n)(*[c.handle.value if isinstance(c.handle, C.c_void_p) else c.handle for c in contexts])
+    scs = (C.c_void_p * n)(*[s.handle.value if isinstance(s.handle, C.c_void_p) else s.handle for s in scenes])
+    blocks = [np.ascontiguousarray(p) for p in params]
+    pp = (C.c_void_p * n)(*[b.ctypes.data for b in blocks])
+    em = ev = None
+    if emitters is not None and any(e is not None for e in emitters):
+        es = [None if e is None else np.ascontiguousarray(e, np.float32) for e in emitters]
+        em = (C.c_void_p * n)(*[None if e is None else e.ctypes.data for e in es])
+        ev = (C.c_uint32 * n)(*[0 if e is None else e.size // 4 for e in es])
+    _check(load().rm_frame_batch(ctxs, scs, pp, em, ev, n), "rm_frame_batch")
 
 
 def bvh_build(mins: np.ndarray, maxs: np.ndarray, max_leaf_size: int = 4, threads: int = 0):

@@ -129,14 +129,19 @@ int rmi::current_sums(rm_context* c, hipStream_t stream, const float4* out[3]) {
 
 namespace {
 
+}  // namespace
+
 // The tree traversals walk: the FP32 nodes, or when quantized nodes are
 // asked for, the 4-wide quantized tree, else the binary quantized one, else
 // (a tree that can't be quantized) the FP32 nodes.
-int tree_of(const rm_context* c, const rm_scene* s) {
+int rmi::tree_of(const rm_context* c, const rm_scene* s) {
     if (!(c->options[RM_OPTION_PRECISION] & RM_PRECISION_QUANTIZED_BVH)) return rm::TREE_FP32;
     if (s->wnodes.ptr) return rm::TREE_WIDE;
     return s->qnodes.ptr ? rm::TREE_QUANTIZED : rm::TREE_FP32;
 }
+
+namespace {
+using rmi::tree_of;
 
 int upload(rm_context* c, DevBuf& b, const void* src, size_t bytes, const char* what) {
     size_t alloc = bytes < 16 ? 16 : bytes;
@@ -175,7 +180,9 @@ int rm_open(int device, rm_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->post, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->prepared_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->denoised_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->batch_ev) (void)hipEventDestroy(c->batch_ev);
         if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
         if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
         if (c->post) (void)hipStreamDestroy(c->post);
@@ -212,6 +219,8 @@ int rm_set_option(rm_context* c, int option, int64_t value) {
                     "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT, _PERSISTENT or _WAVEFRONT.");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
+    if (option == RM_OPTION_BATCH && (value < 0 || value > 1))
+        return fail(RM_ERR_INVALID, "RM_OPTION_BATCH is 0 or 1.");
     if (option == RM_OPTION_AUTOTUNE && (value < 0 || value > 64))
         return fail(RM_ERR_INVALID, "RM_OPTION_AUTOTUNE is 0 (off) to 64 frames per candidate.");
     if (option == RM_OPTION_PRECISION &&
@@ -240,6 +249,7 @@ int rm_close(rm_context* c) {
             (void)hipEventDestroy(p.second);
         }
     for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+    if (c->batch_ev) (void)hipEventDestroy(c->batch_ev);
     if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
     if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
     if (c->post) (void)hipStreamDestroy(c->post);
@@ -388,9 +398,23 @@ int rm_resize(rm_context* c, uint32_t width, uint32_t height) {
     return RM_OK;
 }
 
-int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* emitters, uint32_t emitter_vec4s) {
-    if (!c || !s || !params) return fail(RM_ERR_INVALID, "rm_frame needs a context, a scene and parameters.");
+// A frame in three parts around its path pass, so that rm_frame_batch can
+// give several contexts' path passes one launch.
+struct Frame {
+    const rm_scene* s;
     rm::Params P;
+    rm::DevScene S;
+    rm::Features f;
+    bool caustics;
+    float4* comp;
+    uint32_t px;
+};
+
+// The checks of rm_frame's arguments, before anything is enqueued.
+static int frame_check(rm_context* c, const rm_scene* s, const void* params, const float* emitters,
+                       uint32_t emitter_vec4s, Frame& F) {
+    if (!c || !s || !params) return fail(RM_ERR_INVALID, "rm_frame needs a context, a scene and parameters.");
+    rm::Params& P = F.P;
     std::memcpy(&P, params, sizeof P);
     if (c->width == 0) return fail(RM_ERR_STATE, "rm_frame before rm_resize.");
     if (P.width != c->width || P.height != c->height)
@@ -415,11 +439,24 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
                 return fail(RM_ERR_INVALID, "An emitter names a primitive the scene doesn't have.");
         }
     }
+    F.s = s;
+    F.caustics = caustics;
+    return RM_OK;
+}
+
+// Everything before the path pass: light tracing and the photon gather on
+// the side stream, the compensation terms.
+static int frame_begin(rm_context* c, const void* emitters, uint32_t emitter_vec4s, Frame& F) {
+    const rm_scene* s = F.s;
+    const rm::Params& P = F.P;
+    const bool caustics = F.caustics;
     RM_HIP(hipSetDevice(c->device), "selecting the device");
 
-    rm::DevScene S = dev_scene(c, s);
-    rm::Features f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
-                      s->dispersive, tree_of(c, s)};
+    rm::DevScene& S = F.S;
+    S = dev_scene(c, s);
+    rm::Features& f = F.f;
+    f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
+         s->dispersive, tree_of(c, s)};
     // Light tracing and the photon gather depend on the scene and the
     // parameters only, so they run on the side stream while the path pass
     // runs; k_resolve then adds splats and gathered light, after the path
@@ -427,7 +464,7 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
     // for everything queued before this frame (earlier frames' readers of
     // the splats, photons and gathered light), the join orders k_resolve
     // after it. The side stream keeps its own traversal spill.
-    const uint32_t px = c->width * c->height;
+    const uint32_t px = F.px = c->width * c->height;
     if (caustics) {
         RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
         RM_HIP(c->photons.ensure((size_t)rm::PHOTON_CAPACITY * 48), "allocating photons");
@@ -461,6 +498,28 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         RM_HIP(hipGetLastError(), "launching light tracing");
     }
 
+    // Compensated sums: the error terms start at zero with the image (the
+    // path pass writes them on a first frame), or when compensation starts
+    // part-way through an accumulation.
+    const bool compensated = (c->options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0;
+    float4*& comp = F.comp;
+    comp = nullptr;
+    if (compensated) {
+        RM_HIP(c->comp.ensure((size_t)px * 48), "allocating the compensation terms");
+        comp = c->comp.as<float4>();
+        if (!c->comp_live && P.sample_offset != 0u) rm::launch_fill_u32(c->comp.as<uint32_t>(), 0u, (size_t)px * 12, c->stream);
+    }
+    c->comp_live = compensated;
+    return RM_OK;
+}
+
+// The path pass under the context's schedule.
+static int frame_path(rm_context* c, Frame& F) {
+    const rm::Params& P = F.P;
+    const rm::DevScene& S = F.S;
+    const rm::Features& f = F.f;
+    const uint32_t px = F.px;
+    float4* comp = F.comp;
     uint32_t slots = px;
     uint32_t persistent = 0;
     float4* staging = nullptr;
@@ -482,17 +541,6 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         staging = c->staging.as<float4>();
         slots = threads;
     }
-    // Compensated sums: the error terms start at zero with the image (the
-    // path pass writes them on a first frame), or when compensation starts
-    // part-way through an accumulation.
-    const bool compensated = (c->options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0;
-    float4* comp = nullptr;
-    if (compensated) {
-        RM_HIP(c->comp.ensure((size_t)px * 48), "allocating the compensation terms");
-        comp = c->comp.as<float4>();
-        if (!c->comp_live && P.sample_offset != 0u) rm::launch_fill_u32(c->comp.as<uint32_t>(), 0u, (size_t)px * 12, c->stream);
-    }
-    c->comp_live = compensated;
     if (schedule == RM_SCHEDULE_WAVEFRONT && P.view == 0u) {
         // Every stage's grid covers all paths, a thread per queue entry.
         const uint32_t blocks = (px + rm::BLOCK_THREADS - 1) / rm::BLOCK_THREADS;
@@ -526,6 +574,14 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
         rm::launch_main(L, f, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the trace");
+    return RM_OK;
+}
+
+// Everything after the path pass: the splat and gather resolve, the denoiser.
+static int frame_end(rm_context* c, Frame& F) {
+    const rm::Params& P = F.P;
+    const bool caustics = F.caustics;
+    float4* comp = F.comp;
 
     if (caustics) {
         RM_HIP(hipStreamWaitEvent(c->stream, c->join, 0), "joining the side stream");
@@ -572,6 +628,79 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
     }
     c->last_params = P;
     c->rendered = true;
+    return RM_OK;
+}
+
+
+int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* emitters, uint32_t emitter_vec4s) {
+    Frame F;
+    int r = frame_check(c, s, params, emitters, emitter_vec4s, F);
+    if (r == RM_OK) r = frame_begin(c, emitters, emitter_vec4s, F);
+    if (r == RM_OK) r = frame_path(c, F);
+    if (r == RM_OK) r = frame_end(c, F);
+    return r;
+}
+
+int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const void* const* params,
+                   const float* const* emitters, const uint32_t* emitter_vec4s, uint32_t n) {
+    if (!ctxs || !scenes || !params || n == 0 || n > rm::MAX_BATCH)
+        return fail(RM_ERR_INVALID, "rm_frame_batch needs 1 to " + std::to_string(rm::MAX_BATCH) +
+                                        " contexts, scenes and parameter blocks.");
+    Frame F[rm::MAX_BATCH];
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(RM_ERR_INVALID, "rm_frame_batch needs a context for every frame.");
+        for (uint32_t k = 0; k < i; ++k)
+            if (ctxs[k] == ctxs[i]) return fail(RM_ERR_INVALID, "A batch takes one frame per context.");
+        if (ctxs[i]->device != ctxs[0]->device) return fail(RM_ERR_INVALID, "A batch's contexts share one device.");
+        int r = frame_check(ctxs[i], scenes[i], params[i], emitters ? emitters[i] : nullptr,
+                            emitter_vec4s ? emitter_vec4s[i] : 0u, F[i]);
+        if (r != RM_OK) return r;
+        if (tree_of(ctxs[i], scenes[i]) != tree_of(ctxs[0], scenes[0]))
+            return fail(RM_ERR_INVALID, "A batch's frames walk the same kind of tree (RM_PRECISION_QUANTIZED_BVH "
+                                        "the same for all, and quantizable scenes).");
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        int r = frame_begin(ctxs[i], emitters ? emitters[i] : nullptr, emitter_vec4s ? emitter_vec4s[i] : 0u, F[i]);
+        if (r != RM_OK) return r;
+    }
+    // One launch for all the path passes, on the first context's stream once
+    // every context has queued what comes before; each context's stream then
+    // waits for it and goes on with its own frame.
+    rm::BatchTable T;
+    std::memset(&T, 0, sizeof T);
+    rm::Features u = F[0].f;
+    uint32_t tiles = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        rm_context* c = ctxs[i];
+        const uint32_t tx = (c->width + 15u) / 16u, ty = (c->height + 15u) / 16u;
+        T.job[i] = rm::BatchJob{F[i].S, F[i].P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(),
+                                F[i].comp, c->spill.as<uint32_t>(), F[i].px, tx};
+        T.begin[i] = tiles;
+        tiles += tx * ty;
+        u.volumes |= F[i].f.volumes;
+        u.textures |= F[i].f.textures;
+        u.environment |= F[i].f.environment;
+        u.dispersion |= F[i].f.dispersion;
+    }
+    T.begin[n] = tiles;
+    T.n = n;
+    rm_context* run = ctxs[0];
+    for (uint32_t i = 1; i < n; ++i) {
+        RM_HIP(hipEventRecord(ctxs[i]->batch_ev, ctxs[i]->stream), "ordering a batch");
+        RM_HIP(hipStreamWaitEvent(run->stream, ctxs[i]->batch_ev, 0), "ordering a batch");
+    }
+    {
+        Timed t(run, RM_STAGE_TRACE);
+        rm::launch_main_batch(T, u, run->stream);
+    }
+    RM_HIP(hipGetLastError(), "launching the batched trace");
+    RM_HIP(hipEventRecord(run->batch_ev, run->stream), "ordering a batch");
+    for (uint32_t i = 1; i < n; ++i)
+        RM_HIP(hipStreamWaitEvent(ctxs[i]->stream, run->batch_ev, 0), "ordering a batch");
+    for (uint32_t i = 0; i < n; ++i) {
+        int r = frame_end(ctxs[i], F[i]);
+        if (r != RM_OK) return r;
+    }
     return RM_OK;
 }
 

@@ -76,6 +76,7 @@ struct rm_context {
     // start meanwhile; `prepared_ev` and `denoised_ev` order the two.
     hipStream_t post = nullptr;
     hipEvent_t prepared_ev = nullptr, denoised_ev = nullptr;
+    hipEvent_t batch_ev = nullptr;   // orders a batched path pass (rm_frame_batch) with this context's stream
     bool denoise_pending = false;
     uint32_t width = 0, height = 0, capacity = 0;
     rmi::DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats, guides, albedo_frame;
@@ -103,4 +104,6 @@ namespace rmi {
 // value, ordered on `stream`: the running sums themselves, or when the last
 // frame kept compensated sums, sum + error folded into ctx->folded.
 int current_sums(rm_context* ctx, hipStream_t stream, const float4* out[3]);
+// The tree a frame on ctx walks for scene (rm::TREE_*).
+int tree_of(const rm_context* ctx, const rm_scene* scene);
 }  // namespace rmi

@@ -41,6 +41,29 @@ struct LaunchMain {
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
 
+// Several jobs' path passes in one launch of the per-pixel kernel, the job
+// table passed by value as the kernel's argument.
+constexpr uint32_t MAX_BATCH = 8;
+struct BatchJob {
+    DevScene S;
+    Params P;
+    float4* accum;
+    float4* galb;
+    float4* gnorm;
+    float4* comp;        // non-null: compensated sums
+    uint32_t* spill;
+    uint32_t slots;
+    uint32_t tiles_x;    // 16×16 tiles across the job's image
+};
+struct BatchTable {
+    BatchJob job[MAX_BATCH];
+    uint32_t begin[MAX_BATCH + 1];   // job j's tiles are [begin[j], begin[j + 1]) of the grid
+    uint32_t n;
+};
+static_assert(sizeof(BatchTable) <= 4096, "the job table travels as the kernel argument");
+// f: the union of the jobs' features (a job renders the same in a kernel with more of them).
+void launch_main_batch(const BatchTable& B, const Features& f, hipStream_t stream);
+
 // The wavefront path pass's buffers (RM_SCHEDULE_WAVEFRONT), for n paths (the
 // frame's pixels): path state, closest hits, pending shadow rays, each pixel's
 // sums, the live queues of this bounce and the next, the shadow and done

@@ -98,7 +98,8 @@ struct rm_server {
 
     void run();
     bool admit();
-    bool progress(Slot& s);
+    bool progress(Slot& s, bool enqueue);
+    bool enqueue_batched();
     void finish(Slot& s, int error, const std::string& message);
 };
 
@@ -198,7 +199,7 @@ bool rm_server::admit() {
 
 // Advances one slot: retires finished frames, enqueues more, reads back a
 // finished job. Returns whether anything changed.
-bool rm_server::progress(Slot& s) {
+bool rm_server::progress(Slot& s, bool enqueue) {
     if (!s.job) return false;
     Job& j = *s.job;
     bool changed = false;
@@ -215,7 +216,7 @@ bool rm_server::progress(Slot& s) {
         s.in_flight--;
         changed = true;
     }
-    while (s.in_flight < 2 && j.frames_enqueued < j.frames_total) {
+    while (enqueue && s.in_flight < 2 && j.frames_enqueued < j.frames_total) {
         uint32_t p[RM_PARAMS_BYTES / 4];
         frame_params(j, j.frames_enqueued, p);
         int r = rm_frame(s.ctx, j.scene, p, j.caustics ? j.emitters.data() : nullptr,
@@ -282,15 +283,71 @@ bool rm_server::progress(Slot& s) {
     return changed;
 }
 
+// With RM_OPTION_BATCH: the next frame of every slot with room, in batches
+// of jobs that walk the same tree, one path-pass launch a batch.
+bool rm_server::enqueue_batched() {
+    bool changed = false;
+    for (;;) {
+        std::vector<Slot*> ready;
+        for (Slot& s : slots)
+            if (s.job && s.in_flight < 2 && s.job->frames_enqueued < s.job->frames_total) ready.push_back(&s);
+        if (ready.empty()) return changed;
+        const int tree = rmi::tree_of(ready[0]->ctx, ready[0]->job->scene);
+        std::vector<Slot*> group;
+        for (Slot* s : ready)
+            if (group.size() < rm::MAX_BATCH && rmi::tree_of(s->ctx, s->job->scene) == tree) group.push_back(s);
+        const uint32_t n = (uint32_t)group.size();
+        rm_context* ctxs[rm::MAX_BATCH];
+        const rm_scene* scenes[rm::MAX_BATCH];
+        uint32_t params[rm::MAX_BATCH][RM_PARAMS_BYTES / 4];
+        const void* pp[rm::MAX_BATCH];
+        const float* em[rm::MAX_BATCH];
+        uint32_t ev[rm::MAX_BATCH];
+        for (uint32_t i = 0; i < n; ++i) {
+            const Job& j = *group[i]->job;
+            ctxs[i] = group[i]->ctx;
+            scenes[i] = j.scene;
+            frame_params(j, j.frames_enqueued, params[i]);
+            pp[i] = params[i];
+            em[i] = j.caustics ? j.emitters.data() : nullptr;
+            ev[i] = j.caustics ? (uint32_t)(j.emitters.size() / 4) : 0u;
+        }
+        int r = rm_frame_batch(ctxs, scenes, pp, em, ev, n);
+        changed = true;
+        if (r != RM_OK) {
+            const std::string message = rm_last_error();
+            for (Slot* s : group) finish(*s, r, message);
+            continue;
+        }
+        for (Slot* s : group) {
+            Job& j = *s->job;
+            hipStream_t last = denoises(j, j.frames_enqueued) ? s->ctx->post : s->ctx->stream;
+            hipError_t e = hipEventRecord(s->frame_ev[(s->head + s->in_flight) & 1u], last);
+            if (e != hipSuccess) {
+                finish(*s, RM_ERR_DEVICE, std::string("recording a frame event: ") + hipGetErrorString(e));
+                continue;
+            }
+            j.frames_enqueued++;
+            s->in_flight++;
+        }
+    }
+}
+
 void rm_server::run() {
     (void)hipSetDevice(device);
     for (;;) {
         bool changed = admit();
         bool active = false;
+        bool batched;
+        {
+            std::lock_guard<std::mutex> lock(m);
+            batched = options[RM_OPTION_BATCH] != 0;
+        }
         for (Slot& s : slots) {
-            changed = progress(s) || changed;
+            changed = progress(s, !batched) || changed;
             active = active || (bool)s.job;
         }
+        if (batched) changed = enqueue_batched() || changed;
         if (!active) {
             std::unique_lock<std::mutex> lock(m);
             if (stop && queue.empty()) return;
