@@ -40,6 +40,9 @@ struct LaunchMain {
 };
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
+// The persistent and compacted schedules' launches (launch_main picks them).
+void launch_main_persistent(const LaunchMain& L, const Features& f, hipStream_t stream);
+void launch_main_compact(const LaunchMain& L, const Features& f, hipStream_t stream);
 
 // Several jobs' path passes in one launch of the per-pixel kernel, the job
 // table passed by value as the kernel's argument.
