@@ -165,8 +165,11 @@ int rm_frame(rm_context* ctx, const rm_scene* scene, const void* params, const f
  * grid's tiles split among the jobs, whatever their schedules. Every job
  * renders exactly as it would alone. The frames must walk the same kind of
  * tree (RM_PRECISION_QUANTIZED_BVH set alike). emitters and emitter_vec4s
- * may be NULL when no frame traces light. No counterpart in the reference,
- * which renders one image per Renderer; the job server uses it under
+ * may be NULL when no frame traces light. Arguments are all checked before
+ * anything is queued; a later failure (device memory running out) can leave
+ * earlier contexts of the batch with a frame begun but no path pass, whose
+ * images must then start again. No counterpart in the reference, which
+ * renders one image per Renderer; the job server uses it under
  * RM_OPTION_BATCH. */
 int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const void* const* params,
                    const float* const* emitters, const uint32_t* emitter_vec4s, uint32_t n);
@@ -210,8 +213,8 @@ int rm_set_option(rm_context* ctx, int option, int64_t value);
  * sets the fastest as the context's RM_OPTION_SCHEDULE (and
  * RM_OPTION_PERSISTENT_BLOCKS). The schedules give the same sums bit for bit,
  * so tuning changes speed only. The frames render into the context's
- * buffers: begin the image again (sample offset 0) afterwards. No
- * counterpart in the reference. */
+ * buffers: begin the image again (sample offset 0) afterwards. On failure the
+ * context keeps the schedule it had. No counterpart in the reference. */
 #define RM_TUNE_CANDIDATES 5
 typedef struct rm_tune_result {
     int32_t schedule[RM_TUNE_CANDIDATES];            /* each candidate's RM_SCHEDULE_* */

@@ -822,6 +822,7 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, uint32_t frame
     const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 2u * cus, 0u};
     rm_tune_result res = {};
     const bool timing = c->timing;
+    const int64_t schedule = c->options[RM_OPTION_SCHEDULE], blocks_before = c->options[RM_OPTION_PERSISTENT_BLOCKS];
     c->timing = true;
     double best_ms = 0.0;
     int r = RM_OK;
@@ -848,7 +849,11 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, uint32_t frame
         }
     }
     c->timing = timing;
-    if (r != RM_OK) return r;
+    if (r != RM_OK) {
+        c->options[RM_OPTION_SCHEDULE] = schedule;
+        c->options[RM_OPTION_PERSISTENT_BLOCKS] = blocks_before;
+        return r;
+    }
     c->options[RM_OPTION_SCHEDULE] = res.schedule[res.best];
     c->options[RM_OPTION_PERSISTENT_BLOCKS] = res.persistent_blocks[res.best];
     if (out) *out = res;
