@@ -158,13 +158,14 @@ def bench_frames(args, name: str) -> dict:
     if not tuned:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULES[args.schedule])
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, args.persistent_blocks)
+        ctx.set_option(capi.OPTION_WAVES, args.waves)
     ctx.set_option(capi.OPTION_PRECISION, capi.PRECISION_QUANTIZED_BVH if args.quantized_bvh else 0)
     scene = ctx.upload_scene(p)
     ctx.environment(p.env_rgba, p.meta["envWidth"], p.meta["envHeight"], p.env_table, p.meta["envTableWidth"],
                     p.meta["envTableHeight"])
     ctx.resize(w, h)
-    tuning = ctx.tune(scene, frame_params(0), frames=2) if tuned else None
     emitters = p.emitters if caustics else None
+    tuning = ctx.tune(scene, frame_params(0), frames=2, emitters=emitters) if tuned else None
 
     k = 0
     for _ in range(args.warmup):
@@ -212,7 +213,7 @@ def bench_frames(args, name: str) -> dict:
                    "photon_gather": bool(caustics), "denoise_passes": 5, "frames_timed": args.steps,
                    "primitives": p.primitive_count, "device": device, "compute_units": cus,
                    "path": "one job, megakernel, " + ("8-bit quantized BVH" if args.quantized_bvh else "FP32 BVH"),
-                   "schedule": args.schedule,
+                   "schedule": args.schedule, "waves_per_simd": tuning["best"][2] if tuned else args.waves,
                    "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_summary(wl["traffic"]),
@@ -461,6 +462,8 @@ def main() -> int:
                     default="per-pixel",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
                          "or lanes regenerating paths from a tile queue")
+    ap.add_argument("--waves", type=int, choices=[4, 5], default=4,
+                    help="waves per SIMD of the one-thread-per-pixel path pass (RM_OPTION_WAVES)")
     ap.add_argument("--quantized-bvh", action="store_true",
                     help="traverse the 8-bit child-quantized tree (same hits as the FP32 tree)")
     ap.add_argument("--persistent-blocks", type=int, default=0, help="blocks for the persistent pass (0: fill the GPU)")

@@ -56,7 +56,9 @@ extern "C" {
                                           for each new scene and image size; 0 (default): no tuning */
 #define RM_OPTION_BATCH 4              /* job servers: 1 gives the frames of the jobs running at once one
                                           path-pass launch (rm_frame_batch); 0 (default): a launch per job */
-#define RM_OPTION_COUNT 5
+#define RM_OPTION_WAVES 5              /* waves per SIMD that RM_SCHEDULE_PIXEL's path pass is compiled for:
+                                          4 (or 0, the default) or 5; the same results bit for bit */
+#define RM_OPTION_COUNT 6
 
 /* Reduced- and extended-precision variants (RM_OPTION_PRECISION flags). */
 #define RM_PRECISION_QUANTIZED_BVH 1   /* traverse the 8-bit child-quantized tree (rm_bvh_quantize) in the
@@ -207,23 +209,29 @@ int rm_reset_stage_times(rm_context* ctx);
  * these choose among equivalent kernel schedules (and let a tuner pick). */
 int rm_set_option(rm_context* ctx, int option, int64_t value);
 
-/* The autotuner. Times the path pass alone (denoiser and light tracing off)
- * for `frames` frames after one warm-up under each candidate schedule — one
- * thread per pixel, compacted, persistent at two grid sizes, wavefront — and
- * sets the fastest as the context's RM_OPTION_SCHEDULE (and
- * RM_OPTION_PERSISTENT_BLOCKS). The schedules give the same sums bit for bit,
- * so tuning changes speed only. The frames render into the context's
- * buffers: begin the image again (sample offset 0) afterwards. On failure the
- * context keeps the schedule it had. No counterpart in the reference. */
-#define RM_TUNE_CANDIDATES 5
+/* The autotuner. Renders `frames` whole frames after one warm-up with the
+ * caller's parameters — light tracing (given its emitters) and the denoiser
+ * included when they turn them on — under each candidate: one thread per
+ * pixel at four and at five waves per SIMD, compacted, persistent at two grid
+ * sizes, wavefront. It sets the fastest as the context's RM_OPTION_SCHEDULE,
+ * RM_OPTION_PERSISTENT_BLOCKS and RM_OPTION_WAVES. Whole frames, because the
+ * path pass shares the GPU with light tracing and the previous frame's
+ * denoiser: a path pass that holds more waves can be faster alone and still
+ * make the frame slower. The candidates give the same sums bit for bit, so
+ * tuning changes speed only. The frames render into the context's buffers:
+ * begin the image again (sample offset 0) afterwards. On failure the context
+ * keeps the options it had. No counterpart in the reference. */
+#define RM_TUNE_CANDIDATES 6
 typedef struct rm_tune_result {
     int32_t schedule[RM_TUNE_CANDIDATES];            /* each candidate's RM_SCHEDULE_* */
     uint32_t persistent_blocks[RM_TUNE_CANDIDATES];  /* and its persistent grid (0: fill the GPU) */
-    double ms[RM_TUNE_CANDIDATES];                   /* mean ms of its path pass per frame */
+    uint32_t waves[RM_TUNE_CANDIDATES];              /* and its RM_OPTION_WAVES */
+    double ms[RM_TUNE_CANDIDATES];                   /* mean ms per whole frame */
     uint32_t count;                                  /* candidates tried */
     uint32_t best;                                   /* the one now set */
 } rm_tune_result;
-int rm_tune(rm_context* ctx, const rm_scene* scene, const void* params, uint32_t frames, rm_tune_result* out);
+int rm_tune(rm_context* ctx, const rm_scene* scene, const void* params, const float* emitters,
+            uint32_t emitter_vec4s, uint32_t frames, rm_tune_result* out);
 
 /* The device's name and compute-unit count. */
 int rm_device_info(rm_context* ctx, char* name, size_t name_len, int* compute_units);

@@ -42,7 +42,7 @@ SIGNATURES = {
     "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
     "rm_reset_stage_times": (C.c_int, [_P]),
     "rm_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
-    "rm_tune": (C.c_int, [_P, _P, _P, C.c_uint32, _P]),
+    "rm_tune": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P]),
     "rm_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
     "rm_server_open": (C.c_int, [C.c_int, _U32, C.c_uint64, C.POINTER(_P)]),
     "rm_server_context": (_P, [_P]),
@@ -51,7 +51,7 @@ SIGNATURES = {
     "rm_server_close": (C.c_int, [_P]),
     "rm_server_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
-OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE, OPTION_BATCH = 0, 1, 2, 3, 4
+OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE, OPTION_BATCH, OPTION_WAVES = range(6)
 PRECISION_QUANTIZED_BVH, PRECISION_FP16_GUIDES, PRECISION_COMPENSATED = 1, 2, 4
 SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT, SCHEDULE_WAVEFRONT = 0, 1, 2, 3
 SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT,
@@ -60,12 +60,13 @@ SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persiste
 _lib = None
 
 
-TUNE_CANDIDATES = 5
+TUNE_CANDIDATES = 6
 
 
 class _TuneResult(C.Structure):
     _fields_ = [("schedule", C.c_int32 * TUNE_CANDIDATES), ("persistent_blocks", C.c_uint32 * TUNE_CANDIDATES),
-                ("ms", C.c_double * TUNE_CANDIDATES), ("count", C.c_uint32), ("best", C.c_uint32)]
+                ("waves", C.c_uint32 * TUNE_CANDIDATES), ("ms", C.c_double * TUNE_CANDIDATES), ("count", C.c_uint32),
+                ("best", C.c_uint32)]
 
 
 class RaymondError(RuntimeError):
@@ -249,16 +250,19 @@ class Context:
                "rm_raycast")
         return (hits, cnt) if counters else hits
 
-    def tune(self, scene: Scene, params: np.ndarray, frames: int = 2) -> dict:
-        """rm_tune: times each candidate schedule's path pass and keeps the
-        fastest; returns {"best": (schedule name, blocks), "candidates":
-        [(schedule name, blocks, ms), ...]}."""
+    def tune(self, scene: Scene, params: np.ndarray, frames: int = 2, emitters: np.ndarray | None = None) -> dict:
+        """rm_tune: times whole frames under each candidate (schedule,
+        persistent blocks, waves) and keeps the fastest; returns {"best":
+        (schedule name, blocks, waves), "candidates": [(schedule name,
+        blocks, waves, ms per frame), ...]}. emitters: as for frame(), when
+        the parameters turn light tracing on."""
         res = _TuneResult()
-        _check(load().rm_tune(self.handle, scene.handle, _ptr(np.ascontiguousarray(params)), frames, C.byref(res)),
-               "rm_tune")
+        em = None if emitters is None else np.ascontiguousarray(emitters, dtype=np.float32)
+        _check(load().rm_tune(self.handle, scene.handle, _ptr(np.ascontiguousarray(params)), _ptr(em),
+                              0 if em is None else em.size // 4, frames, C.byref(res)), "rm_tune")
         names = {v: k for k, v in SCHEDULES.items()}
-        cands = [(names[res.schedule[i]], res.persistent_blocks[i], res.ms[i]) for i in range(res.count)]
-        return {"best": cands[res.best][:2], "candidates": cands}
+        cands = [(names[res.schedule[i]], res.persistent_blocks[i], res.waves[i], res.ms[i]) for i in range(res.count)]
+        return {"best": cands[res.best][:3], "candidates": cands}
 
     def set_option(self, option: int, value: int) -> None:
         _check(load().rm_set_option(self.handle, option, value), "rm_set_option")

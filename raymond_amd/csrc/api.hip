@@ -1,5 +1,6 @@
 // The C ABI (include/raymond.h) over the kernels: device memory, the frame's
 // dispatch order (src/Renderer.mjs:441-489), readback and stage timing.
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -219,6 +220,8 @@ int rm_set_option(rm_context* c, int option, int64_t value) {
                     "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT, _PERSISTENT or _WAVEFRONT.");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
+    if (option == RM_OPTION_WAVES && value != 0 && value != 4 && value != 5)
+        return fail(RM_ERR_INVALID, "RM_OPTION_WAVES is 4 or 5 (0: the default, 4).");
     if (option == RM_OPTION_BATCH && (value < 0 || value > 1))
         return fail(RM_ERR_INVALID, "RM_OPTION_BATCH is 0 or 1.");
     if (option == RM_OPTION_AUTOTUNE && (value < 0 || value > 64))
@@ -570,7 +573,8 @@ static int frame_path(rm_context* c, Frame& F) {
     } else {
         Timed t(c, RM_STAGE_TRACE);
         rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), comp,
-                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging, f.tree};
+                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging, f.tree,
+                         c->options[RM_OPTION_WAVES] == 5 ? 5 : 4};
         rm::launch_main(L, f, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the trace");
@@ -808,54 +812,62 @@ int rm_reset_stage_times(rm_context* c) {
     return RM_OK;
 }
 
-int rm_tune(rm_context* c, const rm_scene* s, const void* params, uint32_t frames, rm_tune_result* out) {
+int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* emitters, uint32_t emitter_vec4s,
+            uint32_t frames, rm_tune_result* out) {
     if (!c || !s || !params || frames == 0 || frames > 64)
         return fail(RM_ERR_INVALID, "rm_tune needs a context, a scene, parameters and 1 to 64 frames.");
     rm::Params P;
     std::memcpy(&P, params, sizeof P);
-    P.denoise = 0u;
-    P.caustics = 0u;
     const uint32_t spp = P.samples_per_frame > 0 ? P.samples_per_frame : 1u;
     const uint32_t cus = (uint32_t)(c->compute_units > 0 ? c->compute_units : 1);
-    const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_PIXEL, RM_SCHEDULE_COMPACT, RM_SCHEDULE_PERSISTENT,
+    const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_PIXEL,      RM_SCHEDULE_PIXEL,
+                                                  RM_SCHEDULE_COMPACT,    RM_SCHEDULE_PERSISTENT,
                                                   RM_SCHEDULE_PERSISTENT, RM_SCHEDULE_WAVEFRONT};
-    const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 2u * cus, 0u};
+    const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 0u, 2u * cus, 0u};
+    const uint32_t waves[RM_TUNE_CANDIDATES] = {4u, 5u, 4u, 4u, 4u, 4u};
     rm_tune_result res = {};
-    const bool timing = c->timing;
-    const int64_t schedule = c->options[RM_OPTION_SCHEDULE], blocks_before = c->options[RM_OPTION_PERSISTENT_BLOCKS];
-    c->timing = true;
+    const int64_t before[3] = {c->options[RM_OPTION_SCHEDULE], c->options[RM_OPTION_PERSISTENT_BLOCKS],
+                               c->options[RM_OPTION_WAVES]};
+    auto finish_all = [&]() -> int {
+        RM_HIP(hipStreamSynchronize(c->stream), "tuning");
+        RM_HIP(hipStreamSynchronize(c->side), "tuning");
+        RM_HIP(hipStreamSynchronize(c->post), "tuning");
+        return RM_OK;
+    };
     double best_ms = 0.0;
     int r = RM_OK;
     for (uint32_t k = 0; k < RM_TUNE_CANDIDATES && r == RM_OK; ++k) {
         c->options[RM_OPTION_SCHEDULE] = schedules[k];
         c->options[RM_OPTION_PERSISTENT_BLOCKS] = blocks[k];
+        c->options[RM_OPTION_WAVES] = waves[k];
+        std::chrono::steady_clock::time_point t0;
         for (uint32_t f = 0; f <= frames && r == RM_OK; ++f) {
-            if (f == 1) r = rm_reset_stage_times(c);
+            if (f == 1) {
+                r = finish_all();
+                t0 = std::chrono::steady_clock::now();
+            }
             P.sample_offset = f * spp;
             P.total_samples = (f + 1) * spp;
-            if (r == RM_OK) r = rm_frame(c, s, &P, nullptr, 0);
+            if (r == RM_OK) r = rm_frame(c, s, &P, emitters, emitter_vec4s);
         }
-        double ms[RM_STAGE_COUNT];
-        uint32_t launches[RM_STAGE_COUNT];
-        if (r == RM_OK) r = rm_stage_times(c, ms, launches, RM_STAGE_COUNT);
+        if (r == RM_OK) r = finish_all();
         if (r != RM_OK) break;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         res.schedule[k] = schedules[k];
         res.persistent_blocks[k] = blocks[k];
-        res.ms[k] = ms[RM_STAGE_TRACE] / frames;
+        res.waves[k] = waves[k];
+        res.ms[k] = ms / frames;
         res.count = k + 1;
         if (k == 0 || res.ms[k] < best_ms) {
             best_ms = res.ms[k];
             res.best = k;
         }
     }
-    c->timing = timing;
-    if (r != RM_OK) {
-        c->options[RM_OPTION_SCHEDULE] = schedule;
-        c->options[RM_OPTION_PERSISTENT_BLOCKS] = blocks_before;
-        return r;
-    }
-    c->options[RM_OPTION_SCHEDULE] = res.schedule[res.best];
-    c->options[RM_OPTION_PERSISTENT_BLOCKS] = res.persistent_blocks[res.best];
+    const uint32_t pick = res.best;
+    c->options[RM_OPTION_SCHEDULE] = r == RM_OK ? res.schedule[pick] : before[0];
+    c->options[RM_OPTION_PERSISTENT_BLOCKS] = r == RM_OK ? res.persistent_blocks[pick] : before[1];
+    c->options[RM_OPTION_WAVES] = r == RM_OK ? res.waves[pick] : before[2];
+    if (r != RM_OK) return r;
     if (out) *out = res;
     return RM_OK;
 }

@@ -8,10 +8,12 @@ namespace rm {
 // 256 threads: four wave64s covering a 16×16 pixel tile.
 constexpr int BLOCK_THREADS = 256;
 // Traversal stack entries kept in LDS per thread (the rest of the
-// reference's 64 spill to global memory), 8 bytes each: 32 KiB of LDS per
-// block at 16, which still lets four blocks of the path pass share a CU; the
-// gather and raycast kernels hold more waves, so they keep fewer.
-constexpr int MAIN_LDS_DEPTH = 16;
+// reference's 64 spill to global memory), 8 bytes each: 16 KiB of LDS per
+// block of the path pass. Sixteen entries (32 KiB) measured the same on a
+// 200-sphere scene and 1.4% slower on a 2M-triangle one: the LDS that eight
+// leave free lets the side stream's light tracing and gather share CUs with
+// the path pass.
+constexpr int MAIN_LDS_DEPTH = 8;
 constexpr int TRACE_LDS_DEPTH = 8;
 constexpr int GATHER_LDS_DEPTH = 8;
 // Spill entries (8 bytes each) per lane: the rest of the 64 beyond the
@@ -37,9 +39,12 @@ struct LaunchMain {
     uint32_t* counter;           // k_main_persistent's tile counter, zeroed before the launch
     float4* staging;             // non-null: k_main_compact, 3 float4 per thread of its grid
     int tree;                    // the tree the per-pixel pass walks
+    int waves;                   // waves per SIMD the one-thread-per-pixel pass is compiled for: 4 or 5
 };
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
+// The one-thread-per-pixel pass compiled for five waves per SIMD (main5.hip).
+void launch_main_waves5(const LaunchMain& L, const Features& f, hipStream_t stream);
 // The persistent and compacted schedules' launches (launch_main picks them).
 void launch_main_persistent(const LaunchMain& L, const Features& f, hipStream_t stream);
 void launch_main_compact(const LaunchMain& L, const Features& f, hipStream_t stream);

@@ -6,6 +6,7 @@
 // frames by event queries, and copies a finished job's image to pinned host
 // memory. A job's frames are exactly the frames rm_frame renders on a
 // context of its own, so its image does not depend on what else runs.
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -85,7 +86,8 @@ struct rm_server {
     int64_t options[RM_OPTION_COUNT] = {};   // given to jobs submitted from now on
     // The schedule rm_tune picked for each (scene, width, height, samples per
     // frame), with RM_OPTION_AUTOTUNE; used by the scheduler thread only.
-    std::map<std::tuple<const rm_scene*, uint32_t, uint32_t, uint32_t>, std::pair<int32_t, uint32_t>> tuned;
+    // The options rm_tune set: schedule, persistent blocks, waves.
+    std::map<std::tuple<const rm_scene*, uint32_t, uint32_t, uint32_t>, std::array<int64_t, 3>> tuned;
     rm_context* home = nullptr;
     std::vector<Slot> slots;
     std::mutex m;
@@ -178,19 +180,23 @@ bool rm_server::admit() {
             }
         }
         if (r == RM_OK && j->options[RM_OPTION_AUTOTUNE] > 0) {
-            // Every schedule gives the same image, so the choice changes speed only.
+            // Every candidate gives the same image, so the choice changes speed only.
             const auto key = std::make_tuple(j->scene, j->width, j->height, j->spp);
             auto it = tuned.find(key);
             if (it == tuned.end()) {
                 uint32_t p[RM_PARAMS_BYTES / 4];
                 frame_params(*j, 0, p);
                 rm_tune_result res;
-                r = rm_tune(s.ctx, j->scene, p, (uint32_t)j->options[RM_OPTION_AUTOTUNE], &res);
+                r = rm_tune(s.ctx, j->scene, p, j->caustics ? j->emitters.data() : nullptr,
+                            j->caustics ? (uint32_t)(j->emitters.size() / 4) : 0u,
+                            (uint32_t)j->options[RM_OPTION_AUTOTUNE], &res);
                 if (r == RM_OK)
-                    it = tuned.emplace(key, std::make_pair(res.schedule[res.best], res.persistent_blocks[res.best])).first;
+                    it = tuned.emplace(key, std::array<int64_t, 3>{res.schedule[res.best], res.persistent_blocks[res.best],
+                                                                   res.waves[res.best]}).first;
             }
-            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_SCHEDULE, it->second.first);
-            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_PERSISTENT_BLOCKS, it->second.second);
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_SCHEDULE, it->second[0]);
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_PERSISTENT_BLOCKS, it->second[1]);
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_WAVES, it->second[2]);
         }
         if (r != RM_OK) finish(s, r, rm_last_error());
     }

@@ -98,7 +98,7 @@ def test_new_entry_points_check_their_arguments():
     without touching a device."""
     from raymond_amd import capi
     lib = capi.load()
-    assert lib.rm_tune(None, None, None, 2, None) == -1
+    assert lib.rm_tune(None, None, None, None, 0, 2, None) == -1
     assert b"rm_tune" in lib.rm_last_error()
     assert lib.rm_server_set_option(None, 0, 0) == -1
     used = ctypes.c_uint32()

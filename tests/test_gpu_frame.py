@@ -87,24 +87,27 @@ def test_firefly_clamp(ctx, pack):
                                               (CASES[5][0], CASES[5][1], 2)],
                          ids=["spheres", "spheres-3spp", "triangles-fog-env", "dispersive", "textured"])
 def test_schedules_are_bit_identical(ctx, pack, config, opts, spp):
-    """The compacting, lane-regenerating and wavefront schedules compute the
-    same samples, each pixel's in the same order, as one thread per pixel:
-    accumulation and guides equal bit for bit."""
+    """The compacting, lane-regenerating and wavefront schedules, and the
+    one-thread-per-pixel pass compiled for five waves per SIMD, compute the
+    same samples, each pixel's in the same order, as one thread per pixel at
+    four: accumulation and guides equal bit for bit."""
     from raymond_amd import capi
     p = pack(config, **opts)
     params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0, samples_per_frame=spp)
     try:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         plain = gpu_render(ctx, p, 3, params)
-        runs = [(capi.SCHEDULE_COMPACT, 0), (capi.SCHEDULE_PERSISTENT, 0), (capi.SCHEDULE_PERSISTENT, 7),
-                (capi.SCHEDULE_WAVEFRONT, 0)]
-        for schedule, blocks in runs:
+        runs = [(capi.SCHEDULE_PIXEL, 0, 5), (capi.SCHEDULE_COMPACT, 0, 4), (capi.SCHEDULE_PERSISTENT, 0, 4),
+                (capi.SCHEDULE_PERSISTENT, 7, 4), (capi.SCHEDULE_WAVEFRONT, 0, 4)]
+        for schedule, blocks, waves in runs:
             ctx.set_option(capi.OPTION_SCHEDULE, schedule)
             ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, blocks)
+            ctx.set_option(capi.OPTION_WAVES, waves)
             other = gpu_render(ctx, p, 3, params)
             for name, a, b in zip(["accum", "albedo", "normal"], plain, other):
                 differ = int(np.any(a.view(np.uint32) != b.view(np.uint32), axis=-1).sum())
-                assert differ == 0, f"schedule {schedule} (blocks {blocks}): {differ} pixels of {name} differ"
+                assert differ == 0, f"schedule {schedule} (blocks {blocks}, waves {waves}): {differ} pixels of {name} differ"
     finally:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
+        ctx.set_option(capi.OPTION_WAVES, 0)

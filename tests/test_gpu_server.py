@@ -133,14 +133,50 @@ def test_tune_picks_a_schedule_and_keeps_the_image(ctx, pack):
         result = ctx.tune(scene, params, frames=2)
     finally:
         scene.release()
-    names = [c[0] for c in result["candidates"]]
-    assert names == ["per-pixel", "compact", "persistent", "persistent", "wavefront"]
+    names = [c[:3:2] for c in result["candidates"]]
+    assert names == [("per-pixel", 4), ("per-pixel", 5), ("compact", 4), ("persistent", 4), ("persistent", 4),
+                     ("wavefront", 4)]
     assert all(ms > 0 for *_, ms in result["candidates"])
-    best = min(result["candidates"], key=lambda c: c[2])
-    assert result["best"] == best[:2]
+    best = min(result["candidates"], key=lambda c: c[3])
+    assert result["best"] == best[:3]
     try:
         tuned = alone(ctx, p, params, 3)
     finally:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
+        ctx.set_option(capi.OPTION_WAVES, 0)
     assert np.array_equal(tuned.view(np.uint32), plain.view(np.uint32))
+
+
+def test_tune_times_whole_frames_with_light_tracing(ctx, pack):
+    """With light tracing and the denoiser on, the tuner renders whole frames
+    (the emitters it is given included); without the emitters those frames
+    fail."""
+    p = pack("fixture", scene={"name": "causticLamp", "size": 48})
+    params = p.with_params(caustics=1, denoise=1, firefly_clamp=0.0)
+    scene = ctx.upload_scene(p)
+    try:
+        ctx.upload_environment(p)
+        ctx.resize(p.width, p.height)
+        ctx.set_option(capi.OPTION_WAVES, 5)
+        with pytest.raises(capi.RaymondError):
+            ctx.tune(scene, params, frames=1)
+        result = ctx.tune(scene, params, frames=1, emitters=p.emitters)
+        assert len(result["candidates"]) == capi.TUNE_CANDIDATES
+        assert all(ms > 0 for *_, ms in result["candidates"])
+        assert result["best"] == min(result["candidates"], key=lambda c: c[3])[:3]
+        ctx.frame(scene, params, p.emitters)
+        assert np.isfinite(ctx.read(capi.BUFFER_ACCUM)).all()
+    finally:
+        scene.release()
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+        ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
+        ctx.set_option(capi.OPTION_WAVES, 0)
+
+
+def test_waves_option_is_checked(ctx):
+    for bad in (-1, 1, 3, 6):
+        with pytest.raises(capi.RaymondError):
+            ctx.set_option(capi.OPTION_WAVES, bad)
+    for good in (0, 4, 5, 0):
+        ctx.set_option(capi.OPTION_WAVES, good)
