@@ -34,6 +34,9 @@ export function precisionFlags({ quantizedBvh = false, fp16Guides = false, compe
     return (quantizedBvh ? 1 : 0) | (fp16Guides ? 2 : 0) | (compensated ? 4 : 0);
 }
 
+/** RM_OPTION_WAVES: waves per SIMD of the one-thread-per-pixel path pass, 4 or 5. */
+export const OPTION_WAVES = 5;
+
 /** RM_OPTION_SCHEDULE and its values. */
 export const OPTION_SCHEDULE = 0;
 export const SCHEDULES = Object.freeze({ "per-pixel": 0, compact: 1, persistent: 2, wavefront: 3 });

@@ -4,7 +4,7 @@
 // the frame itself (trace, light tracing, photon gather, denoise) runs as HIP
 // kernels in libraymond_hip.so, reached through the N-API addon raymond.node.
 // There is no other backend: without the addon or a GPU, create() rejects.
-import { OPTION_PRECISION, OPTION_SCHEDULE, SCHEDULES, loadAddon, precisionFlags } from "./addon.mjs";
+import { OPTION_PRECISION, OPTION_SCHEDULE, OPTION_WAVES, SCHEDULES, loadAddon, precisionFlags } from "./addon.mjs";
 import {
     IMAGE_KEY_BYTES, PARAMS_BYTES, VIEWS, deltaObjects, emitters, environmentData,
     sceneTables, textureLayers, writeFrameParams, writeParams,
@@ -48,7 +48,7 @@ export class Renderer {
     #addon; #ctx; #canvas;
     #scene = null; #camera = null;
     #maxBounces = 8; #samplesPerFrame = 1; #exposure = 1.0; #fireflyClamp = 20.0;
-    #view = "color"; #denoise = true; #caustics = true; #precision = {}; #schedule = "per-pixel";
+    #view = "color"; #denoise = true; #caustics = true; #precision = {}; #schedule = "per-pixel"; #waves = 4;
     #causticsActive = false;
     #samples = 0;
     #params = new ArrayBuffer(PARAMS_BYTES);
@@ -120,6 +120,18 @@ export class Renderer {
             throw new RangeError(`schedule is one of ${Object.keys(SCHEDULES).join(", ")}.`);
         this.#addon.setOption(this.#ctx, OPTION_SCHEDULE, SCHEDULES[v]);
         this.#schedule = v;
+    }
+    /**
+     * Waves per SIMD the "per-pixel" path pass is compiled for: 4 (default)
+     * or 5, the same image bit for bit. Five is faster on scenes whose frame
+     * the path pass dominates (large meshes, glass).
+     */
+    get waves() { return this.#waves; }
+    set waves(v) {
+        if (v !== 4 && v !== 5)
+            throw new RangeError("waves is 4 or 5.");
+        this.#addon.setOption(this.#ctx, OPTION_WAVES, v);
+        this.#waves = v;
     }
     get precision() { return Object.assign({}, this.#precision); }
     set precision(v) {

@@ -35,7 +35,12 @@ async function main() {
     let rejected = false;
     try { renderer.schedule = "diagonal"; } catch (e) { rejected = e instanceof RangeError; }
     if (!rejected) throw new Error("an unknown schedule should be a RangeError");
+    rejected = false;
+    try { renderer.waves = 6; } catch (e) { rejected = e instanceof RangeError; }
+    if (!rejected) throw new Error("waves other than 4 or 5 should be a RangeError");
+    renderer.waves = 5;
     renderer.view = "color";
+    renderer.render();
     renderer.schedule = "wavefront";
     renderer.precision = { quantizedBvh: true, fp16Guides: true, compensated: true };
     renderer.render();
