@@ -53,7 +53,7 @@ extern "C" {
 #define RM_OPTION_PERSISTENT_BLOCKS 1  /* blocks of 256 threads for RM_SCHEDULE_PERSISTENT; 0 (default): fill the GPU */
 #define RM_OPTION_PRECISION 2          /* RM_PRECISION_* flags; 0 (default): the plain FP32 path */
 #define RM_OPTION_AUTOTUNE 3           /* job servers: frames per candidate with which rm_tune picks the schedule
-                                          for each new scene and image size; 0 (default): no tuning */
+                                          and waves for each new scene and image size; 0 (default): no tuning */
 #define RM_OPTION_BATCH 4              /* job servers: 1 gives the frames of the jobs running at once one
                                           path-pass launch (rm_frame_batch); 0 (default): a launch per job */
 #define RM_OPTION_WAVES 5              /* waves per SIMD that RM_SCHEDULE_PIXEL's path pass is compiled for:
