@@ -322,15 +322,16 @@ RM_D float4 gather_pixel(const Ctx& C, const uint32_t* heads, const float4* phot
     for (uint32_t depth = 0u; depth < P.max_bounces; depth++) {
         Hit hit = trace<false>(S, o, d, INF, stack, nc);
         if (hit.prim == NONE) break;
+        // The hit primitive's material alone decides whether the path ends
+        // here (a light, or any surface but a delta one at the first hit),
+        // so most camera rays stop before their shading point is built.
+        Mat m = load_mat(S, __float_as_uint(S.prims[(size_t)hit.prim * 4].y));
+        if (m.kind == LIGHT || (depth == 0u && !is_delta(m))) break;
         Surface s = surface<TEXTURES>(S, o, d, hit);
-        Mat m = load_mat(S, s.material);
-        if (m.kind == LIGHT) break;
         V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
         if (!is_delta(m)) {
-            if (depth > 0u) {
-                V3 light = throughput * gather(C, heads, photons, s, m, base, -d);
-                if (finite3(light)) return make_float4(light.x, light.y, light.z, 1.0f);
-            }
+            V3 light = throughput * gather(C, heads, photons, s, m, base, -d);
+            if (finite3(light)) return make_float4(light.x, light.y, light.z, 1.0f);
             break;
         }
         BsdfSample bs = sample_bsdf(g, m, base, s, -d);
