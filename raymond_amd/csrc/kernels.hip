@@ -202,9 +202,9 @@ RM_D float4 atrous_pixel(int x, int y, int W, int H, int step, float4 centre, fl
     return make_float4(r.x, r.y, r.z, sum_variance / (weight * weight));
 }
 
-// An à-trous pass reading its taps straight from the buffers: a wave's tap
-// is one coalesced load, and at wide steps the neighbouring workgroups of an
-// XCD read the same lines.
+// An à-trous pass reading its taps straight from the buffers, for steps 8
+// and 16: a wave's tap is one coalesced load, and the neighbouring
+// workgroups of an XCD read the same lines.
 template <class G>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const typename G::T* __restrict__ guides,
                                                           const float4* __restrict__ light_in,
@@ -223,54 +223,66 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, co
     light_out[i] = atrous_pixel(x, y, (int)P.width, (int)P.height, step, light_in[i], G::get(guides, i), tap, var);
 }
 
-// The staged pass, for the narrow steps. At step s the taps of pixel (x, y)
-// are the pixels x + s·i, y + s·j (|i|, |j| <= 2), all in the same phase
-// (x mod s, y mod s), where the pass is a dense 5×5 filter. A workgroup takes
-// a 16×16 tile of one phase, stages it with its 2-pixel apron (20×20 pixels)
-// in LDS once and reads every tap from there: 400 global loads of each
-// buffer per 256 pixels rather than 25 per pixel. Consecutive logical blocks
-// are neighbouring phases of one tile, whose pixels share cache lines, and
-// each XCD (workgroups go to the 8 XCDs round-robin) takes one contiguous run
-// of them, so those lines are fetched into one L2. At 1920×1080 this takes
-// 61, 77 and 87 µs at steps 1, 2 and 4 against 106, 101 and 90 µs for the
-// direct pass; at steps 8 and 16, where a staged tile's pixels each sit on
-// a cache line of their own, 118 and 137 µs against 80 and 81.
-constexpr int ATROUS_TILE = 16;
-constexpr int ATROUS_SPAN = ATROUS_TILE + 4;
-constexpr int ATROUS_STAGED_MAX_STEP = 4;
+// The staged à-trous pass, for steps 1, 2 and 4. At step s the taps of
+// pixel (x, y) are the pixels x + s·i, y + s·j (|i|, |j| <= 2), all in the
+// same phase (x mod s, y mod s), where the pass is a dense 5×5 filter. A
+// workgroup takes PG = s adjacent x-phases of one y-phase, a TU×TV tile of
+// each, stages them with their 2-pixel apron in LDS once and reads every tap
+// from there: 1.6–2.3 staged loads of each buffer per pixel rather than 25.
+// A staged row holds PG neighbouring pixels, so its loads stay coalesced.
+// Consecutive logical blocks are the phase groups of one tile, and each XCD
+// (workgroups go to the 8 XCDs round-robin) takes one contiguous run of them.
+// At 1920×1080, alone, steps 1, 2 and 4 take 60, 67 and 68 µs against 106,
+// 101 and 90 µs for the direct pass. Staged with eight phases a workgroup,
+// steps 8 and 16 take 69 and 72 µs against 80 and 81 alone, but need 40 VGPRs
+// and 24 KB of LDS, and beside the next frame's path pass they made the frame
+// 1% slower, so those steps stay direct.
+template <int PG> struct AtrousShape;
+template <> struct AtrousShape<1> { static constexpr int TU = 16, TV = 16; };
+template <> struct AtrousShape<2> { static constexpr int TU = 8, TV = 16; };
+template <> struct AtrousShape<4> { static constexpr int TU = 8, TV = 8; };
+
 struct AtrousGrid {
+    uint32_t groups;    // phase groups: s / PG x-groups × s y-phases
     uint32_t tiles_u;   // tiles across one phase
     uint32_t tiles;     // tiles per phase
-    uint32_t blocks;    // logical blocks: phases × tiles
+    uint32_t blocks;    // logical blocks: groups × tiles
     uint32_t launched;  // blocks rounded up to a multiple of the XCD count
 };
+template <int PG>
 inline AtrousGrid atrous_grid(uint32_t width, uint32_t height, uint32_t step) {
+    using A = AtrousShape<PG>;
     const uint32_t u = (width + step - 1u) / step, v = (height + step - 1u) / step;
     AtrousGrid g;
-    g.tiles_u = (u + ATROUS_TILE - 1u) / ATROUS_TILE;
-    g.tiles = g.tiles_u * ((v + ATROUS_TILE - 1u) / ATROUS_TILE);
-    g.blocks = step * step * g.tiles;
+    g.groups = step / PG * step;
+    g.tiles_u = (u + A::TU - 1u) / A::TU;
+    g.tiles = g.tiles_u * ((v + A::TV - 1u) / A::TV);
+    g.blocks = g.groups * g.tiles;
     g.launched = (g.blocks + XCDS - 1u) / XCDS * XCDS;
     return g;
 }
 
-template <class G>
+template <class G, int PG>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_staged(Params P, int step, AtrousGrid grid,
                                                                  const typename G::T* __restrict__ guides,
                                                                  const float4* __restrict__ light_in,
                                                                  float4* __restrict__ light_out) {
-    constexpr int SPAN = ATROUS_SPAN;
-    __shared__ float4 tile_light[SPAN * SPAN];
-    __shared__ float4 tile_guide[SPAN * SPAN];
+    constexpr int TU = AtrousShape<PG>::TU, TV = AtrousShape<PG>::TV;
+    static_assert(PG * TU * TV == BLOCK_THREADS, "a thread per staged output");
+    constexpr int SU = TU + 4, SV = TV + 4, N = PG * SU * SV;
+    __shared__ float4 tile_light[N];
+    __shared__ float4 tile_guide[N];
     const uint32_t logical = (blockIdx.x % XCDS) * (grid.launched / XCDS) + blockIdx.x / XCDS;
     if (logical >= grid.blocks) return;
     const int s = step;
-    const uint32_t phase = logical % (uint32_t)(s * s), t = logical / (uint32_t)(s * s);
-    const int px = (int)(phase % (uint32_t)s), py = (int)(phase / (uint32_t)s);
-    const int u0 = (int)(t % grid.tiles_u) * ATROUS_TILE, v0 = (int)(t / grid.tiles_u) * ATROUS_TILE;
+    const uint32_t group = logical % grid.groups, t = logical / grid.groups;
+    const int xgroups = s / PG;
+    const int gx = (int)(group % (uint32_t)xgroups) * PG, py = (int)(group / (uint32_t)xgroups);
+    const int u0 = (int)(t % grid.tiles_u) * TU, v0 = (int)(t / grid.tiles_u) * TV;
     const int W = (int)P.width, H = (int)P.height;
-    for (int e = (int)threadIdx.x; e < SPAN * SPAN; e += BLOCK_THREADS) {
-        const int xx = px + s * (u0 + e % SPAN - 2), yy = py + s * (v0 + e / SPAN - 2);
+    for (int e = (int)threadIdx.x; e < N; e += BLOCK_THREADS) {
+        const int a = e % PG, cu = (e / PG) % SU, rv = e / (PG * SU);
+        const int xx = gx + a + s * (u0 + cu - 2), yy = py + s * (v0 + rv - 2);
         if (xx >= 0 && yy >= 0 && xx < W && yy < H) {
             const uint32_t j = (uint32_t)yy * P.width + (uint32_t)xx;
             tile_light[e] = light_in[j];
@@ -278,17 +290,18 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_staged(Params P, int s
         }
     }
     __syncthreads();
-    const int tx = (int)threadIdx.x % ATROUS_TILE, ty = (int)threadIdx.x / ATROUS_TILE;
-    const int x = px + s * (u0 + tx), y = py + s * (v0 + ty);
+    const int th = (int)threadIdx.x;
+    const int a = th % PG, tu = (th / PG) % TU, tv = th / (PG * TU);
+    const int x = gx + a + s * (u0 + tu), y = py + s * (v0 + tv);
     if (x >= W || y >= H) return;
-    const int c = (ty + 2) * SPAN + tx + 2;
+    const int c = a + PG * ((tu + 2) + SU * (tv + 2));
     auto tap = [&](int dx, int dy, float4& q, float4& g) {
-        q = tile_light[c + dy * SPAN + dx];
-        g = tile_guide[c + dy * SPAN + dx];
+        q = tile_light[c + PG * (dx + SU * dy)];
+        g = tile_guide[c + PG * (dx + SU * dy)];
     };
     // The variance neighbourhood is in the tile only at step 1.
     auto var = [&](int dx, int dy) {
-        return s == 1 ? tile_light[c + dy * SPAN + dx].w
+        return s == 1 ? tile_light[c + dx + SU * dy].w
                       : light_in[(uint32_t)(y + dy) * P.width + (uint32_t)(x + dx)].w;
     };
     light_out[(uint32_t)y * P.width + (uint32_t)x] = atrous_pixel(x, y, W, H, s, tile_light[c], tile_guide[c], tap, var);
@@ -390,25 +403,33 @@ void launch_prepare(const Params& P, const float4* accum, const float4* galb, co
                            static_cast<const float4*>(guides), out, albedo_out);
 }
 
+template <int PG>
+static void launch_atrous_pg(const Params& P, int step, const void* guides, bool fp16, const float4* in,
+                                 float4* out, hipStream_t stream) {
+    const AtrousGrid g = atrous_grid<PG>(P.width, P.height, (uint32_t)step);
+    if (fp16)
+        hipLaunchKernelGGL((k_atrous_staged<GuidesF16, PG>), dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, step,
+                           g, static_cast<const ushort4*>(guides), in, out);
+    else
+        hipLaunchKernelGGL((k_atrous_staged<GuidesF32, PG>), dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, step,
+                           g, static_cast<const float4*>(guides), in, out);
+}
+
 void launch_atrous(const Params& P, int step, const void* guides, bool fp16, const float4* in, float4* out,
                    hipStream_t stream) {
-    if (step <= ATROUS_STAGED_MAX_STEP) {
-        const AtrousGrid g = atrous_grid(P.width, P.height, (uint32_t)step);
+    if (step > 4) {
+        dim3 grid(stencil_blocks(P.width, P.height));
         if (fp16)
-            hipLaunchKernelGGL(k_atrous_staged<GuidesF16>, dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, step,
-                               g, static_cast<const ushort4*>(guides), in, out);
+            hipLaunchKernelGGL(k_atrous<GuidesF16>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
+                               static_cast<const ushort4*>(guides), in, out);
         else
-            hipLaunchKernelGGL(k_atrous_staged<GuidesF32>, dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, step,
-                               g, static_cast<const float4*>(guides), in, out);
+            hipLaunchKernelGGL(k_atrous<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
+                               static_cast<const float4*>(guides), in, out);
         return;
     }
-    dim3 grid(stencil_blocks(P.width, P.height));
-    if (fp16)
-        hipLaunchKernelGGL(k_atrous<GuidesF16>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
-                           static_cast<const ushort4*>(guides), in, out);
-    else
-        hipLaunchKernelGGL(k_atrous<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
-                           static_cast<const float4*>(guides), in, out);
+    if (step % 4 == 0) return launch_atrous_pg<4>(P, step, guides, fp16, in, out, stream);
+    if (step % 2 == 0) return launch_atrous_pg<2>(P, step, guides, fp16, in, out, stream);
+    launch_atrous_pg<1>(P, step, guides, fp16, in, out, stream);
 }
 
 void launch_display(const Params& P, const float4* accum, const float4* denoised, uchar4* out, hipStream_t stream) {

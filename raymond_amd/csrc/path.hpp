@@ -19,7 +19,7 @@ RM_D void tile_pixel_of(uint32_t t, uint32_t& x, uint32_t& y) {
 }
 RM_D void tile_pixel(uint32_t& x, uint32_t& y) { tile_pixel_of(threadIdx.x, x, y); }
 
-// Stencil passes (the denoiser) read a neighbourhood up to 32 pixels away.
+// Stencil passes (the denoiser's prepare) read a neighbourhood of pixels.
 // Workgroups are dealt to the 8 XCDs round-robin by linear block id, and each
 // XCD has its own L2, so a plain row-major grid makes every L2 fetch nearly
 // the whole image. Instead the 16×16 tiles are ordered in vertical strips
