@@ -404,6 +404,7 @@ def bench_serving(args) -> dict:
     flags = precision_flags(args.served_precision)
     served = run_trace(packs, trace, args.slots, budget)
     served_batched = run_trace(packs, trace, args.slots, budget, batch=True)
+    served_tuned = run_trace(packs, trace, args.slots, budget, autotune=1)
     served_described = run_trace(packs, trace, args.slots, budget, flags, batch=True)
     one_slot = run_trace(packs, trace, 1, budget)
     plain = run_plain(packs, trace)
@@ -430,10 +431,12 @@ def bench_serving(args) -> dict:
                    "memory_budget_bytes": budget, "device": device, "compute_units": cus,
                    "served_precision": args.served_precision,
                    "path": "served: concurrent slots, FP32, a job's first and last frames denoised (also reported: "
-                           "the running jobs' path passes batched into one launch, and batched with the precision "
-                           "variants named); plain: one context, one job at a time, FP32, every frame denoised"},
+                           "the running jobs' path passes batched into one launch, autotuned per scene and size, and "
+                           "batched with the precision variants named); plain: one context, one job at a time, FP32, "
+                           "every frame denoised"},
         "served": served,
         "served_batched": served_batched,
+        "served_tuned": served_tuned,
         "served_batched_with_precision_variants": served_described,
         "served_one_slot": one_slot,
         "plain": plain,
