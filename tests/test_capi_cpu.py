@@ -5,6 +5,8 @@ import re
 import subprocess
 from pathlib import Path
 
+import pytest
+
 ROOT = Path(__file__).resolve().parent.parent
 
 
@@ -104,3 +106,50 @@ def test_new_entry_points_check_their_arguments():
     used = ctypes.c_uint32()
     assert lib.rm_bvh_quantize_wide(None, 0, None, 0, ctypes.byref(used)) == -1
     assert lib.rm_set_option(None, capi.OPTION_AUTOTUNE, 1) == -1
+
+
+def test_python_and_js_constants_match_the_header(tmp_path):
+    """capi.py's and addon.mjs's option, schedule, precision, buffer and stage
+    numbers are the header's, and the ctypes tuner result has the C layout
+    (sizes and offsets printed by a C program built against raymond.h)."""
+    import re
+    import shutil
+    from raymond_amd import capi
+    root = Path(__file__).resolve().parent.parent
+    header = (root / "include" / "raymond.h").read_text()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define (RM_\w+) \(?(-?\d+)\)?", header)}
+    py = {f"RM_OPTION_{n}": getattr(capi, f"OPTION_{n}")
+          for n in ("SCHEDULE", "PERSISTENT_BLOCKS", "PRECISION", "AUTOTUNE", "BATCH", "WAVES")}
+    py.update({f"RM_SCHEDULE_{n}": getattr(capi, f"SCHEDULE_{n}") for n in ("PIXEL", "COMPACT", "PERSISTENT",
+                                                                            "WAVEFRONT")})
+    py.update({f"RM_PRECISION_{n}": getattr(capi, f"PRECISION_{n}") for n in ("QUANTIZED_BVH", "FP16_GUIDES",
+                                                                              "COMPENSATED")})
+    py.update({f"RM_BUFFER_{n}": getattr(capi, f"BUFFER_{n}") for n in ("ACCUM", "GUIDE_ALBEDO", "GUIDE_NORMAL",
+                                                                        "DENOISED")})
+    py.update({f"RM_STAGE_{s.upper()}": k for k, s in enumerate(capi.STAGES)})
+    py["RM_STAGE_COUNT"] = len(capi.STAGES)
+    py["RM_TUNE_CANDIDATES"] = capi.TUNE_CANDIDATES
+    assert {k: defs[k] for k in py} == py
+    assert max(v for k, v in defs.items() if k.startswith("RM_OPTION_") and k != "RM_OPTION_COUNT") + 1 == \
+        defs["RM_OPTION_COUNT"]
+    js = (root / "raymond_amd" / "js" / "addon.mjs").read_text()
+    for name in ("OPTION_PRECISION", "OPTION_SCHEDULE", "OPTION_WAVES"):
+        assert int(re.search(rf"export const {name} = (\d+);", js).group(1)) == defs["RM_" + name]
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    structs = {"rm_tune_result": capi._TuneResult, "rm_job": capi._Job, "rm_job_stats": capi._JobStats}
+    body = "".join(f'printf("{c} %zu\\n", sizeof({c}));' +
+                   "".join(f'printf("{c}.{f} %zu\\n", offsetof({c}, {f}));' for f, *_ in t._fields_)
+                   for c, t in structs.items())
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include <raymond.h>\n"
+                   f"int main(void) {{ {body} return 0; }}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", f"-I{root / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    want = {}
+    for c, t in structs.items():
+        want[c] = str(ctypes.sizeof(t))
+        want.update({f"{c}.{f}": str(getattr(t, f).offset) for f, *_ in t._fields_})
+    assert got == want
