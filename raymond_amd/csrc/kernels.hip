@@ -150,8 +150,15 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
 // (step 1) neighbourhood's variance. tap(dx, dy) gives the light and guide
 // of the pixel (dx, dy) taps away, and var(dx, dy) the variance of the pixel
 // (dx, dy) away; both are asked only for pixels inside the image.
+//
+// Unlike the path pass, the denoiser is held to a relative tolerance, not
+// bit for bit, so its multiply-adds are fused and each weight's exponential
+// is the hardware exp2 (weights below 2^-126 become 0): with the taps staged
+// in LDS the passes are bound by arithmetic, and this makes them 8% faster
+// and the frame 1.2%.
 template <class TAP, class VAR>
 RM_D float4 atrous_pixel(int x, int y, int W, int H, int step, float4 centre, float4 gp, TAP&& tap, VAR&& var) {
+#pragma clang fp contract(fast)
     const float lp = luminance(v3(centre.x, centre.y, centre.z));
     float variance = 0.0f, variance_weight = 0.0f;
     for (int dy = -1; dy <= 1; dy++)
@@ -191,7 +198,7 @@ RM_D float4 atrous_pixel(int x, int y, int W, int H, int step, float4 centre, fl
                 sim = pow64(fmaxf(gp.x * g.x + gp.y * g.y + gp.z * g.z, 0.0f));
                 e -= fabsf(gp.w - g.w) * k_z;
             }
-            float w = kernel_w(abs(dx)) * kernel_w(abs(dy)) * sim * exp2f(e);
+            float w = kernel_w(abs(dx)) * kernel_w(abs(dy)) * sim * __builtin_amdgcn_exp2f(e);
             sum = sum + w * v3(q.x, q.y, q.z);
             sum_variance += w * w * q.w;
             weight += w;
