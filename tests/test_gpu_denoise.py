@@ -19,11 +19,15 @@ ROOM = ("fixture", {"scene": {"name": "room", "size": 96}})
 
 @pytest.mark.parametrize("frames", [1, 3, 8], ids=["spatial1", "spatial3", "temporal8"])
 # 101×67: no side a multiple of the passes' 16-pixel tiles or of their steps,
-# so every pass has partial tiles and phases of unequal size.
+# so every pass has partial tiles and phases of unequal size; 17×5 and 3×2:
+# narrower than the wide steps, so whole phases and workgroups hold no pixel.
 @pytest.mark.parametrize("config,opts", [ROOM, ("spheres", {"scene": {"width": 128, "height": 72}, "width": 128,
                                                                "height": 72}),
                                          ("spheres", {"scene": {"width": 101, "height": 67}, "width": 101,
-                                                      "height": 67})], ids=["room", "spheres", "spheres101x67"])
+                                                      "height": 67}),
+                                         ("spheres", {"scene": {"width": 17, "height": 5}, "width": 17, "height": 5}),
+                                         ("spheres", {"scene": {"width": 3, "height": 2}, "width": 3, "height": 2})],
+                         ids=["room", "spheres", "spheres101x67", "spheres17x5", "spheres3x2"])
 def test_denoiser_matches_oracle(ctx, pack, config, opts, frames):
     from helpers import frame_params
     p = pack(config, **opts)
