@@ -128,10 +128,6 @@ int rmi::current_sums(rm_context* c, hipStream_t stream, const float4* out[3]) {
     return RM_OK;
 }
 
-namespace {
-
-}  // namespace
-
 // The tree traversals walk: the FP32 nodes, or when quantized nodes are
 // asked for, the 4-wide quantized tree, else the binary quantized one, else
 // (a tree that can't be quantized) the FP32 nodes.
@@ -587,8 +583,11 @@ static int frame_end(rm_context* c, Frame& F) {
     const bool caustics = F.caustics;
     float4* comp = F.comp;
 
-    if (caustics) {
-        RM_HIP(hipStreamWaitEvent(c->stream, c->join, 0), "joining the side stream");
+    // From the fourth sample on, with plain sums, resolve, guides and prepare
+    // run as one pass (k_finish) timed as the prepare stage.
+    const bool fused = P.denoise != 0u && comp == nullptr && P.total_samples >= 4u;
+    if (caustics) RM_HIP(hipStreamWaitEvent(c->stream, c->join, 0), "joining the side stream");
+    if (caustics && !fused) {
         {
             Timed t(c, RM_STAGE_RESOLVE);
             rm::launch_resolve(P, c->splats.as<uint4>(), c->gathered.as<const float4>(), c->accum.as<float4>(), comp,
@@ -604,7 +603,12 @@ static int frame_end(rm_context* c, Frame& F) {
         RM_HIP(c->albedo_frame.ensure((size_t)c->width * c->height * 16), "allocating the denoiser's albedo");
         // The previous frame's passes still read light_a/b, guides and the albedo.
         if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "ordering the denoiser");
-        {
+        if (fused) {
+            Timed t(c, RM_STAGE_PREPARE);
+            rm::launch_finish(P, caustics ? c->splats.as<uint4>() : nullptr, c->gathered.as<const float4>(),
+                              c->accum.as<float4>(), c->galb.as<const float4>(), c->gnorm.as<const float4>(),
+                              c->guides.ptr, fp16, a, c->albedo_frame.as<float4>(), c->stream);
+        } else {
             Timed t(c, RM_STAGE_PREPARE);
             const float4* sums[3];
             int r = rmi::current_sums(c, c->stream, sums);

@@ -98,6 +98,15 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_guides(Params P, const float4
     guides[i] = G::put(normal_at(gnorm, i, (float)max(P.total_samples, 1u)));
 }
 
+// Prepare's variance once 4 or more samples have accumulated: from the
+// pixel's own luminance moments, demodulated by its albedo.
+RM_D float temporal_variance(float4 ac, V3 own_albedo, float n) {
+    float a = luminance(own_albedo);
+    float mean = luminance(v3(ac.x, ac.y, ac.z)) / n;
+    float spread = fmaxf(ac.w / n - mean * mean, 0.0f) * n / (n - 1.0f);
+    return spread / n / (a * a);
+}
+
 // src/shaders.mjs:1974-2021. The pixel's albedo (albedo_at) also goes to
 // albedo_out for k_compose, so that the passes after this one no longer read
 // the accumulation buffers the next frame writes.
@@ -115,12 +124,9 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
     albedo_out[i] = make_float4(own_albedo.x, own_albedo.y, own_albedo.z, 0.0f);
     V3 light = (v3(ac.x, ac.y, ac.z) / n) / own_albedo;
     float4 gp = G::get(guides, i);
-    float a = luminance(own_albedo);
     float variance;
     if (n >= 4.0f) {
-        float mean = luminance(v3(ac.x, ac.y, ac.z)) / n;
-        float spread = fmaxf(ac.w / n - mean * mean, 0.0f) * n / (n - 1.0f);
-        variance = spread / n / (a * a);
+        variance = temporal_variance(ac, own_albedo, n);
     } else {
         float sum = 0.0f, sum2 = 0.0f, weight = 0.0f;
         for (int dy = -2; dy <= 2; dy++)
@@ -139,6 +145,33 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_prepare(Params P, const float
         variance = fmaxf(sum2 / fmaxf(weight, 1e-6f) - mean * mean, 0.0f) + 0.01f * mean * mean;
     }
     light_out[i] = make_float4(light.x, light.y, light.z, variance);
+}
+
+// The path stream's work between two path passes once 4 or more samples
+// have accumulated, when prepare reads only its own pixel: the splat and
+// gather resolve (RESOLVE, with light tracing), the pixel's guide and
+// prepare, in one pass over the pixels. The same arithmetic as k_resolve,
+// k_guides and k_prepare, with each buffer read once and two launches fewer.
+template <class G, bool RESOLVE>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_finish(Params P, uint4* splats, const float4* __restrict__ gathered,
+                                                          float4* __restrict__ accum, const float4* __restrict__ galb,
+                                                          const float4* __restrict__ gnorm,
+                                                          typename G::T* __restrict__ guides,
+                                                          float4* __restrict__ light_out,
+                                                          float4* __restrict__ albedo_out) {
+    const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    if (i >= P.width * P.height) return;
+    const float n = (float)max(P.total_samples, 1u);
+    float4 ac = accum[i];
+    if (RESOLVE) {
+        ac = resolve_add(P, splats, gathered, nullptr, i, ac);
+        accum[i] = ac;
+    }
+    guides[i] = G::put(normal_at(gnorm, i, n));
+    const V3 own_albedo = albedo_at(galb, i, n);
+    albedo_out[i] = make_float4(own_albedo.x, own_albedo.y, own_albedo.z, 0.0f);
+    const V3 light = (v3(ac.x, ac.y, ac.z) / n) / own_albedo;
+    light_out[i] = make_float4(light.x, light.y, light.z, temporal_variance(ac, own_albedo, n));
 }
 
 // src/shaders.mjs:2023-2086: one pixel of an à-trous pass at `step`. The 25
@@ -397,6 +430,30 @@ void launch_guides(const Params& P, const float4* gnorm, void* guides, bool fp16
     else
         hipLaunchKernelGGL(k_guides<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, gnorm,
                            static_cast<float4*>(guides));
+}
+
+template <class G>
+static void launch_finish_g(const Params& P, uint4* splats, const float4* gathered, float4* accum,
+                            const float4* galb, const float4* gnorm, void* guides, float4* out, float4* albedo_out,
+                            dim3 grid, hipStream_t stream) {
+    auto* g = static_cast<typename G::T*>(guides);
+    if (splats)
+        hipLaunchKernelGGL((k_finish<G, true>), grid, dim3(BLOCK_THREADS), 0, stream, P, splats, gathered, accum,
+                           galb, gnorm, g, out, albedo_out);
+    else
+        hipLaunchKernelGGL((k_finish<G, false>), grid, dim3(BLOCK_THREADS), 0, stream, P, splats, gathered, accum,
+                           galb, gnorm, g, out, albedo_out);
+}
+
+void launch_finish(const Params& P, uint4* splats, const float4* gathered, float4* accum, const float4* galb,
+                   const float4* gnorm, void* guides, bool fp16, float4* out, float4* albedo_out,
+                   hipStream_t stream) {
+    uint32_t n = P.width * P.height;
+    dim3 grid((n + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    if (fp16)
+        launch_finish_g<GuidesF16>(P, splats, gathered, accum, galb, gnorm, guides, out, albedo_out, grid, stream);
+    else
+        launch_finish_g<GuidesF32>(P, splats, gathered, accum, galb, gnorm, guides, out, albedo_out, grid, stream);
 }
 
 void launch_prepare(const Params& P, const float4* accum, const float4* galb, const void* guides, bool fp16,

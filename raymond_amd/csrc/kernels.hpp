@@ -124,6 +124,11 @@ void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* he
                           float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream);
 // fp16: the guide buffer holds FP16 (8 bytes a pixel) rather than FP32.
 void launch_guides(const Params& P, const float4* gnorm, void* guides, bool fp16, hipStream_t stream);
+// Resolve (when splats is non-null), guides and prepare in one pass, for a
+// frame with 4 or more samples accumulated and plain (not compensated) sums.
+void launch_finish(const Params& P, uint4* splats, const float4* gathered, float4* accum, const float4* galb,
+                   const float4* gnorm, void* guides, bool fp16, float4* out, float4* albedo_out,
+                   hipStream_t stream);
 void launch_prepare(const Params& P, const float4* accum, const float4* galb, const void* guides, bool fp16,
                     float4* out, float4* albedo_out, hipStream_t stream);
 void launch_atrous(const Params& P, int step, const void* guides, bool fp16, const float4* in, float4* out,

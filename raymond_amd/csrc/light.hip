@@ -226,30 +226,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* spla
                                                            float4* accum, float4* comp) {
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     if (i >= P.width * P.height) return;
-    uint4 s = splats[i];
-    splats[i] = make_uint4(0u, 0u, 0u, s.w);
-    V3 light = v3((float)s.x, (float)s.y, (float)s.z) / SPLAT_SCALE;
-    float l = luminance(light);
-    float spp = (float)P.samples_per_frame;
-    const float4 splat = make_float4(light.x * spp, light.y * spp, light.z * spp, l * l * spp);
-    float4 g = gathered[i];
-    float4 gather = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (g.w != 0.0f) {
-        V3 gl = v3(g.x, g.y, g.z);
-        float lg = luminance(gl);
-        gather = make_float4(gl.x * spp, gl.y * spp, gl.z * spp, lg * lg * spp);
-    }
-    float4 a = accum[i];
-    if (comp) {
-        float4 e = comp[(size_t)i * 3];
-        compensated_add(a, e, splat);
-        if (g.w != 0.0f) compensated_add(a, e, gather);
-        comp[(size_t)i * 3] = e;
-    } else {
-        a = make_float4(a.x + splat.x, a.y + splat.y, a.z + splat.z, a.w + splat.w);
-        if (g.w != 0.0f) a = make_float4(a.x + gather.x, a.y + gather.y, a.z + gather.z, a.w + gather.w);
-    }
-    accum[i] = a;
+    accum[i] = resolve_add(P, splats, gathered, comp, i, accum[i]);
 }
 
 // The compensated sums' best FP32 value, sum + error, for their readers

@@ -308,6 +308,36 @@ RM_D void compensated_add(float4& sum_ref, float4& err_ref, float4 v) {
     err_ref = e;
 }
 
+// src/shaders.mjs resolve-then-gather order: pixel i's splats (fixed point,
+// cleared for the next frame) and then its gathered light, when it has any,
+// added to its sum a, with the error term comp[3i] when the sums are
+// compensated. Returns the new sum.
+RM_D float4 resolve_add(const Params& P, uint4* splats, const float4* gathered, float4* comp, uint32_t i, float4 a) {
+    uint4 s = splats[i];
+    splats[i] = make_uint4(0u, 0u, 0u, s.w);
+    V3 light = v3((float)s.x, (float)s.y, (float)s.z) / SPLAT_SCALE;
+    float l = luminance(light);
+    float spp = (float)P.samples_per_frame;
+    const float4 splat = make_float4(light.x * spp, light.y * spp, light.z * spp, l * l * spp);
+    float4 g = gathered[i];
+    float4 gather = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (g.w != 0.0f) {
+        V3 gl = v3(g.x, g.y, g.z);
+        float lg = luminance(gl);
+        gather = make_float4(gl.x * spp, gl.y * spp, gl.z * spp, lg * lg * spp);
+    }
+    if (comp) {
+        float4 e = comp[(size_t)i * 3];
+        compensated_add(a, e, splat);
+        if (g.w != 0.0f) compensated_add(a, e, gather);
+        comp[(size_t)i * 3] = e;
+    } else {
+        a = make_float4(a.x + splat.x, a.y + splat.y, a.z + splat.z, a.w + splat.w);
+        if (g.w != 0.0f) a = make_float4(a.x + gather.x, a.y + gather.y, a.z + gather.z, a.w + gather.w);
+    }
+    return a;
+}
+
 struct PixelSums {
     V3 c, albedo;
     float w;

@@ -48,3 +48,17 @@ def test_light_tracing_adds_light(ctx, pack):
     # The trace pass alone with caustics set leaves the caustic paths out.
     only_trace = oracle_render(p, 4, off_params, light=False)[0]
     assert on[..., :3].sum() > only_trace[..., :3].sum() * 1.01
+
+
+@pytest.mark.parametrize("config,opts", [CAUSTIC_SCENES[0], CAUSTIC_SCENES[5]], ids=["lamp", "dispersive_water"])
+def test_denoised_frames_resolve_the_same_sums(ctx, pack, config, opts):
+    """From the fourth sample on, a denoised frame resolves its splats and
+    gathered light inside the denoiser's first pass (k_finish) rather than in
+    k_resolve: the sums equal those of frames without the denoiser, to the
+    rounding of the gathered sums (photon order), and the splats alone bit
+    for bit (light tracing's PCG streams are the same)."""
+    p = pack(config, **opts)
+    on = gpu_render(ctx, p, 6, p.with_params(caustics=1, denoise=1, firefly_clamp=0.0))[0]
+    off = gpu_render(ctx, p, 6, p.with_params(caustics=1, denoise=0, firefly_clamp=0.0))[0]
+    assert np.isfinite(on).all()
+    np.testing.assert_allclose(on, off, rtol=1e-5, atol=1e-6)
