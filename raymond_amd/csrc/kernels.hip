@@ -1,9 +1,12 @@
 // Raymond's frame kernels for gfx950 (one wave64 per 8×8 pixel tile). This
 // file holds the raycast, the denoiser and the display pass:
 //   k_raycast        closest-hit / any-hit hit records for a batch of rays
-//   k_guides / k_prepare / k_atrous / k_compose   the à-trous denoiser
+//   k_guides / k_prepare / k_atrous_staged / k_atrous / k_compose
+//                    the à-trous denoiser
+//   k_finish         resolve, guides and prepare in one pass once 4 or more
+//                    samples have accumulated
 //   k_display        the display pass's RGBA8 image
-// The path-tracing passes are in main.hip (k_main), batch.hip (k_main_batch)
+// The path-tracing passes are in main.hpp (k_main), batch.hip (k_main_batch)
 // and schedules.hip (compacted, persistent, wavefront); light tracing, the
 // photon gather and the resolve in light.hip; the path integrator they share
 // in path.hpp. Each kernel restates the reference entry point cited beside
