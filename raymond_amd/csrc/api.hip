@@ -172,8 +172,9 @@ int rm_open(int device, rm_context** out) {
     c->device = device;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->resolved_ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->resolved_ev[1], hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->post, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->prepared_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->denoised_ev, hipEventDisableTiming);
@@ -183,8 +184,9 @@ int rm_open(int device, rm_context** out) {
         if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
         if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
         if (c->post) (void)hipStreamDestroy(c->post);
+        for (hipEvent_t ev : c->resolved_ev)
+            if (ev) (void)hipEventDestroy(ev);
         if (c->join) (void)hipEventDestroy(c->join);
-        if (c->fork) (void)hipEventDestroy(c->fork);
         if (c->side) (void)hipStreamDestroy(c->side);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -239,8 +241,10 @@ int rm_close(rm_context* c) {
         s->release();
         delete s;
     }
-    for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides, &c->albedo_frame, &c->heads,
-                      &c->photons, &c->emitters, &c->spill, &c->side_spill, &c->gathered, &c->env, &c->env_table, &c->rays, &c->hits, &c->counters, &c->work, &c->staging, &c->display, &c->comp, &c->folded, &c->wavefront})
+    for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats[0], &c->splats[1],
+                      &c->guides, &c->albedo_frame, &c->heads, &c->photons, &c->emitters, &c->spill, &c->side_spill,
+                      &c->gathered[0], &c->gathered[1], &c->env, &c->env_table, &c->rays, &c->hits, &c->counters,
+                      &c->work, &c->staging, &c->display, &c->comp, &c->folded, &c->wavefront})
         b->release();
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
@@ -252,8 +256,9 @@ int rm_close(rm_context* c) {
     if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
     if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
     if (c->post) (void)hipStreamDestroy(c->post);
+    for (hipEvent_t ev : c->resolved_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (c->join) (void)hipEventDestroy(c->join);
-    if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->side) (void)hipStreamDestroy(c->side);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -378,15 +383,23 @@ int rm_resize(rm_context* c, uint32_t width, uint32_t height) {
     RM_HIP(hipSetDevice(c->device), "selecting the device");
     uint32_t px = width * height;
     if (px > c->capacity) {
+        // Earlier frames on any of the three streams may still use the buffers.
+        RM_HIP(hipStreamSynchronize(c->stream), "resizing");
+        RM_HIP(hipStreamSynchronize(c->side), "resizing");
+        RM_HIP(hipStreamSynchronize(c->post), "resizing");
         size_t bytes = (size_t)px * 16;
-        for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats, &c->guides}) {
+        for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats[0],
+                          &c->splats[1], &c->guides}) {
             b->release();
             RM_HIP(b->ensure(bytes), "allocating per-pixel buffers");
         }
         RM_HIP(hipMemsetAsync(c->accum.ptr, 0, bytes, c->stream), "clearing the accumulation");
         RM_HIP(hipMemsetAsync(c->galb.ptr, 0, bytes, c->stream), "clearing the guides");
         RM_HIP(hipMemsetAsync(c->gnorm.ptr, 0, bytes, c->stream), "clearing the guides");
-        RM_HIP(hipMemsetAsync(c->splats.ptr, 0, bytes, c->stream), "clearing the splats");
+        RM_HIP(hipMemsetAsync(c->splats[0].ptr, 0, bytes, c->stream), "clearing the splats");
+        RM_HIP(hipMemsetAsync(c->splats[1].ptr, 0, bytes, c->stream), "clearing the splats");
+        // Light tracing, on the side stream, adds to the cleared splats.
+        RM_HIP(hipStreamSynchronize(c->stream), "clearing the splats");
         c->capacity = px;
         c->comp_live = false;
     }
@@ -407,6 +420,7 @@ struct Frame {
     bool caustics;
     float4* comp;
     uint32_t px;
+    uint32_t pair = 0;   // the splat and gather pair of a frame with light tracing
 };
 
 // The checks of rm_frame's arguments, before anything is enqueued.
@@ -457,41 +471,58 @@ static int frame_begin(rm_context* c, const void* emitters, uint32_t emitter_vec
     f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
          s->dispersive, tree_of(c, s)};
     // Light tracing and the photon gather depend on the scene and the
-    // parameters only, so they run on the side stream while the path pass
-    // runs; k_resolve then adds splats and gathered light, after the path
-    // pass, in the reference's order. The fork makes the side stream wait
-    // for everything queued before this frame (earlier frames' readers of
-    // the splats, photons and gathered light), the join orders k_resolve
-    // after it. The side stream keeps its own traversal spill.
+    // parameters only, so they run on the side stream beside the path pass;
+    // k_resolve (or k_finish) then adds splats and gathered light, after the
+    // path pass, in the reference's order, once the join says they are done.
+    // The side stream reads nothing the path stream writes but the splats and
+    // gathered light, double-buffered by frame: it waits only for the resolve
+    // that last read this frame's pair, so it can start while the previous
+    // frame's path pass still runs. Scenes, environments and textures are
+    // uploaded synchronously, and the emitters go up on the side stream.
     const uint32_t px = F.px = c->width * c->height;
     if (caustics) {
-        RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
-        RM_HIP(c->photons.ensure((size_t)rm::PHOTON_CAPACITY * 48), "allocating photons");
-        RM_HIP(c->gathered.ensure((size_t)px * 16), "allocating the gathered light");
-        int r = upload(c, c->emitters, emitters, (size_t)emitter_vec4s * 16, "uploading emitters");
-        if (r != RM_OK) return r;
         uint32_t count = rm::PHOTON_CAPACITY;
         if ((uint64_t)P.width * P.height * P.samples_per_frame < rm::PHOTON_CAPACITY)
             count = P.width * P.height * P.samples_per_frame;
         const uint32_t side_slots = count > px ? count : px;
-        RM_HIP(c->side_spill.ensure((size_t)side_slots * rm::SPILL_ENTRIES * sizeof(uint2)),
-               "allocating the side traversal spill");
-        RM_HIP(hipEventRecord(c->fork, c->stream), "forking the side stream");
-        RM_HIP(hipStreamWaitEvent(c->side, c->fork, 0), "forking the side stream");
+        const uint32_t pair = F.pair = c->pair;
+        c->pair ^= 1u;
+        const size_t emitter_bytes = (size_t)emitter_vec4s * 16;
+        const size_t spill_bytes = (size_t)side_slots * rm::SPILL_ENTRIES * sizeof(uint2);
+        // Growing a buffer frees the old one, and new emitters overwrite the
+        // old: earlier frames' light tracing may still be using either.
+        const bool same_emitters = c->emitters_host.size() == emitter_bytes &&
+                                   std::memcmp(c->emitters_host.data(), emitters, emitter_bytes) == 0;
+        if (!same_emitters || c->heads.bytes == 0 || c->photons.bytes == 0 ||
+            c->gathered[pair].bytes < (size_t)px * 16 || c->side_spill.bytes < spill_bytes)
+            RM_HIP(hipStreamSynchronize(c->side), "waiting for light tracing");
+        RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
+        RM_HIP(c->photons.ensure((size_t)rm::PHOTON_CAPACITY * 48), "allocating photons");
+        RM_HIP(c->gathered[pair].ensure((size_t)px * 16), "allocating the gathered light");
+        RM_HIP(c->side_spill.ensure(spill_bytes), "allocating the side traversal spill");
+        if (!same_emitters) {
+            c->emitters_host.clear();
+            RM_HIP(c->emitters.ensure(emitter_bytes), "allocating emitters");
+            RM_HIP(hipMemcpy(c->emitters.ptr, emitters, emitter_bytes, hipMemcpyHostToDevice), "uploading emitters");
+            const uint8_t* e = reinterpret_cast<const uint8_t*>(emitters);
+            c->emitters_host.assign(e, e + emitter_bytes);
+        }
+        RM_HIP(hipStreamWaitEvent(c->side, c->resolved_ev[pair], 0), "ordering light tracing");
         {
             Timed t(c, RM_STAGE_CLEAR, c->side);
             rm::launch_fill_u32(c->heads.as<uint32_t>(), 0u, rm::PHOTON_TABLE + 1, c->side);
         }
         {
             Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
-            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats.as<uint32_t>(),
+            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(),
                                    c->heads.as<uint32_t>(), c->photons.as<float4>(), c->side_spill.as<uint32_t>(),
                                    side_slots, count, f, c->side);
         }
         {
             Timed t(c, RM_STAGE_GATHER, c->side);
             rm::launch_photon_gather(S, P, c->heads.as<const uint32_t>(), c->photons.as<const float4>(),
-                                     c->gathered.as<float4>(), c->side_spill.as<uint32_t>(), side_slots, f, c->side);
+                                     c->gathered[pair].as<float4>(), c->side_spill.as<uint32_t>(), side_slots, f,
+                                     c->side);
         }
         RM_HIP(hipEventRecord(c->join, c->side), "joining the side stream");
         RM_HIP(hipGetLastError(), "launching light tracing");
@@ -590,9 +621,10 @@ static int frame_end(rm_context* c, Frame& F) {
     if (caustics && !fused) {
         {
             Timed t(c, RM_STAGE_RESOLVE);
-            rm::launch_resolve(P, c->splats.as<uint4>(), c->gathered.as<const float4>(), c->accum.as<float4>(), comp,
-                               c->stream);
+            rm::launch_resolve(P, c->splats[F.pair].as<uint4>(), c->gathered[F.pair].as<const float4>(),
+                               c->accum.as<float4>(), comp, c->stream);
         }
+        RM_HIP(hipEventRecord(c->resolved_ev[F.pair], c->stream), "ordering light tracing");
         RM_HIP(hipGetLastError(), "launching the splat resolve");
     }
 
@@ -605,9 +637,11 @@ static int frame_end(rm_context* c, Frame& F) {
         if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "ordering the denoiser");
         if (fused) {
             Timed t(c, RM_STAGE_PREPARE);
-            rm::launch_finish(P, caustics ? c->splats.as<uint4>() : nullptr, c->gathered.as<const float4>(),
-                              c->accum.as<float4>(), c->galb.as<const float4>(), c->gnorm.as<const float4>(),
-                              c->guides.ptr, fp16, a, c->albedo_frame.as<float4>(), c->stream);
+            rm::launch_finish(P, caustics ? c->splats[F.pair].as<uint4>() : nullptr,
+                              caustics ? c->gathered[F.pair].as<const float4>() : nullptr, c->accum.as<float4>(),
+                              c->galb.as<const float4>(), c->gnorm.as<const float4>(), c->guides.ptr, fp16, a,
+                              c->albedo_frame.as<float4>(), c->stream);
+            if (caustics) RM_HIP(hipEventRecord(c->resolved_ev[F.pair], c->stream), "ordering light tracing");
         } else {
             Timed t(c, RM_STAGE_PREPARE);
             const float4* sums[3];

@@ -68,9 +68,9 @@ struct rm_context {
     int device = 0;
     hipStream_t stream = nullptr;
     // Light tracing and photon gather run on `side`, alongside the path pass
-    // on `stream`; fork and join order them against the rest of the frame.
+    // on `stream`; join orders a frame's resolve after them.
     hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t join = nullptr;
     // The denoiser's à-trous passes and compose run on `post` once prepare
     // has read the frame's accumulation, so the next frame's path pass can
     // start meanwhile; `prepared_ev` and `denoised_ev` order the two.
@@ -79,8 +79,16 @@ struct rm_context {
     hipEvent_t batch_ev = nullptr;   // orders a batched path pass (rm_frame_batch) with this context's stream
     bool denoise_pending = false;
     uint32_t width = 0, height = 0, capacity = 0;
-    rmi::DevBuf accum, galb, gnorm, light_a, light_b, denoised, splats, guides, albedo_frame;
-    rmi::DevBuf heads, photons, emitters, spill, side_spill, gathered;
+    rmi::DevBuf accum, galb, gnorm, light_a, light_b, denoised, guides, albedo_frame;
+    // Light tracing's splats and the photon gather's output, a pair per frame
+    // parity: the side stream fills one pair while the path stream's resolve
+    // reads the other, so a frame's light tracing waits only for the resolve
+    // that last read its pair (resolved_ev), not for the path pass before it.
+    rmi::DevBuf splats[2], gathered[2];
+    hipEvent_t resolved_ev[2] = {nullptr, nullptr};
+    uint32_t pair = 0;   // the pair the next frame with light tracing uses
+    rmi::DevBuf heads, photons, emitters, spill, side_spill;
+    std::vector<uint8_t> emitters_host;   // the bytes now in `emitters`
     rmi::DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;
     rmi::DevBuf rays, hits, counters, work, staging, display;
