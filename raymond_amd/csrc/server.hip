@@ -68,10 +68,13 @@ struct Slot {
 // Device memory a job's context needs, by the buffers rm_frame sizes for it.
 uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, bool compensated) {
     const uint64_t px = (uint64_t)w * h;
+    // accum, two guides, two denoiser buffers, denoised, two splat buffers,
+    // guides, albedo: 10 float4 a pixel, and the traversal spill.
     uint64_t b = px * (10 * 16 + rm::SPILL_ENTRIES * 8);
     if (compensated) b += px * 96;   // error terms and their folded sums
     if (caustics) {
         const uint64_t photons = rm::PHOTON_CAPACITY;
+        b += px * 2 * 16;   // the two gathered-light buffers
         b += (px > photons ? px : photons) * rm::SPILL_ENTRIES * 8;
         b += (uint64_t)(rm::PHOTON_TABLE + 1) * 4 + (uint64_t)rm::PHOTON_CAPACITY * 48;
     }
