@@ -18,7 +18,7 @@ namespace rm {
 // (96 VGPRs) spills more and is faster again alone: 12% on a 2M-triangle
 // scene and on glass caustics, whose frames the path pass dominates. On the
 // sphere scene, whose light tracing and gather take half as long as the
-// path pass, the fifth wave crowds them out and the frame is 3% slower.
+// path pass, the fifth wave crowds them out and the frame is 1% slower.
 // RM_OPTION_WAVES chooses; rm_tune tries both.
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, int TREE, int WAVES>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_main(
