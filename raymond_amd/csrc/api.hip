@@ -478,7 +478,8 @@ static int frame_begin(rm_context* c, const void* emitters, uint32_t emitter_vec
     // gathered light, double-buffered by frame: it waits only for the resolve
     // that last read this frame's pair, so it can start while the previous
     // frame's path pass still runs. Scenes, environments and textures are
-    // uploaded synchronously, and the emitters go up on the side stream.
+    // uploaded synchronously, and the emitters too, only when they change and
+    // once the side stream has drained.
     const uint32_t px = F.px = c->width * c->height;
     if (caustics) {
         uint32_t count = rm::PHOTON_CAPACITY;
