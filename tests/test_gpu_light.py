@@ -62,3 +62,23 @@ def test_denoised_frames_resolve_the_same_sums(ctx, pack, config, opts):
     off = gpu_render(ctx, p, 6, p.with_params(caustics=1, denoise=0, firefly_clamp=0.0))[0]
     assert np.isfinite(on).all()
     np.testing.assert_allclose(on, off, rtol=1e-5, atol=1e-6)
+
+
+def test_switching_scenes_on_one_context(pack):
+    """Light-traced frames of one scene, then of another with other emitters
+    and a larger image, on one context: the second scene's sums equal those it
+    gets on a fresh context, to the rounding of the gathered sums. (The
+    emitters are uploaded again when they change, and growing the image
+    clears both splat buffers.)"""
+    from raymond_amd import capi
+    a = pack(*CAUSTIC_SCENES[1][:1], **CAUSTIC_SCENES[1][1])
+    b = pack("caustics", scene={"res": 24, "width": 160, "height": 90}, width=160, height=90)
+    pa = a.with_params(caustics=1, denoise=1, firefly_clamp=0.0)
+    pb = b.with_params(caustics=1, denoise=1, firefly_clamp=0.0)
+    with capi.Context(0) as one:
+        gpu_render(one, a, 5, pa)
+        switched = gpu_render(one, b, 5, pb)[0]
+    with capi.Context(0) as fresh:
+        alone = gpu_render(fresh, b, 5, pb)[0]
+    assert np.isfinite(switched).all()
+    np.testing.assert_allclose(switched, alone, rtol=1e-5, atol=1e-6)
