@@ -215,6 +215,54 @@ static napi_value js_frame(napi_env env, napi_callback_info info) {
     return undefined(env);
 }
 
+/* tune(ctx, scene, params, emitters, frames): rm_tune, which renders whole
+ * frames under each candidate and keeps the fastest on the context. Blocks
+ * until the frames are done. Returns {best, candidates: [{schedule,
+ * persistentBlocks, waves, ms}]}, best an index into candidates. */
+static napi_value candidate_object(napi_env env, const rm_tune_result* res, uint32_t k) {
+    napi_value o, v;
+    CHECK(napi_create_object(env, &o));
+    CHECK(napi_create_int32(env, res->schedule[k], &v));
+    CHECK(napi_set_named_property(env, o, "schedule", v));
+    CHECK(napi_create_uint32(env, res->persistent_blocks[k], &v));
+    CHECK(napi_set_named_property(env, o, "persistentBlocks", v));
+    CHECK(napi_create_uint32(env, res->waves[k], &v));
+    CHECK(napi_set_named_property(env, o, "waves", v));
+    CHECK(napi_create_double(env, res->ms[k], &v));
+    CHECK(napi_set_named_property(env, o, "ms", v));
+    return o;
+}
+
+static napi_value js_tune(napi_env env, napi_callback_info info) {
+    napi_value argv[5];
+    void *ctx, *scene, *params, *emitters;
+    size_t pb, eb;
+    uint32_t frames;
+    if (!args(env, info, 5, argv) || !get_ptr(env, argv[0], &ctx) || !get_ptr(env, argv[1], &scene)) return NULL;
+    if (!get_bytes(env, argv[2], &params, &pb) || !get_bytes(env, argv[3], &emitters, &eb)) return NULL;
+    if (!get_u32(env, argv[4], &frames)) return NULL;
+    if (pb < RM_PARAMS_BYTES) {
+        napi_throw_range_error(env, NULL, "The parameter block is 208 bytes.");
+        return NULL;
+    }
+    rm_tune_result res;
+    int r = rm_tune((rm_context*)ctx, (const rm_scene*)scene, params, (const float*)emitters, (uint32_t)(eb / 16),
+                    frames, &res);
+    if (r != RM_OK) return throw_rm(env, r);
+    napi_value out, list, v;
+    CHECK(napi_create_object(env, &out));
+    CHECK(napi_create_array_with_length(env, res.count, &list));
+    for (uint32_t k = 0; k < res.count; ++k) {
+        napi_value c = candidate_object(env, &res, k);
+        if (!c) return NULL;
+        CHECK(napi_set_element(env, list, k, c));
+    }
+    CHECK(napi_set_named_property(env, out, "candidates", list));
+    CHECK(napi_create_uint32(env, res.best, &v));
+    CHECK(napi_set_named_property(env, out, "best", v));
+    return out;
+}
+
 /* ---- asynchronous reads and waits ---- */
 
 typedef struct {
@@ -556,6 +604,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_enumerable, NULL},
         {"bvhBuild", NULL, js_bvh_build, NULL, NULL, NULL, napi_enumerable, NULL},
         {"setOption", NULL, js_set_option, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"tune", NULL, js_tune, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverSetOption", NULL, js_server_set_option, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverOpen", NULL, js_server_open, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverContext", NULL, js_server_context, NULL, NULL, NULL, napi_enumerable, NULL},

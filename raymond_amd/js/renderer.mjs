@@ -213,6 +213,32 @@ export class Renderer {
     }
 
     /**
+     * Renders a few whole frames under each candidate schedule and occupancy
+     * (rm_tune) and keeps the fastest for the frames from now on; the image
+     * starts again. Blocks until the frames are done. Returns { schedule,
+     * waves, persistentBlocks, ms, candidates: [{ schedule, waves,
+     * persistentBlocks, ms }] }. Not in the reference.
+     */
+    tune({ frames = 2 } = {}) {
+        if (!this.#scene || !this.#camera)
+            throw new Error("tune() needs a scene and a camera.");
+        this.render();
+        const emit = emitters(this.#scene.sky, this.#tables.powers, this.#delta);
+        const result = this.#addon.tune(this.#ctx, this.#sceneHandle, this.#params,
+            this.#causticsActive ? emit : null, frames);
+        const names = Object.keys(SCHEDULES);
+        const candidates = result.candidates.map((c) => ({
+            schedule: names.find((k) => SCHEDULES[k] == c.schedule), waves: c.waves,
+            persistentBlocks: c.persistentBlocks, ms: c.ms,
+        }));
+        const best = candidates[result.best];
+        this.#schedule = best.schedule;
+        this.#waves = best.waves;
+        this.#samples = 0;
+        return Object.assign({ candidates }, best);
+    }
+
+    /**
      * The average so far, 4 floats per pixel from the top left, in linear
      * light; with denoised, the denoised colour (needs denoise on and the
      * color view).

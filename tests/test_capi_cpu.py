@@ -48,8 +48,8 @@ def test_addon_exports():
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().split(",") == sorted(
         ["open", "close", "uploadScene", "releaseScene", "uploadTextures", "uploadEnvironment", "resize", "frame",
-         "read", "synchronize", "display", "raycast", "deviceInfo", "bvhBuild", "setOption", "serverSetOption", "serverOpen", "serverContext", "serverSubmit", "serverWait",
-         "serverClose"])
+         "read", "synchronize", "display", "raycast", "deviceInfo", "bvhBuild", "setOption", "serverSetOption",
+         "tune", "serverOpen", "serverContext", "serverSubmit", "serverWait", "serverClose"])
 
 
 def test_oracle_is_not_linked_into_the_product():
