@@ -37,9 +37,12 @@ def gpu_render(ctx, p, frames: int = 1, params=None, denoised: bool = False):
         scene.release()
 
 
-def oracle_render(p, frames: int = 1, params=None, light=True):
+def oracle_render(p, frames: int = 1, params=None, light=True, rows=None):
     """The oracle's accumulation after `frames` frames (light tracing and
-    photon gather included when the parameters turn caustics on)."""
+    photon gather included when the parameters turn caustics on). rows =
+    (y0, y1) runs the camera paths of those rows only (light tracing and the
+    gather still cover the image, since splats land anywhere): compare that
+    band."""
     from oracle.pyoracle import PHOTON_CAPACITY, PHOTON_TABLE, Oracle
     o = Oracle(p)
     shape = (p.height, p.width, 4)
@@ -48,7 +51,7 @@ def oracle_render(p, frames: int = 1, params=None, light=True):
     splats = np.zeros(p.width * p.height * 4, np.uint32)
     for k in range(frames):
         fp = frame_params(p, k, params)
-        o.main(fp, acc, ga, gn)
+        o.main(fp, acc, ga, gn, rows=rows)
         if caustics and light:
             heads = np.zeros(PHOTON_TABLE + 1, np.uint32)
             photons = np.zeros(PHOTON_CAPACITY * 12, np.float32)

@@ -45,7 +45,8 @@ def test_first_frame_per_sample(ctx, pack, config, opts):
     for name, g, c in zip(("accum", "albedo", "normal"), gpu, cpu):
         assert np.isfinite(g).all(), name
         frac = agreement(g, c)
-        assert frac > 0.97, f"{case_id((config, opts))} {name}: only {frac:.4f} of pixels agree"
+        print(f"\n{case_id((config, opts))} {name}: {frac:.5f} of pixels agree")
+        assert frac >= 0.999, f"{case_id((config, opts))} {name}: only {frac:.4f} of pixels agree"
 
 
 @pytest.mark.parametrize("config,opts", [CASES[0], CASES[3], CASES[5]], ids=["spheres", "triangles", "textured"])
@@ -56,7 +57,8 @@ def test_later_frames_accumulate(ctx, pack, config, opts):
     gpu = gpu_render(ctx, p, 4, params)
     cpu = oracle_render(p, 4, params)
     frac = agreement(gpu[0], cpu[0], rel=1e-4, abs_=1e-4)
-    assert frac > 0.93, f"{config}: only {frac:.4f} of pixels agree after 4 frames"
+    print(f"\n{config} after 4 frames: {frac:.5f} of pixels agree")
+    assert frac >= 0.999, f"{config}: only {frac:.4f} of pixels agree after 4 frames"
 
 
 @pytest.mark.parametrize("view", [1, 2, 3], ids=["albedo", "normal", "depth"])
@@ -79,7 +81,7 @@ def test_firefly_clamp(ctx, pack):
     g = gpu_render(ctx, p, 1, params)[0]
     assert g[..., :3].max() <= 1.5 * (1 + 1e-6)
     c = oracle_render(p, 1, params)[0]
-    assert agreement(g, c) > 0.97
+    assert agreement(g, c) >= 0.999
 
 
 @pytest.mark.parametrize("config,opts,spp", [(CASES[0][0], CASES[0][1], 1), (CASES[0][0], CASES[0][1], 3),

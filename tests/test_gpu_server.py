@@ -49,7 +49,7 @@ def test_jobs_match_their_frames_rendered_alone(ctx, pack):
         assert 0.0 <= stats["queued_ms"] <= stats["first_frame_ms"] <= stats["done_ms"], (name, stats)
         if params[j].view(np.uint32)[47]:
             frac = agreement(image, expected[j], rel=1e-3, abs_=1e-4)
-            assert frac > 0.97, (name, frac)
+            assert frac >= 0.999, (name, frac)
         else:
             assert np.array_equal(image.view(np.uint32), expected[j].view(np.uint32)), name
 

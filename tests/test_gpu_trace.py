@@ -49,11 +49,11 @@ def test_cornell_primary_rays(ctx, pack, any_hit):
     assert_same_hits(gpu, cpu, "cornell primary rays")
 
 
-@pytest.mark.parametrize("config", ["mixed", "spheres", "cornell"])
-def test_random_rays(ctx, pack, config):
+@pytest.mark.parametrize("config,seed", [("mixed", 101), ("spheres", 202), ("cornell", 303)])
+def test_random_rays(ctx, pack, config, seed):
     p = pack(config)
     o = Oracle(p)
-    rays = random_rays(200_000, seed=hash(config) % 1000)
+    rays = random_rays(200_000, seed=seed)
     scene = ctx.upload_scene(p)
     try:
         for any_hit in (False, True):
