@@ -198,6 +198,21 @@ int rm_synchronize(rm_context* ctx);
 int rm_raycast(rm_context* ctx, const rm_scene* scene, const float* rays, uint32_t n, int any_hit, uint32_t* hits,
                uint32_t* counters);
 
+/* rm_raycast for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * no counterpart in the reference. rays, hits and counters (NULL or 2 words
+ * per ray) are DEVICE pointers in rm_raycast's layouts; the kernel is
+ * enqueued on the context's stream and the call returns without waiting
+ * (rm_synchronize waits). It allocates nothing: the traversal spill is a
+ * static 28 MiB array each device holds until the process ends, so at most
+ * RM_RAYCAST_STATIC_MAX_RAYS rays a call (more is RM_ERR_INVALID). Calls on
+ * one device share that array: calls that overlap on one GPU (two contexts'
+ * streams, or threads) race, and must be ordered by the caller. The same
+ * kernel as rm_raycast, so the same hit records bit for bit. */
+#define RM_RAYCAST_STATIC_MAX_RAYS 65536
+int rm_raycast_static(rm_context* ctx, const rm_scene* scene, const float* rays, uint32_t n, int any_hit,
+                      uint32_t* hits, uint32_t* counters);
+
 /* Per-stage HIP event timing: enable or disable, read (waits; fills ms and
  * launch counts for RM_STAGE_COUNT stages, accumulated since the last
  * reset), and reset. No counterpart in the reference, which has no GPU timers. */

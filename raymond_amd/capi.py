@@ -38,6 +38,7 @@ SIGNATURES = {
     "rm_bvh_quantize": (C.c_int, [_P, C.c_uint32, _P]),
     "rm_bvh_quantize_wide": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P]),
     "rm_raycast": (C.c_int, [_P, _P, _P, _U32, C.c_int, _P, _P]),
+    "rm_raycast_static": (C.c_int, [_P, _P, _P, _U32, C.c_int, _P, _P]),
     "rm_timing": (C.c_int, [_P, C.c_int]),
     "rm_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(_U32), _U32]),
     "rm_reset_stage_times": (C.c_int, [_P]),
@@ -249,6 +250,15 @@ class Context:
         _check(load().rm_raycast(self.handle, scene.handle, _ptr(r), n, int(any_hit), _ptr(hits), _ptr(cnt)),
                "rm_raycast")
         return (hits, cnt) if counters else hits
+
+    def raycast_static(self, scene: Scene, rays: int, n: int, hits: int, any_hit: bool = False,
+                       counters: int | None = None) -> None:
+        """rm_raycast_static: rays, hits and counters are device addresses
+        (ints); enqueues the kernel on the context's stream without waiting
+        and without allocating anything."""
+        _check(load().rm_raycast_static(self.handle, scene.handle, C.c_void_p(rays), n, int(any_hit),
+                                        C.c_void_p(hits), None if counters is None else C.c_void_p(counters)),
+               "rm_raycast_static")
 
     def tune(self, scene: Scene, params: np.ndarray, frames: int = 2, emitters: np.ndarray | None = None) -> dict:
         """rm_tune: times whole frames under each candidate (schedule,

@@ -3,10 +3,44 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "context.hpp"
+
+// The traversal spill of rm_raycast_static, which allocates nothing: stack
+// entries beyond the raycast kernel's LDS depth for up to
+// RM_RAYCAST_STATIC_MAX_RAYS lanes, laid out [entry][lane] as the kernel
+// indexes any spill ((STACK_LIMIT - TRACE_LDS_DEPTH) × 65,536 × 8 B = 28 MiB).
+// Each device holds its own copy, at its own address, until the process ends.
+// In host code `scratch` only names the symbol: kernels get the device
+// address that rm_open looks up, never the name itself.
+constexpr size_t STATIC_SCRATCH_ENTRIES =
+    (size_t)(rm::STACK_LIMIT - rm::TRACE_LDS_DEPTH) * RM_RAYCAST_STATIC_MAX_RAYS;
+static __device__ __align__(256) uint2 scratch[STATIC_SCRATCH_ENTRIES];
+
+namespace {
+// Each device's address of `scratch`, looked up once with that device current.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_lock;
+uint2* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The scratch's address on the current device `device`.
+hipError_t static_scratch_of(int device, uint2** out) {
+    *out = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<uint2*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
 
 namespace rmi {
 
@@ -202,6 +236,11 @@ int rm_open(int device, rm_context** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->compute_units = prop.multiProcessorCount;
+    e = static_scratch_of(device, &c->static_scratch);
+    if (e != hipSuccess) {
+        rm_close(c);
+        return hip_fail(e, "finding the static raycast scratch");
+    }
     if (c->work.ensure(256) != hipSuccess) {
         rm_close(c);
         return fail(RM_ERR_MEMORY, "allocating the work counter");
@@ -819,6 +858,26 @@ int rm_raycast(rm_context* c, const rm_scene* s, const float* rays, uint32_t n, 
         RM_HIP(hipMemcpyAsync(counters, c->counters.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream),
                "reading counters");
     RM_HIP(hipStreamSynchronize(c->stream), "tracing rays");
+    return RM_OK;
+}
+
+int rm_raycast_static(rm_context* c, const rm_scene* s, const float* rays, uint32_t n, int any_hit, uint32_t* hits,
+                      uint32_t* counters) {
+    if (n > RM_RAYCAST_STATIC_MAX_RAYS)
+        return fail(RM_ERR_INVALID, "rm_raycast_static traces at most " + std::to_string(RM_RAYCAST_STATIC_MAX_RAYS) +
+                                        " rays a call (asked for " + std::to_string(n) + "); use rm_raycast, or split "
+                                        "the rays.");
+    if (!c || !s || (!rays && n) || (!hits && n))
+        return fail(RM_ERR_INVALID, "rm_raycast_static needs a context, a scene, rays and room for hits.");
+    if (n == 0) return RM_OK;
+    if (!c->static_scratch) return fail(RM_ERR_STATE, "The context has no static raycast scratch.");
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    // The kernel's spill slot stride is n, so it indexes entries below
+    // (STACK_LIMIT - TRACE_LDS_DEPTH) · n, inside the scratch.
+    rm::launch_raycast(dev_scene(c, s), reinterpret_cast<const float4*>(rays), n, reinterpret_cast<uint4*>(hits),
+                       any_hit != 0, tree_of(c, s), reinterpret_cast<uint32_t*>(c->static_scratch), n,
+                       reinterpret_cast<uint2*>(counters), c->stream);
+    RM_HIP(hipGetLastError(), "launching the raycast");
     return RM_OK;
 }
 

@@ -101,6 +101,9 @@ struct rm_context {
     rm::Params last_params = {};   // the last frame's, for rm_display
     bool rendered = false;
     int compute_units = 0;
+    // This device's copy of the static raycast scratch (rm_raycast_static),
+    // looked up by rm_open.
+    uint2* static_scratch = nullptr;
     int64_t options[RM_OPTION_COUNT] = {};
     bool timing = false;
     rmi::StageTimer timer;

@@ -40,6 +40,10 @@ def test_null_arguments_are_rejected_without_touching_a_device():
     assert b"context" in lib.rm_last_error()
     assert lib.rm_close(None) == -1
     assert lib.rm_frame(None, None, None, None, 0) == -1
+    # More rays than the static scratch holds: refused before anything else.
+    assert lib.rm_raycast_static(None, None, None, 65537, 0, None, None) == -1
+    assert b"at most 65536 rays" in lib.rm_last_error()
+    assert lib.rm_raycast_static(None, None, None, 1, 0, None, None) == -1
 
 
 def test_addon_exports():
