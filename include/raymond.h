@@ -26,6 +26,7 @@ extern "C" {
 #define RM_ERR_DEVICE (-2)    /* a HIP runtime error */
 #define RM_ERR_MEMORY (-3)    /* device memory exhausted */
 #define RM_ERR_STATE (-4)     /* a call before what it needs (e.g. rm_frame before rm_resize) */
+#define RM_ERR_CANCELLED (-5) /* a served job cancelled by rm_server_cancel, or past its deadline */
 
 #define RM_PARAMS_BYTES 208
 
@@ -167,10 +168,9 @@ int rm_frame(rm_context* ctx, const rm_scene* scene, const void* params, const f
  * grid's tiles split among the jobs, whatever their schedules. Every job
  * renders exactly as it would alone. The frames must walk the same kind of
  * tree (RM_PRECISION_QUANTIZED_BVH set alike). emitters and emitter_vec4s
- * may be NULL when no frame traces light. Arguments are all checked before
- * anything is queued; a later failure (device memory running out) can leave
- * earlier contexts of the batch with a frame begun but no path pass, whose
- * images must then start again. No counterpart in the reference, which
+ * may be NULL when no frame traces light. Arguments are all checked, and
+ * every context's buffers allocated, before anything is queued, so a failure
+ * (device memory running out) queues nothing. No counterpart in the reference, which
  * renders one image per Renderer; the job server uses it under
  * RM_OPTION_BATCH. */
 int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const void* const* params,
@@ -278,6 +278,8 @@ typedef struct rm_job {
     const float* env_table;
     uint32_t table_width, table_height;
     uint32_t target_samples;       /* samples per pixel at which the job is done */
+    double deadline_ms;            /* 0: none; else the job must have queued its last frame this many ms after
+                                      submission, or it ends with RM_ERR_CANCELLED (as if cancelled then) */
 } rm_job;
 
 typedef struct rm_job_stats {
@@ -296,6 +298,11 @@ int rm_server_submit(rm_server* server, const rm_job* job, uint64_t* id);
 /* Waits for a job and copies its image: the denoised buffer when the job's
  * params.denoise is set, else the accumulation (W·H·4 floats; dst may be NULL). */
 int rm_server_wait(rm_server* server, uint64_t id, float* dst, size_t floats, rm_job_stats* stats);
+/* Cancels a job: a queued job leaves the queue, a running one queues no
+ * more frames and ends once those in flight are done; rm_server_wait then
+ * returns RM_ERR_CANCELLED (its stats count the frames done). A job whose
+ * last frame is already queued finishes with its image. */
+int rm_server_cancel(rm_server* server, uint64_t id);
 /* Sets an RM_OPTION_* (as rm_set_option does for a context) for the jobs
  * submitted from now on; each job keeps the options it was submitted with. */
 int rm_server_set_option(rm_server* server, int option, int64_t value);

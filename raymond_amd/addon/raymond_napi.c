@@ -500,18 +500,23 @@ static napi_value js_server_context(napi_env env, napi_callback_info info) {
 }
 
 /* serverSubmit(server, scene, params, targetSamples, emitters|null, envRgba|null, envWidth, envHeight,
- *              envTable|null, tableWidth, tableHeight) → job id (a Number) */
+ *              envTable|null, tableWidth, tableHeight, deadlineMs) → job id (a Number) */
 static napi_value js_server_submit(napi_env env, napi_callback_info info) {
-    napi_value argv[11];
+    napi_value argv[12];
     void *sv, *scene, *params, *emitters, *rgba, *table;
     size_t pb, eb, rb, tb;
     uint32_t target, ew, eh, tw, th;
-    if (!args(env, info, 11, argv) || !get_ptr(env, argv[0], &sv) || !get_ptr(env, argv[1], &scene) ||
+    double deadline = 0.0;
+    if (!args(env, info, 12, argv) || !get_ptr(env, argv[0], &sv) || !get_ptr(env, argv[1], &scene) ||
         !get_bytes(env, argv[2], &params, &pb) || !get_u32(env, argv[3], &target) ||
         !get_bytes(env, argv[4], &emitters, &eb) || !get_bytes(env, argv[5], &rgba, &rb) ||
         !get_u32(env, argv[6], &ew) || !get_u32(env, argv[7], &eh) || !get_bytes(env, argv[8], &table, &tb) ||
         !get_u32(env, argv[9], &tw) || !get_u32(env, argv[10], &th))
         return NULL;
+    if (napi_get_value_double(env, argv[11], &deadline) != napi_ok) {
+        napi_throw_type_error(env, NULL, "Expected a deadline in milliseconds.");
+        return NULL;
+    }
     if (pb < RM_PARAMS_BYTES) {
         napi_throw_range_error(env, NULL, "The parameter block is shorter than 208 bytes.");
         return NULL;
@@ -531,12 +536,28 @@ static napi_value js_server_submit(napi_env env, napi_callback_info info) {
         job.table_height = th;
     }
     job.target_samples = target;
+    job.deadline_ms = deadline;
     uint64_t id = 0;
     int r = rm_server_submit((rm_server*)sv, &job, &id);
     if (r != RM_OK) return throw_rm(env, r);
     napi_value out;
     CHECK(napi_create_double(env, (double)id, &out));
     return out;
+}
+
+/* serverCancel(server, id) */
+static napi_value js_server_cancel(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    void* sv;
+    double id;
+    if (!args(env, info, 2, argv) || !get_ptr(env, argv[0], &sv)) return NULL;
+    if (napi_get_value_double(env, argv[1], &id) != napi_ok) {
+        napi_throw_type_error(env, NULL, "Expected a job id.");
+        return NULL;
+    }
+    int r = rm_server_cancel((rm_server*)sv, (uint64_t)id);
+    if (r != RM_OK) return throw_rm(env, r);
+    return undefined(env);
 }
 
 /* serverWait(server, id, Float32Array|null) → Promise of the job's timings */
@@ -610,6 +631,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"serverContext", NULL, js_server_context, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverSubmit", NULL, js_server_submit, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverWait", NULL, js_server_wait, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"serverCancel", NULL, js_server_cancel, NULL, NULL, NULL, napi_enumerable, NULL},
         {"serverClose", NULL, js_server_close, NULL, NULL, NULL, napi_enumerable, NULL},
     };
     if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return NULL;

@@ -14,6 +14,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libraymond_hip.so"
 
 RM_OK = 0
+ERR_INVALID, ERR_DEVICE, ERR_MEMORY, ERR_STATE, ERR_CANCELLED = -1, -2, -3, -4, -5
 BUFFER_ACCUM, BUFFER_GUIDE_ALBEDO, BUFFER_GUIDE_NORMAL, BUFFER_DENOISED = 0, 1, 2, 3
 STAGES = ("trace", "light_trace", "resolve", "gather", "prepare", "atrous", "compose", "clear", "raycast")
 
@@ -50,6 +51,7 @@ SIGNATURES = {
     "rm_server_submit": (C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
     "rm_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.c_size_t, _P]),
     "rm_server_close": (C.c_int, [_P]),
+    "rm_server_cancel": (C.c_int, [_P, C.c_uint64]),
     "rm_server_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
 OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE, OPTION_BATCH, OPTION_WAVES = range(6)
@@ -71,7 +73,11 @@ class _TuneResult(C.Structure):
 
 
 class RaymondError(RuntimeError):
-    pass
+    """A failed call; `code` is its RM_ERR_* value."""
+
+    def __init__(self, message: str, code: int = 0):
+        super().__init__(message)
+        self.code = code
 
 
 def load() -> C.CDLL:
@@ -95,7 +101,7 @@ def _ptr(a: np.ndarray | None):
 
 def _check(code: int, what: str) -> None:
     if code != RM_OK:
-        raise RaymondError(f"{what}: {load().rm_last_error().decode()} (code {code})")
+        raise RaymondError(f"{what}: {load().rm_last_error().decode()} (code {code})", code)
 
 
 def frame_batch(contexts, scenes, params, emitters=None) -> None:
@@ -300,7 +306,7 @@ class Context:
 class _Job(C.Structure):
     _fields_ = [("scene", _P), ("params", _P), ("emitters", _P), ("emitter_vec4s", _U32), ("env_rgba", _P),
                 ("env_width", _U32), ("env_height", _U32), ("env_table", _P), ("table_width", _U32),
-                ("table_height", _U32), ("target_samples", _U32)]
+                ("table_height", _U32), ("target_samples", _U32), ("deadline_ms", C.c_double)]
 
 
 class _JobStats(C.Structure):
@@ -322,12 +328,13 @@ class Server:
         self.sizes = {}
 
     def submit(self, scene: Scene, params: np.ndarray, target_samples: int, emitters: np.ndarray | None = None,
-               packed=None) -> int:
-        """Queues a job; `packed` (a scenefile.Packed) supplies the environment when it has a map."""
+               packed=None, deadline_ms: float = 0.0) -> int:
+        """Queues a job; `packed` (a scenefile.Packed) supplies the environment when it has a map;
+        deadline_ms > 0 ends the job (RM_ERR_CANCELLED) if its last frame isn't queued by then."""
         p = np.ascontiguousarray(params, dtype=np.uint8)
         e = None if emitters is None else np.ascontiguousarray(emitters, dtype=np.float32)
         job = _Job(scene=scene.handle, params=_ptr(p), emitters=_ptr(e), emitter_vec4s=0 if e is None else e.size // 4,
-                   target_samples=target_samples)
+                   target_samples=target_samples, deadline_ms=deadline_ms)
         keep = [p, e]
         if packed is not None and packed.meta.get("envTableHeight", 0) > 0:
             rgba = np.ascontiguousarray(packed.env_rgba, dtype=np.uint16)
@@ -341,6 +348,10 @@ class Server:
         words = p.view(np.uint32)
         self.sizes[job_id.value] = (int(words[41]), int(words[40]))
         return job_id.value
+
+    def cancel(self, job_id: int) -> None:
+        """rm_server_cancel: the job stops (wait() then raises, saying it was cancelled)."""
+        _check(load().rm_server_cancel(self.handle, job_id), "rm_server_cancel")
 
     def set_option(self, option: int, value: int) -> None:
         """An OPTION_* for the jobs submitted from now on."""

@@ -162,6 +162,38 @@ int rmi::current_sums(rm_context* c, hipStream_t stream, const float4* out[3]) {
     return RM_OK;
 }
 
+namespace {
+// Every growable buffer of a context but its environment and work counter.
+std::vector<DevBuf*> working_buffers(rm_context* c) {
+    return {&c->accum,     &c->galb,        &c->gnorm,       &c->light_a,     &c->light_b,   &c->denoised,
+            &c->guides,    &c->albedo_frame, &c->splats[0],  &c->splats[1],   &c->gathered[0], &c->gathered[1],
+            &c->heads,     &c->photons,     &c->emitters,    &c->spill,       &c->side_spill, &c->rays,
+            &c->hits,      &c->counters,    &c->staging,     &c->display,     &c->comp,      &c->folded,
+            &c->wavefront};
+}
+}  // namespace
+
+uint64_t rmi::context_bytes(const rm_context* c) {
+    uint64_t b = c->env.bytes + c->env_table.bytes + c->work.bytes;
+    for (DevBuf* d : working_buffers(const_cast<rm_context*>(c))) b += d->bytes;
+    return b;
+}
+
+int rmi::trim(rm_context* c) {
+    RM_HIP(hipSetDevice(c->device), "selecting the device");
+    RM_HIP(hipStreamSynchronize(c->stream), "waiting for the stream");
+    RM_HIP(hipStreamSynchronize(c->side), "waiting for the side stream");
+    RM_HIP(hipStreamSynchronize(c->post), "waiting for the denoiser's stream");
+    for (DevBuf* d : working_buffers(c)) d->release();
+    c->emitters_host.clear();
+    c->capacity = 0;
+    c->width = c->height = 0;
+    c->comp_live = false;
+    c->denoise_pending = false;
+    c->rendered = false;
+    return RM_OK;
+}
+
 // The tree traversals walk: the FP32 nodes, or when quantized nodes are
 // asked for, the 4-wide quantized tree, else the binary quantized one, else
 // (a tree that can't be quantized) the FP32 nodes.
@@ -361,6 +393,10 @@ int rm_scene_upload(rm_context* c, const void* primitives, uint32_t primitive_sl
     if (r == RM_OK) r = upload(c, s->lights, lights, (size_t)lights_len * 4, "uploading the lights table");
     if (r == RM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(RM_ERR_DEVICE, "uploading a scene");
     if (r == RM_OK && lights) s->lights_host.assign(lights, lights + lights_len);
+    if (r == RM_OK) {
+        s->kinds_host.resize(primitive_slots);
+        for (uint32_t i = 0; i < primitive_slots; ++i) s->kinds_host[i] = (uint8_t)(pw[i * 16] < 255u ? pw[i * 16] : 255u);
+    }
     if (r != RM_OK) {
         s->release();
         delete s;
@@ -460,6 +496,12 @@ struct Frame {
     float4* comp;
     uint32_t px;
     uint32_t pair = 0;   // the splat and gather pair of a frame with light tracing
+    // Set by frame_alloc: light tracing's photons and spill lanes; the path
+    // pass's spill lanes, persistent grid, staging records, wavefront state.
+    uint32_t photons = 0, side_slots = 0;
+    uint32_t slots = 0, persistent = 0;
+    float4* staging = nullptr;
+    bool wavefront = false;
 };
 
 // The checks of rm_frame's arguments, before anything is enqueued.
@@ -483,12 +525,18 @@ static int frame_check(rm_context* c, const rm_scene* s, const void* params, con
     if (caustics) {
         const uint32_t* ew = reinterpret_cast<const uint32_t*>(emitters);
         uint64_t need = 1ull + ew[0] + ew[1];
-        if (ew[0] == 0 || need > emitter_vec4s)
+        // Light tracing aims every photon at one of the targets and samples
+        // a source as a quad or a sphere (or the sun, NONE).
+        if (ew[0] == 0 || ew[1] == 0)
+            return fail(RM_ERR_INVALID, "Light tracing needs at least one emitter and one target.");
+        if (need > emitter_vec4s)
             return fail(RM_ERR_INVALID, "The emitters buffer is shorter than its header says.");
         for (uint32_t k = 0; k < ew[0]; ++k) {
             uint32_t prim = ew[(1 + k) * 4 + 1];
             if (prim != 0xffffffffu && prim >= s->primitive_slots)
                 return fail(RM_ERR_INVALID, "An emitter names a primitive the scene doesn't have.");
+            if (prim != 0xffffffffu && s->kinds_host[prim] != rm::QUAD && s->kinds_host[prim] != rm::SPHERE)
+                return fail(RM_ERR_INVALID, "An emitter is neither a quad nor a sphere.");
         }
     }
     F.s = s;
@@ -496,39 +544,28 @@ static int frame_check(rm_context* c, const rm_scene* s, const void* params, con
     return RM_OK;
 }
 
-// Everything before the path pass: light tracing and the photon gather on
-// the side stream, the compensation terms.
-static int frame_begin(rm_context* c, const void* emitters, uint32_t emitter_vec4s, Frame& F) {
+// Every allocation a frame needs, before anything of it is enqueued: a
+// frame that fails here leaves the context as it was (no light tracing
+// queued into a splat pair that no resolve would then clear).
+// path_buffers: also the buffers of the context's own path-pass schedule
+// (a batched frame's path pass runs in the batch's kernel instead).
+static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec4s, Frame& F, bool path_buffers) {
     const rm_scene* s = F.s;
     const rm::Params& P = F.P;
-    const bool caustics = F.caustics;
     RM_HIP(hipSetDevice(c->device), "selecting the device");
-
-    rm::DevScene& S = F.S;
-    S = dev_scene(c, s);
-    rm::Features& f = F.f;
-    f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
-         s->dispersive, tree_of(c, s)};
-    // Light tracing and the photon gather depend on the scene and the
-    // parameters only, so they run on the side stream beside the path pass;
-    // k_resolve (or k_finish) then adds splats and gathered light, after the
-    // path pass, in the reference's order, once the join says they are done.
-    // The side stream reads nothing the path stream writes but the splats and
-    // gathered light, double-buffered by frame: it waits only for the resolve
-    // that last read this frame's pair, so it can start while the previous
-    // frame's path pass still runs. Scenes, environments and textures are
-    // uploaded synchronously, and the emitters too, only when they change and
-    // once the side stream has drained.
+    F.S = dev_scene(c, s);
+    F.f = {P.volume_count > 0u, s->tex_layers + s->nmap_layers > 0, P.env_strength > 0.0f && c->table_h > 0,
+           s->dispersive, tree_of(c, s)};
     const uint32_t px = F.px = c->width * c->height;
-    if (caustics) {
+    if (F.caustics) {
         uint32_t count = rm::PHOTON_CAPACITY;
         if ((uint64_t)P.width * P.height * P.samples_per_frame < rm::PHOTON_CAPACITY)
             count = P.width * P.height * P.samples_per_frame;
-        const uint32_t side_slots = count > px ? count : px;
+        F.photons = count;
+        F.side_slots = count > px ? count : px;
         const uint32_t pair = F.pair = c->pair;
-        c->pair ^= 1u;
         const size_t emitter_bytes = (size_t)emitter_vec4s * 16;
-        const size_t spill_bytes = (size_t)side_slots * rm::SPILL_ENTRIES * sizeof(uint2);
+        const size_t spill_bytes = (size_t)F.side_slots * rm::SPILL_ENTRIES * sizeof(uint2);
         // Growing a buffer frees the old one, and new emitters overwrite the
         // old: earlier frames' light tracing may still be using either.
         const bool same_emitters = c->emitters_host.size() == emitter_bytes &&
@@ -547,6 +584,67 @@ static int frame_begin(rm_context* c, const void* emitters, uint32_t emitter_vec
             const uint8_t* e = reinterpret_cast<const uint8_t*>(emitters);
             c->emitters_host.assign(e, e + emitter_bytes);
         }
+    }
+    const bool compensated = (c->options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0;
+    if (compensated) {
+        RM_HIP(c->comp.ensure((size_t)px * 48), "allocating the compensation terms");
+        RM_HIP(c->folded.ensure((size_t)px * 48), "allocating the folded sums");
+    }
+    if (P.denoise != 0u) RM_HIP(c->albedo_frame.ensure((size_t)px * 16), "allocating the denoiser's albedo");
+
+    // The path pass's own buffers, by schedule.
+    F.slots = px;
+    F.persistent = 0;
+    F.staging = nullptr;
+    F.wavefront = false;
+    const int64_t schedule = path_buffers ? c->options[RM_OPTION_SCHEDULE] : RM_SCHEDULE_PIXEL;
+    if (schedule == RM_SCHEDULE_PERSISTENT && P.view == 0u) {
+        int64_t blocks = c->options[RM_OPTION_PERSISTENT_BLOCKS];
+        if (blocks == 0)
+            blocks = (int64_t)(c->compute_units > 0 ? c->compute_units : 1) * rm::main_resident_blocks_per_cu(F.f);
+        F.persistent = (uint32_t)blocks;
+        const uint32_t threads = F.persistent * rm::BLOCK_THREADS;
+        int r = ensure_spill(c, threads > px ? threads : px);
+        if (r != RM_OK) return r;
+        F.slots = threads;
+    } else if (schedule == RM_SCHEDULE_COMPACT && P.view == 0u) {
+        const uint32_t threads = ((c->width + 15u) / 16u) * ((c->height + 15u) / 16u) * rm::BLOCK_THREADS;
+        int r = ensure_spill(c, threads);
+        if (r != RM_OK) return r;
+        RM_HIP(c->staging.ensure((size_t)threads * 48), "allocating the path staging records");
+        F.staging = c->staging.as<float4>();
+        F.slots = threads;
+    } else if (schedule == RM_SCHEDULE_WAVEFRONT && P.view == 0u) {
+        // Every stage's grid covers all paths, a thread per queue entry.
+        const uint32_t threads = (px + rm::BLOCK_THREADS - 1) / rm::BLOCK_THREADS * rm::BLOCK_THREADS;
+        int r = ensure_spill(c, threads);
+        if (r != RM_OK) return r;
+        RM_HIP(c->wavefront.ensure(rm::wavefront_bytes(px)), "allocating the wavefront path state");
+        F.wavefront = true;
+        F.slots = threads;
+    }
+    return RM_OK;
+}
+
+// Everything before the path pass: light tracing and the photon gather on
+// the side stream, the compensation terms.
+static int frame_begin(rm_context* c, Frame& F) {
+    const rm::Params& P = F.P;
+    const rm::DevScene& S = F.S;
+    const rm::Features& f = F.f;
+    // Light tracing and the photon gather depend on the scene and the
+    // parameters only, so they run on the side stream beside the path pass;
+    // k_resolve (or k_finish) then adds splats and gathered light, after the
+    // path pass, in the reference's order, once the join says they are done.
+    // The side stream reads nothing the path stream writes but the splats and
+    // gathered light, double-buffered by frame: it waits only for the resolve
+    // that last read this frame's pair, so it can start while the previous
+    // frame's path pass still runs. Scenes, environments and textures are
+    // uploaded synchronously, and the emitters too, only when they change and
+    // once the side stream has drained.
+    if (F.caustics) {
+        const uint32_t pair = F.pair;
+        c->pair = pair ^ 1u;
         RM_HIP(hipStreamWaitEvent(c->side, c->resolved_ev[pair], 0), "ordering light tracing");
         {
             Timed t(c, RM_STAGE_CLEAR, c->side);
@@ -556,12 +654,12 @@ static int frame_begin(rm_context* c, const void* emitters, uint32_t emitter_vec
             Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
             rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(),
                                    c->heads.as<uint32_t>(), c->photons.as<float4>(), c->side_spill.as<uint32_t>(),
-                                   side_slots, count, f, c->side);
+                                   F.side_slots, F.photons, f, c->side);
         }
         {
             Timed t(c, RM_STAGE_GATHER, c->side);
             rm::launch_photon_gather(S, P, c->heads.as<const uint32_t>(), c->photons.as<const float4>(),
-                                     c->gathered[pair].as<float4>(), c->side_spill.as<uint32_t>(), side_slots, f,
+                                     c->gathered[pair].as<float4>(), c->side_spill.as<uint32_t>(), F.side_slots, f,
                                      c->side);
         }
         RM_HIP(hipEventRecord(c->join, c->side), "joining the side stream");
@@ -572,52 +670,24 @@ static int frame_begin(rm_context* c, const void* emitters, uint32_t emitter_vec
     // path pass writes them on a first frame), or when compensation starts
     // part-way through an accumulation.
     const bool compensated = (c->options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0;
-    float4*& comp = F.comp;
-    comp = nullptr;
+    F.comp = nullptr;
     if (compensated) {
-        RM_HIP(c->comp.ensure((size_t)px * 48), "allocating the compensation terms");
-        comp = c->comp.as<float4>();
-        if (!c->comp_live && P.sample_offset != 0u) rm::launch_fill_u32(c->comp.as<uint32_t>(), 0u, (size_t)px * 12, c->stream);
+        F.comp = c->comp.as<float4>();
+        if (!c->comp_live && P.sample_offset != 0u)
+            rm::launch_fill_u32(c->comp.as<uint32_t>(), 0u, (size_t)F.px * 12, c->stream);
     }
     c->comp_live = compensated;
     return RM_OK;
 }
 
-// The path pass under the context's schedule.
+// The path pass under the context's schedule, into the buffers frame_alloc sized.
 static int frame_path(rm_context* c, Frame& F) {
     const rm::Params& P = F.P;
     const rm::DevScene& S = F.S;
     const rm::Features& f = F.f;
     const uint32_t px = F.px;
-    float4* comp = F.comp;
-    uint32_t slots = px;
-    uint32_t persistent = 0;
-    float4* staging = nullptr;
-    const int64_t schedule = c->options[RM_OPTION_SCHEDULE];
-    if (schedule == RM_SCHEDULE_PERSISTENT && P.view == 0u) {
-        int64_t blocks = c->options[RM_OPTION_PERSISTENT_BLOCKS];
-        if (blocks == 0) blocks = (int64_t)(c->compute_units > 0 ? c->compute_units : 1) * rm::main_resident_blocks_per_cu(f);
-        persistent = (uint32_t)blocks;
-        uint32_t threads = persistent * rm::BLOCK_THREADS;
-        int r = ensure_spill(c, threads > slots ? threads : slots);
-        if (r != RM_OK) return r;
-        slots = threads;
-        rm::launch_fill_u32(c->work.as<uint32_t>(), 0u, 1, c->stream);
-    } else if (schedule == RM_SCHEDULE_COMPACT && P.view == 0u) {
-        uint32_t threads = ((c->width + 15u) / 16u) * ((c->height + 15u) / 16u) * rm::BLOCK_THREADS;
-        int r = ensure_spill(c, threads);
-        if (r != RM_OK) return r;
-        RM_HIP(c->staging.ensure((size_t)threads * 48), "allocating the path staging records");
-        staging = c->staging.as<float4>();
-        slots = threads;
-    }
-    if (schedule == RM_SCHEDULE_WAVEFRONT && P.view == 0u) {
-        // Every stage's grid covers all paths, a thread per queue entry.
-        const uint32_t blocks = (px + rm::BLOCK_THREADS - 1) / rm::BLOCK_THREADS;
-        const uint32_t threads = blocks * rm::BLOCK_THREADS;
-        int r = ensure_spill(c, threads);
-        if (r != RM_OK) return r;
-        RM_HIP(c->wavefront.ensure(rm::wavefront_bytes(px)), "allocating the wavefront path state");
+    if (F.persistent) rm::launch_fill_u32(c->work.as<uint32_t>(), 0u, 1, c->stream);
+    if (F.wavefront) {
         rm::Wavefront W;
         char* base = c->wavefront.as<char>();
         auto carve = [&](size_t bytes) {
@@ -634,13 +704,13 @@ static int frame_path(rm_context* c, Frame& F) {
         W.sums = reinterpret_cast<float4*>(carve((size_t)px * 48));
         W.counts = reinterpret_cast<uint32_t*>(carve(16));
         Timed t(c, RM_STAGE_TRACE);
-        rm::LaunchWavefront L{S, P, W, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), comp,
-                              c->spill.as<uint32_t>(), threads, blocks, f.tree};
+        rm::LaunchWavefront L{S, P, W, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), F.comp,
+                              c->spill.as<uint32_t>(), F.slots, F.slots / rm::BLOCK_THREADS, f.tree};
         rm::launch_wavefront(L, f, c->stream);
     } else {
         Timed t(c, RM_STAGE_TRACE);
-        rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), comp,
-                         c->spill.as<uint32_t>(), slots, persistent, c->work.as<uint32_t>(), staging, f.tree,
+        rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), F.comp,
+                         c->spill.as<uint32_t>(), F.slots, F.persistent, c->work.as<uint32_t>(), F.staging, f.tree,
                          c->options[RM_OPTION_WAVES] == 5 ? 5 : 4};
         rm::launch_main(L, f, c->stream);
     }
@@ -672,7 +742,6 @@ static int frame_end(rm_context* c, Frame& F) {
         const bool fp16 = (c->options[RM_OPTION_PRECISION] & RM_PRECISION_FP16_GUIDES) != 0;
         float4* a = c->light_a.as<float4>();
         float4* b = c->light_b.as<float4>();
-        RM_HIP(c->albedo_frame.ensure((size_t)c->width * c->height * 16), "allocating the denoiser's albedo");
         // The previous frame's passes still read light_a/b, guides and the albedo.
         if (c->denoise_pending) RM_HIP(hipStreamWaitEvent(c->stream, c->denoised_ev, 0), "ordering the denoiser");
         if (fused) {
@@ -717,7 +786,8 @@ static int frame_end(rm_context* c, Frame& F) {
 int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* emitters, uint32_t emitter_vec4s) {
     Frame F;
     int r = frame_check(c, s, params, emitters, emitter_vec4s, F);
-    if (r == RM_OK) r = frame_begin(c, emitters, emitter_vec4s, F);
+    if (r == RM_OK) r = frame_alloc(c, emitters, emitter_vec4s, F, true);
+    if (r == RM_OK) r = frame_begin(c, F);
     if (r == RM_OK) r = frame_path(c, F);
     if (r == RM_OK) r = frame_end(c, F);
     return r;
@@ -742,7 +812,12 @@ int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const
                                         "the same for all, and quantizable scenes).");
     }
     for (uint32_t i = 0; i < n; ++i) {
-        int r = frame_begin(ctxs[i], emitters ? emitters[i] : nullptr, emitter_vec4s ? emitter_vec4s[i] : 0u, F[i]);
+        int r = frame_alloc(ctxs[i], emitters ? emitters[i] : nullptr, emitter_vec4s ? emitter_vec4s[i] : 0u, F[i],
+                            false);
+        if (r != RM_OK) return r;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        int r = frame_begin(ctxs[i], F[i]);
         if (r != RM_OK) return r;
     }
     // One launch for all the path passes, on the first context's stream once

@@ -30,7 +30,14 @@ struct DevBuf {
         ptr = nullptr;
         bytes = 0;
         hipError_t e = hipMalloc(&ptr, need);
-        if (e == hipSuccess) bytes = need;
+        if (e == hipSuccess) {
+            bytes = need;
+        } else {
+            // The failure is returned here; left as the thread's last error,
+            // it would be reported again by the next launch's check.
+            ptr = nullptr;
+            (void)hipGetLastError();
+        }
         return e;
     }
     void release() {
@@ -58,6 +65,7 @@ struct rm_scene {
     uint32_t tex_w = 0, tex_h = 0, tex_layers = 0, nmap_w = 0, nmap_h = 0, nmap_layers = 0;
     bool dispersive = false;
     std::vector<uint32_t> lights_host;   // the lights table, for checking parameters against
+    std::vector<uint8_t> kinds_host;     // each primitive slot's kind, for checking emitters against
     void release() {
         for (rmi::DevBuf* b : {&prims, &nodes, &mats, &attr, &lights, &textures, &nmaps, &qnodes, &wnodes})
             b->release();
@@ -117,4 +125,9 @@ namespace rmi {
 int current_sums(rm_context* ctx, hipStream_t stream, const float4* out[3]);
 // The tree a frame on ctx walks for scene (rm::TREE_*).
 int tree_of(const rm_context* ctx, const rm_scene* scene);
+// Bytes of device memory the context's own buffers hold (its scenes apart).
+uint64_t context_bytes(const rm_context* ctx);
+// Waits for the context's streams and frees its per-image and working
+// buffers (the next rm_resize and frames allocate them again at their size).
+int trim(rm_context* ctx);
 }  // namespace rmi

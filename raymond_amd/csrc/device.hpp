@@ -320,6 +320,8 @@ constexpr int TREE_WIDE = 2;        // the 4-wide 8-bit quantized records
 template <int LDS_DEPTH, int BLOCK, int TREE = TREE_FP32>
 struct Stack {
     static constexpr int tree = TREE;
+    static constexpr int lds_depth = LDS_DEPTH;
+    static constexpr int block = BLOCK;
     uint2* lds;      // &shared[0][threadIdx.x]
     uint2* spill;    // this thread's column of an [entry][slot] spill array
     uint32_t stride; // slots in the spill array
@@ -352,9 +354,17 @@ struct Count {
 // sphere scenes. Shadow rays, which stop at their first hit, are faster with
 // one node or leaf per pass. Either way a ray visits its nodes and
 // primitives in the same order.
-template <bool ANY, class NODE, class POP, class COUNT>
+//
+// TIES (closest hit over a quantized tree): a primitive hit at exactly the
+// nearest t so far sets `tie` (a nearer hit clears it). The quantized boxes
+// can order two overlapping children differently from the FP32 boxes, and
+// an equal-t hit goes to whichever primitive is reached first, so a ray that
+// ends with a tie is resolved again by the FP32 walk. To see such hits, a
+// primitive is tested against the float just above the nearest t; that
+// bound changes which hits are accepted, never the t, u, v computed.
+template <bool ANY, bool TIES = false, class NODE, class POP, class COUNT>
 RM_D Hit walk(const DevScene& S, V3 o, V3 d, float t_max, uint32_t& first, uint32_t& count, uint32_t& sp,
-              NODE&& node, POP&& pop, COUNT& cnt) {
+              NODE&& node, POP&& pop, COUNT& cnt, bool* tie = nullptr) {
     Hit hit = {t_max, NONE, 0.0f, 0.0f};
     if constexpr (ANY) {
         for (;;) {
@@ -384,8 +394,22 @@ RM_D Hit walk(const DevScene& S, V3 o, V3 d, float t_max, uint32_t& first, uint3
             if (done) break;
             for (uint32_t i = first; i < first + count; i++) {
                 cnt.prims(1);
-                V3 h = intersect(S, i, o, d, hit.t);
-                if (h.x > 0.0f) hit = {h.x, i, h.y, h.z};
+                if constexpr (TIES) {
+                    // Positive and finite: the float just above is one ulp up.
+                    const float bound = hit.prim == NONE ? hit.t : __uint_as_float(__float_as_uint(hit.t) + 1u);
+                    V3 h = intersect(S, i, o, d, bound);
+                    if (h.x > 0.0f) {
+                        if (h.x < hit.t || hit.prim == NONE) {
+                            hit = {h.x, i, h.y, h.z};
+                            *tie = false;
+                        } else {
+                            *tie = true;
+                        }
+                    }
+                } else {
+                    V3 h = intersect(S, i, o, d, hit.t);
+                    if (h.x > 0.0f) hit = {h.x, i, h.y, h.z};
+                }
             }
             if (sp == 0u) break;
             pop();
@@ -434,6 +458,26 @@ RM_D Hit trace_fp32(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, CO
         count = e.y;
     };
     return walk<ANY>(S, o, d, t_max, first, count, sp, node, pop, cnt);
+}
+
+// A quantized tree's walk. Any-hit rays need no care: the decoded boxes
+// enclose the FP32 ones, so every primitive the FP32 walk could report is
+// reachable and occlusion agrees. A closest-hit ray that ends on an exact-t
+// tie is traced again over the FP32 nodes (resident beside the quantized
+// ones), which picks the primitive the FP32 walk picks; the walk is done,
+// so its stack is free to reuse.
+template <bool ANY, class NODE, class POP, class STACK, class COUNT>
+RM_D Hit walk_resolving_ties(const DevScene& S, V3 o, V3 d, float t_max, uint32_t& first, uint32_t& count,
+                             uint32_t& sp, NODE&& node, POP&& pop, STACK& stack, COUNT& cnt) {
+    if constexpr (ANY) {
+        return walk<true>(S, o, d, t_max, first, count, sp, node, pop, cnt);
+    } else {
+        bool tie = false;
+        Hit h = walk<false, true>(S, o, d, t_max, first, count, sp, node, pop, cnt, &tie);
+        if (!tie) return h;
+        Stack<STACK::lds_depth, STACK::block, TREE_FP32> flat{stack.lds, stack.spill, stack.stride};
+        return trace_fp32<false>(S, o, d, t_max, flat, cnt);
+    }
 }
 
 // One axis of a quantized child box: origin + q·step, q·step exact (an
@@ -502,7 +546,7 @@ RM_D Hit trace_q(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT
         const uint2 e = stack.get(sp);
         go(e.x, e.y);
     };
-    return walk<ANY>(S, o, d, t_max, first, count, sp, node, pop, cnt);
+    return walk_resolving_ties<ANY>(S, o, d, t_max, first, count, sp, node, pop, stack, cnt);
 }
 
 // Byte i (a constant once unrolled) of six packed words.
@@ -582,7 +626,7 @@ RM_D Hit trace_w(const DevScene& S, V3 o, V3 d, float t_max, STACK& stack, COUNT
         first = en.x;
         count = en.y;
     };
-    return walk<ANY>(S, o, d, t_max, first, count, sp, node, pop, cnt);
+    return walk_resolving_ties<ANY>(S, o, d, t_max, first, count, sp, node, pop, stack, cnt);
 }
 
 // The walk the stack's type asks for.

@@ -7,6 +7,7 @@
 // memory. A job's frames are exactly the frames rm_frame renders on a
 // context of its own, so its image does not depend on what else runs.
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -46,6 +47,12 @@ struct Job {
     float radius0 = 0.0f;
     uint64_t bytes = 0;
     int64_t options[RM_OPTION_COUNT] = {};   // the server's options when the job was submitted
+    bool has_deadline = false;
+    Clock::time_point deadline;
+    // Set by rm_server_cancel (CANCEL_ASKED) or on a missed deadline
+    // (CANCEL_LATE): no more frames are queued, and the job ends with
+    // RM_ERR_CANCELLED once those in flight are done.
+    std::atomic<int> cancel{0};
     // progress, owned by the scheduler thread until `finished`
     uint32_t frames_enqueued = 0, frames_done = 0;
     Clock::time_point t_submit, t_admit, t_first, t_done;
@@ -65,13 +72,28 @@ struct Slot {
     size_t pinned_floats = 0;
 };
 
-// Device memory a job's context needs, by the buffers rm_frame sizes for it.
-uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, bool compensated) {
+constexpr int CANCEL_ASKED = 1, CANCEL_LATE = 2;
+
+std::string cancel_message(int why) {
+    return why == CANCEL_LATE ? "it missed its deadline." : "it was cancelled.";
+}
+
+// Device memory a job's context needs, by the buffers rm_frame (and rm_tune)
+// size for it. options: the job's RM_OPTION_* values.
+uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options, uint64_t env_bytes) {
     const uint64_t px = (uint64_t)w * h;
+    const bool compensated = (options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0;
+    const int64_t schedule = options[RM_OPTION_SCHEDULE];
+    const bool tuned = options[RM_OPTION_AUTOTUNE] > 0;   // rm_tune tries every schedule
     // accum, two guides, two denoiser buffers, denoised, two splat buffers,
     // guides, albedo: 10 float4 a pixel, and the traversal spill.
-    uint64_t b = px * (10 * 16 + rm::SPILL_ENTRIES * 8);
+    uint64_t b = px * (10 * 16 + rm::SPILL_ENTRIES * 8) + env_bytes;
     if (compensated) b += px * 96;   // error terms and their folded sums
+    // The compacted schedule's staging records (16×16 tiles rounded up) and
+    // the wavefront schedule's path state and queues.
+    const uint64_t tiled = (uint64_t)((w + 15) / 16) * ((h + 15) / 16) * rm::BLOCK_THREADS;
+    if (tuned || schedule == RM_SCHEDULE_COMPACT) b += tiled * 48 + (tiled > px ? tiled - px : 0) * rm::SPILL_ENTRIES * 8;
+    if (tuned || schedule == RM_SCHEDULE_WAVEFRONT) b += rm::wavefront_bytes(px);
     if (caustics) {
         const uint64_t photons = rm::PHOTON_CAPACITY;
         b += px * 2 * 16;   // the two gathered-light buffers
@@ -102,6 +124,7 @@ struct rm_server {
     std::thread worker;
 
     void run();
+    void drop_queued();
     bool admit();
     bool progress(Slot& s, bool enqueue);
     bool enqueue_batched();
@@ -148,6 +171,28 @@ void rm_server::finish(Slot& s, int error, const std::string& message) {
     done.notify_all();
 }
 
+// Ends queued jobs that were cancelled or missed their deadline, without a slot.
+void rm_server::drop_queued() {
+    const Clock::time_point now = Clock::now();
+    std::lock_guard<std::mutex> lock(m);
+    bool dropped = false;
+    for (auto it = queue.begin(); it != queue.end();) {
+        Job& j = **it;
+        if (j.cancel == 0 && j.has_deadline && now > j.deadline) j.cancel = CANCEL_LATE;
+        if (j.cancel != 0) {
+            j.error = RM_ERR_CANCELLED;
+            j.message = cancel_message(j.cancel);
+            j.t_admit = j.t_done = now;
+            j.finished = true;
+            it = queue.erase(it);
+            dropped = true;
+        } else {
+            ++it;
+        }
+    }
+    if (dropped) done.notify_all();
+}
+
 // Moves queued jobs into free slots while their memory fits. Returns whether
 // any job was admitted.
 bool rm_server::admit() {
@@ -170,6 +215,11 @@ bool rm_server::admit() {
         j->t_admit = Clock::now();
         any = true;
         int r = RM_OK;
+        // A slot keeps the buffers of the largest job it ran; charged only
+        // this job's size, it could hold more than the budget allows. So a
+        // slot holding well more than this job needs gives its buffers back
+        // first (a slot about the right size keeps them: no reallocation).
+        if (rmi::context_bytes(s.ctx) > j->bytes + j->bytes / 4 + (64ull << 20)) r = rmi::trim(s.ctx);
         for (int k = 0; k < RM_OPTION_COUNT && r == RM_OK; ++k) r = rm_set_option(s.ctx, k, j->options[k]);
         if (r == RM_OK) r = rm_resize(s.ctx, j->width, j->height);
         if (r == RM_OK) {
@@ -201,6 +251,17 @@ bool rm_server::admit() {
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_PERSISTENT_BLOCKS, it->second[1]);
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_WAVES, it->second[2]);
         }
+        if (r == RM_OK) {
+            // The job is charged what its slot now holds, if that is more
+            // than the estimate (its frames only grow light tracing's buffers,
+            // which the estimate counts).
+            const uint64_t held = rmi::context_bytes(s.ctx);
+            std::lock_guard<std::mutex> lock(m);
+            if (held > j->bytes) {
+                used += held - j->bytes;
+                j->bytes = held;
+            }
+        }
         if (r != RM_OK) finish(s, r, rm_last_error());
     }
     return any;
@@ -212,6 +273,12 @@ bool rm_server::progress(Slot& s, bool enqueue) {
     if (!s.job) return false;
     Job& j = *s.job;
     bool changed = false;
+    // A deadline is missed only while frames remain to be queued; once the
+    // last is queued the job finishes with its image.
+    if (j.cancel == 0 && j.has_deadline && j.frames_enqueued < j.frames_total && Clock::now() > j.deadline)
+        j.cancel = CANCEL_LATE;
+    const int cancelled = j.frames_enqueued < j.frames_total ? j.cancel.load() : 0;
+    if (cancelled) enqueue = false;
     while (s.in_flight > 0) {
         hipError_t q = hipEventQuery(s.frame_ev[s.head]);
         if (q == hipErrorNotReady) break;
@@ -246,6 +313,10 @@ bool rm_server::progress(Slot& s, bool enqueue) {
         s.in_flight++;
         changed = true;
     }
+    if (cancelled && s.in_flight == 0) {
+        finish(s, RM_ERR_CANCELLED, cancel_message(cancelled));
+        return true;
+    }
     if (j.frames_done == j.frames_total && !j.readback) {
         const size_t floats = (size_t)j.width * j.height * 4;
         if (s.pinned_floats < floats) {
@@ -254,6 +325,8 @@ bool rm_server::progress(Slot& s, bool enqueue) {
             s.pinned_floats = 0;
             hipError_t e = hipHostMalloc((void**)&s.pinned, floats * 4, hipHostMallocDefault);
             if (e != hipSuccess) {
+                s.pinned = nullptr;
+                (void)hipGetLastError();
                 finish(s, RM_ERR_MEMORY, std::string("allocating pinned memory: ") + hipGetErrorString(e));
                 return true;
             }
@@ -299,7 +372,8 @@ bool rm_server::enqueue_batched() {
     for (;;) {
         std::vector<Slot*> ready;
         for (Slot& s : slots)
-            if (s.job && s.in_flight < 2 && s.job->frames_enqueued < s.job->frames_total) ready.push_back(&s);
+            if (s.job && s.in_flight < 2 && s.job->frames_enqueued < s.job->frames_total && s.job->cancel == 0)
+                ready.push_back(&s);
         if (ready.empty()) return changed;
         const int tree = rmi::tree_of(ready[0]->ctx, ready[0]->job->scene);
         std::vector<Slot*> group;
@@ -345,6 +419,7 @@ bool rm_server::enqueue_batched() {
 void rm_server::run() {
     (void)hipSetDevice(device);
     for (;;) {
+        drop_queued();
         bool changed = admit();
         bool active = false;
         bool batched;
@@ -420,6 +495,8 @@ int rm_server_submit(rm_server* sv, const rm_job* job, uint64_t* id) {
     if (j->width == 0 || j->height == 0 || j->spp == 0)
         return rmi::fail(RM_ERR_INVALID, "A job's parameters need a size and at least one sample per frame.");
     if (job->target_samples == 0) return rmi::fail(RM_ERR_INVALID, "A job needs a target of at least one sample.");
+    if (!(job->deadline_ms >= 0.0))
+        return rmi::fail(RM_ERR_INVALID, "A job's deadline is a number of milliseconds, 0 for none.");
     j->frames_total = (job->target_samples + j->spp - 1) / j->spp;
     if (j->caustics) {
         if (!job->emitters || job->emitter_vec4s < 2)
@@ -440,12 +517,19 @@ int rm_server_submit(rm_server* sv, const rm_job* job, uint64_t* id) {
         std::lock_guard<std::mutex> lock(sv->m);
         std::memcpy(j->options, sv->options, sizeof j->options);
     }
-    j->bytes = job_bytes(j->width, j->height, j->caustics,
-                         (j->options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0);
+    const uint64_t env_bytes = j->env_rgba.empty()
+                                   ? 0
+                                   : (uint64_t)j->env_w * j->env_h * 8 + (uint64_t)j->table_w * (j->table_h + 1) * 4;
+    j->bytes = job_bytes(j->width, j->height, j->caustics, j->options, env_bytes);
     // Scene uploads on the home context must have landed before any slot reads them.
     RM_HIP(hipSetDevice(sv->device), "selecting the device");
     RM_HIP(hipStreamSynchronize(sv->home->stream), "waiting for scene uploads");
     j->t_submit = Clock::now();
+    if (job->deadline_ms > 0.0) {
+        j->has_deadline = true;
+        j->deadline = j->t_submit + std::chrono::duration_cast<Clock::duration>(
+                                        std::chrono::duration<double, std::milli>(job->deadline_ms));
+    }
     {
         std::lock_guard<std::mutex> lock(sv->m);
         if (sv->stop) return rmi::fail(RM_ERR_STATE, "The server is closing.");
@@ -470,7 +554,7 @@ int rm_server_wait(rm_server* sv, uint64_t id, float* dst, size_t floats, rm_job
         sv->jobs.erase(it);
     }
     if (stats) {
-        stats->queued_ms = ms_between(j->t_submit, j->t_admit);
+        stats->queued_ms = ms_between(j->t_submit, j->t_admit);   // a job dropped from the queue: until then
         stats->first_frame_ms = j->frames_done ? ms_between(j->t_submit, j->t_first) : 0.0;
         stats->done_ms = ms_between(j->t_submit, j->t_done);
         stats->samples = j->frames_done * j->spp;
@@ -481,6 +565,19 @@ int rm_server_wait(rm_server* sv, uint64_t id, float* dst, size_t floats, rm_job
         if (floats > j->image.size()) return rmi::fail(RM_ERR_INVALID, "rm_server_wait asked for more than the image holds.");
         std::memcpy(dst, j->image.data(), floats * 4);
     }
+    return RM_OK;
+}
+
+int rm_server_cancel(rm_server* sv, uint64_t id) {
+    if (!sv) return rmi::fail(RM_ERR_INVALID, "rm_server_cancel needs a server.");
+    {
+        std::lock_guard<std::mutex> lock(sv->m);
+        auto it = sv->jobs.find(id);
+        if (it == sv->jobs.end()) return rmi::fail(RM_ERR_INVALID, "No job " + std::to_string(id) + " is known.");
+        int expected = 0;
+        it->second->cancel.compare_exchange_strong(expected, CANCEL_ASKED);
+    }
+    sv->wake.notify_all();
     return RM_OK;
 }
 

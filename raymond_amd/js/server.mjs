@@ -42,14 +42,18 @@ export class RenderServer {
      * Queues a job and returns its id: `scene` seen through `camera` at
      * width × height, until `targetSamples` samples per pixel, with the
      * Renderer's settings (maxBounces, samplesPerFrame, exposure,
-     * fireflyClamp, denoise, caustics).
+     * fireflyClamp, denoise, caustics). With `deadlineMs`, a job that has
+     * not queued its last frame that many ms after submission ends, and
+     * wait() rejects, as if cancelled then.
      */
     submit({ scene, camera, width, height, targetSamples, maxBounces = 8, samplesPerFrame = 1, exposure = 1.0,
-        fireflyClamp = 20.0, denoise = true, caustics = true }) {
+        fireflyClamp = 20.0, denoise = true, caustics = true, deadlineMs = 0 }) {
         if (this.#closed)
             throw new Error("The server is closed.");
         if (!(targetSamples >= 1))
             throw new RangeError("A job needs a target of at least one sample per pixel.");
+        if (!(deadlineMs >= 0))
+            throw new RangeError("A deadline is a number of milliseconds, 0 for none.");
         const state = { maxBounces, samplesPerFrame, exposure, fireflyClamp, view: "color", denoise, caustics, samples: 0 };
         const packed = packForDevice(scene, camera, width, height, state);
         const handle = upload(this.#addon, this.#ctx, this.#scenes, scene, packed);
@@ -57,7 +61,7 @@ export class RenderServer {
         const hasMap = env.tableHeight > 0;
         const id = this.#addon.serverSubmit(this.#server, handle, packed.params, targetSamples,
             packed.causticsActive ? packed.emitters : null, hasMap ? env.rgba : null, env.width, env.height,
-            hasMap ? env.table : null, env.tableWidth, env.tableHeight);
+            hasMap ? env.table : null, env.tableWidth, env.tableHeight, deadlineMs);
         const frames = Math.ceil(targetSamples / samplesPerFrame);
         this.#sizes.set(id, { width, height, samples: frames * samplesPerFrame, denoise });
         return id;
@@ -79,6 +83,18 @@ export class RenderServer {
             for (let i = 0; i < pixels.length; ++i)
                 pixels[i] /= job.samples;
         return { pixels, stats };
+    }
+
+    /**
+     * Cancels job `id`: a queued job leaves the queue, a running one stops
+     * after the frames it has in flight; wait(id) then rejects with an
+     * Error saying it was cancelled. A job whose last frame is queued
+     * finishes with its image.
+     */
+    cancel(id) {
+        if (!this.#sizes.has(id))
+            throw new Error(`No job ${id} is waiting.`);
+        this.#addon.serverCancel(this.#server, id);
     }
 
     /** Waits for every job, then frees the server and its scenes. */

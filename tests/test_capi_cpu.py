@@ -53,7 +53,7 @@ def test_addon_exports():
     assert out.stdout.strip().split(",") == sorted(
         ["open", "close", "uploadScene", "releaseScene", "uploadTextures", "uploadEnvironment", "resize", "frame",
          "read", "synchronize", "display", "raycast", "deviceInfo", "bvhBuild", "setOption", "serverSetOption",
-         "tune", "serverOpen", "serverContext", "serverSubmit", "serverWait", "serverClose"])
+         "tune", "serverOpen", "serverContext", "serverSubmit", "serverWait", "serverClose", "serverCancel"])
 
 
 def test_oracle_is_not_linked_into_the_product():
@@ -73,6 +73,7 @@ def test_server_rejects_null_arguments_without_touching_a_device():
     assert lib.rm_server_submit(None, None, None) == -1
     assert lib.rm_server_wait(None, 1, None, 0, None) == -1
     assert lib.rm_server_close(None) == -1
+    assert lib.rm_server_cancel(None, 1) == -1
 
 
 def test_bvh_build_on_the_host():
