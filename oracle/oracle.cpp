@@ -639,14 +639,15 @@ inline float ggx_alpha2(float r) { float a = fmax_(r * r, 0.002f); return a * a;
 inline float ggx_d(float nh, float a2) { float k = nh * nh * (a2 - 1.0f) + 1.0f; return a2 / (PI * k * k); }
 inline float smith_g1(float nv, float a2) { return 2.0f * nv / (nv + std::sqrt(a2 + (1.0f - a2) * nv * nv)); }
 
-V3 sample_ggx(Rng& g, V3 n, float a2) {
-    V2 u = g.rand2();
+// sample_ggx with its two uniforms given (src/shaders.mjs:728-736).
+V3 ggx_half(V3 n, float a2, V2 u) {
     float r = u.x;
     float cos_t = std::sqrt((1.0f - r) / (1.0f + (a2 - 1.0f) * r));
     float sin_t = std::sqrt(fmax_(0.0f, 1.0f - cos_t * cos_t));
     float phi = TWO_PI * u.y;
     return normalize(fmul(frame(n), {sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t}));
 }
+V3 sample_ggx(Rng& g, V3 n, float a2) { return ggx_half(n, a2, g.rand2()); }
 
 inline V3 schlick(V3 f0, float c) {
     float k = std::pow(1.0f - clampf(c, 0.0f, 1.0f), 5.0f);
@@ -1894,6 +1895,33 @@ void oracle_eval_bsdf(const uint32_t* material, const float* base, const float* 
 }
 
 float oracle_fresnel_dielectric(float cos_i, float eta) { return fresnel_dielectric(cos_i, eta); }
+
+void oracle_ggx_terms(float roughness, float nh, float nv, float nl, float* out3) {
+    const float a2 = ggx_alpha2(roughness);
+    out3[0] = ggx_d(nh, a2);
+    out3[1] = smith_g1(nv, a2);
+    out3[2] = smith_g1(nl, a2);
+}
+
+void oracle_ggx_sample(float roughness, float u1, float u2, float* h3) {
+    V3 h = ggx_half({0.0f, 0.0f, 1.0f}, ggx_alpha2(roughness), {u1, u2});
+    h3[0] = h.x; h3[1] = h.y; h3[2] = h.z;
+}
+
+void oracle_sample_bsdf(const uint32_t* material, const float* base, const float* n, const float* wo,
+                        uint32_t seed, float* out9) {
+    const Mat& m = *reinterpret_cast<const Mat*>(material);
+    Rng g;
+    g.state = seed;
+    Surface s = {};
+    s.n = xyz(n);
+    s.ng = xyz(n);
+    s.front = true;
+    BsdfSample b = sample_bsdf(g, m, xyz(base), s, xyz(wo));
+    const float r[9] = {b.wi.x, b.wi.y, b.wi.z, b.weight.x, b.weight.y, b.weight.z, b.pdf, b.delta ? 1.0f : 0.0f,
+                        b.ok ? 1.0f : 0.0f};
+    for (int k = 0; k < 9; ++k) out9[k] = r[k];
+}
 
 void oracle_wavelength_rgb(float l, float* rgb) {
     V3 c = wavelength_rgb(l);

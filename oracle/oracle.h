@@ -103,6 +103,14 @@ void oracle_intersect(const OracleScene* s, uint32_t prim, const float* ray, flo
 void oracle_eval_bsdf(const uint32_t* material, const float* base, const float* n, const float* wo,
                       const float* wi, float* out4);
 float oracle_fresnel_dielectric(float cos_i, float eta);
+/* GGX for a roughness (alpha = max(r², 0.002)): D(n·h), G1(n·v), G1(n·l). */
+void oracle_ggx_terms(float roughness, float nh, float nv, float nl, float* out3);
+/* sample_ggx's half vector about n = (0, 0, 1) for the uniforms (u1, u2). */
+void oracle_ggx_sample(float roughness, float u1, float u2, float* h3);
+/* sample_bsdf on a front surface with n = ng, drawing from a PCG stream
+ * started at seed: wi[3], weight[3], pdf, delta (0/1), ok (0/1). */
+void oracle_sample_bsdf(const uint32_t* material, const float* base, const float* n, const float* wo,
+                        uint32_t seed, float* out9);
 void oracle_wavelength_rgb(float l, float* rgb);
 float oracle_env_pdf(const OracleScene* s, const uint8_t* params, const float* d);
 void oracle_env_radiance(const OracleScene* s, const uint8_t* params, const float* d, float* rgb);

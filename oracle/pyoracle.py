@@ -63,6 +63,9 @@ def load() -> C.CDLL:
             "oracle_intersect": (None, [P, U, P, C.c_float, P]),
             "oracle_eval_bsdf": (None, [P, P, P, P, P, P]),
             "oracle_fresnel_dielectric": (C.c_float, [C.c_float, C.c_float]),
+            "oracle_ggx_terms": (None, [C.c_float, C.c_float, C.c_float, C.c_float, P]),
+            "oracle_ggx_sample": (None, [C.c_float, C.c_float, C.c_float, P]),
+            "oracle_sample_bsdf": (None, [P, P, P, P, U, P]),
             "oracle_wavelength_rgb": (None, [C.c_float, P]),
             "oracle_env_pdf": (C.c_float, [P, P, P]),
             "oracle_env_radiance": (None, [P, P, P, P]),
@@ -229,6 +232,27 @@ def eval_bsdf(material_words: np.ndarray, base, n, wo, wi) -> np.ndarray:
     out = np.zeros(4, np.float32)
     args = [np.ascontiguousarray(x, np.float32) for x in (base, n, wo, wi)]
     load().oracle_eval_bsdf(_p(np.ascontiguousarray(material_words, np.uint32)), *[_p(a) for a in args], _p(out))
+    return out
+
+
+def ggx_terms(roughness: float, nh: float, nv: float, nl: float) -> np.ndarray:
+    """(D(n·h), G1(n·v), G1(n·l)) for a roughness."""
+    out = np.zeros(3, np.float32)
+    load().oracle_ggx_terms(roughness, nh, nv, nl, _p(out))
+    return out
+
+
+def ggx_sample(roughness: float, u1: float, u2: float) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    load().oracle_ggx_sample(roughness, u1, u2, _p(out))
+    return out
+
+
+def sample_bsdf(material_words: np.ndarray, base, n, wo, seed: int) -> np.ndarray:
+    """(wi[3], weight[3], pdf, delta, ok) of one sample_bsdf draw."""
+    out = np.zeros(9, np.float32)
+    args = [np.ascontiguousarray(x, np.float32) for x in (base, n, wo)]
+    load().oracle_sample_bsdf(_p(np.ascontiguousarray(material_words, np.uint32)), *[_p(a) for a in args], seed, _p(out))
     return out
 
 
