@@ -105,6 +105,17 @@ const SCENES = {
         Mesh.torus(Material.diffuse(WHITE).withTexture(GRADIENT, 3).withNormalMap(BUMPS, 2), { radius: 0.4, tube: 0.18, segments: 32, sides: 16 })
             .transformed(Mat4.translate(0.6, 0.35, 1.0).multiplyMat4(Mat4.rotateX(1.1))),
         new Quad(new Vec3(-0.6, 2.5, -0.6), new Vec3(1.2, 0, 0), new Vec3(0, 0, 1.2), Material.light(WHITE, 4.0))), TEXTURED_VIEW],
+    // A lamp with a pattern over a floor: a light (sampled directly) or a
+    // black diffuse quad glowing with the same pattern (found only by
+    // bouncing); the floor under it must come out alike, quarter by quarter.
+    patternedLamp: (as = "light") => [new Scene({ sky: Sky.black() }).add(floor(),
+        new Quad(new Vec3(-1.0, 1.2, -1.5), new Vec3(2.0, 0, 0), new Vec3(0, 0, 1.4),
+            (as == "light" ? Material.light(WHITE) : Material.diffuse(BLACK)).withEmission(WHITE, 4, PATTERN))), LOOK_DOWN],
+    // A white strip lamp behind a glass ball, with Abbe number `abbe` (0:
+    // no dispersion): dispersion fringes the strip's edges with colour.
+    stripThroughGlass: (abbe = "0") => [new Scene({ sky: Sky.black() }).add(
+        new Quad(new Vec3(-0.3, -2, -3), new Vec3(0.6, 0, 0), new Vec3(0, 4, 0), Material.light(WHITE, 1.0)),
+        new Sphere(new Vec3(0, 0, 0), 1, Material.glass(1.5, WHITE, Number(abbe)))), FRONT],
     patternedLamps: () => [new Scene({ sky: Sky.black() }).add(
         new Quad(new Vec3(-1.5, -1, 0), new Vec3(1.4, 0, 0), new Vec3(0, 2, 0), Material.light(WHITE).withEmission(WHITE, 1, PATTERN)),
         new Sphere(new Vec3(0.8, 0, 0), 0.8, Material.light(WHITE).withEmission(WHITE, 1, PATTERN), { rotation: Mat4.rotateY(0.7) })),
