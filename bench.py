@@ -47,6 +47,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# Non-packed VALU issue peak of MI355X: 256 CUs × 4 SIMDs × 16 lanes × 2.4 GHz
+# = 39.3 T lane-operations/s (the datasheet's 157.3 TFLOP/s FP32 counts an
+# FMA as two flops and packed-FP32 issue, which this build does not use).
+VALU_PEAK_GLANE_OPS = 256 * 64 * 2.4
+SIMDS = 1024
 METRIC = "Msamples/s at 1080p 8-bounce; served p50/p99 latency vs plain, 1 MI355X"
 
 WORKLOADS = {
@@ -88,12 +93,70 @@ SERVING_POOL = [
 SERVING_SIZES = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
 
 
-def traffic_from_summary(path):
-    """HBM bytes per launch of the trace kernel, from a committed PMC summary."""
+def summary_figure(path, key):
+    """A per-launch figure (`key: value`) from a committed PMC summary."""
     if path is None or not Path(path).exists():
         return None
-    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", Path(path).read_text())
+    m = re.search(rf"{key}:\s*([0-9.eE+]+)", Path(path).read_text())
     return float(m.group(1)) if m else None
+
+
+def traffic_from_summary(path):
+    """HBM bytes per launch of the trace kernel, from a committed PMC summary."""
+    return summary_figure(path, "hbm_bytes_per_launch")
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu), for the CPU baseline."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        if m:
+            return m.group(1).strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    for line in Path("/proc/cpuinfo").read_text().splitlines():
+        if line.startswith("model name"):
+            return line.split(":", 1)[1].strip()
+    return "unknown"
+
+
+def roofline(kernel_s: float, algorithmic_bytes: float, summary) -> dict:
+    """The path pass against both rooflines that could bound it.
+
+    VALU: lane-operations per launch (VALU wave instructions × mean active
+    lanes, from SQ_INSTS_VALU, SQ_THREAD_CYCLES_VALU and SQ_ACTIVE_INST_VALU
+    in the committed PMC summary) per second of the launch, against the
+    non-packed issue peak. HBM: the measured FETCH_SIZE + WRITE_SIZE bytes of
+    a launch per second against 8 TB/s. `bound` names the larger fraction.
+    The algorithmic bytes (node, primitive, material and per-pixel records)
+    mostly hit L2; they are reported as the working-set rate, not as HBM."""
+    lane_ops = summary_figure(summary, "valu_lane_ops_per_launch")
+    issue = summary_figure(summary, "valu_insts_per_launch")
+    traffic = traffic_from_summary(summary)
+    hbm = None if traffic is None else traffic / kernel_s / 1e9
+    valu = None if lane_ops is None else lane_ops / kernel_s / 1e9
+    fracs = {"valu": None if valu is None else valu / VALU_PEAK_GLANE_OPS,
+             "hbm": None if hbm is None else hbm / HBM_PEAK_GBS}
+    if fracs["valu"] is not None and (fracs["hbm"] is None or fracs["valu"] >= fracs["hbm"]):
+        line = {"bound": "valu", "achieved": round(valu, 1), "peak": round(VALU_PEAK_GLANE_OPS, 1),
+                "unit": "Glane-op/s", "frac": round(fracs["valu"], 4)}
+    else:
+        alg = algorithmic_bytes / kernel_s / 1e9
+        line = {"bound": "hbm", "achieved": round(hbm if hbm is not None else alg, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round((hbm if hbm is not None else alg) / HBM_PEAK_GBS, 4)}
+    line["traffic"] = traffic
+    line["valu"] = None if valu is None else {
+        "achieved_glane_ops": round(valu, 1), "frac": round(fracs["valu"], 4),
+        "lanes_active_of_64": round(lane_ops / issue, 2) if issue else None,
+        "issue_busy": round(issue * 4 / (SIMDS * 2.4e9 * kernel_s), 4) if issue else None}
+    line["hbm"] = None if hbm is None else {"achieved_gbs": round(hbm, 1), "frac": round(fracs["hbm"], 4)}
+    line["l2_working_set"] = {"algorithmic_bytes_per_launch": round(algorithmic_bytes),
+                              "gbs": round(algorithmic_bytes / kernel_s / 1e9, 1),
+                              "note": "node, primitive, material and per-pixel record bytes the traversal and "
+                                      "shading read; the scene stays in L2, so this is not HBM traffic"}
+    return line
 
 
 def algorithmic_bytes(p, params, stride: int) -> dict:
@@ -115,27 +178,41 @@ def cpu_threads() -> int:
     return int(os.environ.get("RAYMOND_CPU_THREADS", os.cpu_count() or 1))
 
 
-def cpu_baseline(p, params_for_frame, budget_s: float = 12.0) -> dict:
-    """The oracle's main() (camera paths only) over whole frames of the same
-    workload, as many as fit in about budget_s seconds of wall time."""
+def cpu_paths(p, params_for_frame, threads: int, budget_s: float, rows: int) -> tuple:
+    """Times the oracle's main() (camera paths) on bands of `rows` rows of
+    successive frames until about budget_s seconds: (samples, seconds, bands)."""
     from oracle.pyoracle import Oracle
     o = Oracle(p)
-    threads = cpu_threads()
     w, h = p.width, p.height
-    shape = (h, w, 4)
-    acc, ga, gn = (np.zeros(shape, np.float32) for _ in range(3))
-    frames = 0
+    acc, ga, gn = (np.zeros((h, w, 4), np.float32) for _ in range(3))
+    bands = 0
     t0 = time.perf_counter()
     while True:
-        o.main(params_for_frame(frames), acc, ga, gn, threads=threads)
-        frames += 1
+        frame, band = divmod(bands, max(h // rows, 1))
+        y0 = band * rows
+        o.main(params_for_frame(frame), acc, ga, gn, rows=(y0, min(h, y0 + rows)), threads=threads)
+        bands += 1
         dt = time.perf_counter() - t0
-        if dt >= budget_s or dt / frames * (frames + 1) > 2.5 * budget_s:
-            break
-    samples = frames * w * h
+        if dt >= budget_s:
+            return bands * rows * w, dt, bands
+
+
+def cpu_baseline(p, params_for_frame, budget_s: float = 10.0) -> dict:
+    """The oracle's main() (camera paths only, the same arithmetic and RNG
+    streams) on the host's cores, then on one: `value` is the all-cores
+    rate, `single_thread` the one-core rate, each over bands of whole rows
+    of the same 1080p frames for about budget_s seconds."""
+    threads = cpu_threads()
+    w, h = p.width, p.height
+    samples, dt, bands = cpu_paths(p, params_for_frame, threads, budget_s, rows=h // 4)
+    one, dt1, bands1 = cpu_paths(p, params_for_frame, 1, budget_s / 2, rows=8)
     return {"value": round(samples / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"oracle main() camera paths: {frames} whole {w}x{h} frames at 1 spp ({samples} samples, "
-                      f"{dt:.1f} s); light tracing, photon gather and denoise not included"}
+            "cpu_model": cpu_model(),
+            "sample": f"oracle main() camera paths over {bands} quarter-frames of {w}x{h} at 1 spp ({samples} "
+                      f"samples, {dt:.1f} s, {threads} threads); light tracing, photon gather and denoise not "
+                      f"included",
+            "single_thread": {"value": round(one / dt1 / 1e6, 4), "unit": "Msamples/s", "cores": 1,
+                              "sample": f"{bands1} bands of 8 rows ({one} samples, {dt1:.1f} s)"}}
 
 
 def bench_frames(args, name: str) -> dict:
@@ -193,7 +270,6 @@ def bench_frames(args, name: str) -> dict:
 
     alg = algorithmic_bytes(p, frame_params(0), args.stride or wl["stride"])
     bytes_per_launch = alg["per_sample"] * w * h * spp + 96 * w * h
-    achieved = bytes_per_launch / trace_avg_s / 1e9
     bounces = wl["bounces"]
     line = {
         "metric": METRIC if name == "spheres200-1920x1080" else f"Msamples/s at 1080p {bounces}-bounce, 1 MI355X",
@@ -215,17 +291,17 @@ def bench_frames(args, name: str) -> dict:
                    "path": "one job, megakernel, " + ("8-bit quantized BVH" if args.quantized_bvh else "FP32 BVH"),
                    "schedule": args.schedule, "waves_per_simd": tuning["best"][2] if tuned else args.waves,
                    "tuning": tuning},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_summary(wl["traffic"]),
-                     "kernel": {"compact": "k_main_compact", "per-pixel": "k_main",
-                                "persistent": "k_main_persistent", "wavefront": "the wavefront stages (k_wf_*)"}[
-                         tuning["best"][0] if tuned else args.schedule],
-                     "launch_ms": round(trace_avg_s * 1e3, 4),
-                     "algorithmic_bytes_per_launch": round(bytes_per_launch),
-                     "bytes_per_sample": round(alg["per_sample"], 1),
-                     "rays_per_sample": round(alg["rays_per_sample"], 3)},
+        "roofline": dict(roofline(trace_avg_s, bytes_per_launch, wl["traffic"]),
+                         kernel={"compact": "k_main_compact", "per-pixel": "k_main",
+                                 "persistent": "k_main_persistent", "wavefront": "the wavefront stages (k_wf_*)"}[
+                             tuning["best"][0] if tuned else args.schedule],
+                         launch_ms=round(trace_avg_s * 1e3, 4),
+                         bytes_per_sample=round(alg["per_sample"], 1),
+                         rays_per_sample=round(alg["rays_per_sample"], 3)),
         "stages_ms_per_frame": {s: round(ms / max(n, 1), 4) for s, (ms, n) in stages.items() if n},
     }
+    if name == "spheres200-1920x1080" and not args.no_serving:
+        line["served"] = served_vs_plain(args)
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(p, frame_params)
     return line
 
@@ -393,12 +469,53 @@ def run_plain(packs, trace) -> dict:
             "done_ms_p99": round(float(np.percentile(done, 99)), 2)}
 
 
+def serving_packs():
+    from raymond_amd import scenefile
+    return [scenefile.pack(c, dict(o, maxBounces=12, samplesPerFrame=1, fireflyClamp=0.0, denoise=True,
+                                   caustics=True)) for c, o in SERVING_POOL]
+
+
+def served_vs_plain(args) -> dict:
+    """The served half of the metric on a short trace: Poisson arrivals at
+    args.rate jobs/s for args.serve_seconds s over the scene pool, served
+    (8 slots, FP32), served in the form the served workload describes
+    (batched path passes, quantized BVH, FP16 guides, compensated sums), and
+    the plain path one job at a time; Msamples/s and p50/p99 time to the
+    first denoised frame and to the target samples of each."""
+    packs = serving_packs()
+    trace = serving_trace(args.rate, args.serve_seconds, args.spp_min, args.spp_max)
+    budget = 200 << 30
+    served = run_trace(packs, trace, args.slots, budget)
+    described = run_trace(packs, trace, args.slots, budget, precision_flags(args.served_precision), batch=True)
+    plain = run_plain(packs, trace)
+    # At the described rate the scaled-down trace is arrival-bound (both
+    # paths keep up with the arrivals); a rate that saturates both shows the
+    # capacity of each.
+    busy = serving_trace(args.capacity_rate, args.serve_seconds, args.spp_min, args.spp_max)
+    busy_served = run_trace(packs, busy, args.slots, budget)
+    busy_plain = run_plain(packs, busy)
+    keep = ("Msamples_per_s", "first_frame_ms_p50", "first_frame_ms_p99", "done_ms_p50", "done_ms_p99", "wall_s")
+    return {"trace": {"rate_jobs_per_s": args.rate, "arrivals_s": args.serve_seconds, "jobs": len(trace),
+                      "spp_target": [args.spp_min, args.spp_max], "bounces": [4, 12],
+                      "sizes": [f"{w}x{h}" for w, h in SERVING_SIZES],
+                      "scene_primitives": [p.primitive_count for p in packs],
+                      "scaled_down": f"{args.serve_seconds:g} s of arrivals and spp U{{{args.spp_min}..{args.spp_max}}} "
+                                     "rather than 600 s and U{16..1024}, so the run fits the bench's minute"},
+            "served": {k: served[k] for k in keep},
+            "served_batched_precision_variants": {k: described[k] for k in keep},
+            "plain": {k: plain[k] for k in keep},
+            "speedup_vs_plain": round(served["Msamples_per_s"] / max(plain["Msamples_per_s"], 1e-9), 3),
+            "capacity": {"rate_jobs_per_s": args.capacity_rate, "jobs": len(busy),
+                         "served": {k: busy_served[k] for k in keep}, "plain": {k: busy_plain[k] for k in keep},
+                         "speedup_vs_plain": round(busy_served["Msamples_per_s"] /
+                                                   max(busy_plain["Msamples_per_s"], 1e-9), 3)}}
+
+
 def bench_serving(args) -> dict:
     """A Poisson trace of jobs over the scene pool, served with concurrent
     slots and, on the same trace, one job at a time (the plain path)."""
-    from raymond_amd import capi, scenefile
-    packs = [scenefile.pack(c, dict(o, maxBounces=12, samplesPerFrame=1, fireflyClamp=0.0, denoise=True,
-                                    caustics=True)) for c, o in SERVING_POOL]
+    from raymond_amd import capi
+    packs = serving_packs()
     trace = serving_trace(args.rate, args.duration, args.spp_min, args.spp_max)
     budget = 200 << 30
     flags = precision_flags(args.served_precision)
@@ -458,6 +575,11 @@ def main() -> int:
     ap.add_argument("--spp-min", type=int, default=4, help="serving: smallest target spp")
     ap.add_argument("--spp-max", type=int, default=64, help="serving: largest target spp")
     ap.add_argument("--slots", type=int, default=8, help="serving: concurrent job slots")
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="default workload: seconds of arrivals of the short served-vs-plain trace")
+    ap.add_argument("--no-serving", action="store_true", help="default workload: skip the served-vs-plain trace")
+    ap.add_argument("--capacity-rate", type=float, default=60.0,
+                    help="default workload: arrival rate (jobs/s) of the saturating served-vs-plain trace")
     ap.add_argument("--served-precision", default="quantized-bvh,fp16-guides,compensated",
                     help="serving: precision variants of the batched variant run, comma-separated "
                          "(quantized-bvh, fp16-guides, compensated)")
