@@ -59,7 +59,10 @@ extern "C" {
                                           path-pass launch (rm_frame_batch); 0 (default): a launch per job */
 #define RM_OPTION_WAVES 5              /* waves per SIMD that RM_SCHEDULE_PIXEL's path pass is compiled for:
                                           4 (or 0, the default) or 5; the same results bit for bit */
-#define RM_OPTION_COUNT 6
+#define RM_OPTION_LIGHT_PATHS 6        /* light paths (and photon slots) a frame with light tracing; 0 (default):
+                                          the reference's min(W·H·spp, 524288). A budget past that cap changes
+                                          the image's light-traced part (more photons, less noise) */
+#define RM_OPTION_COUNT 7
 
 /* Reduced- and extended-precision variants (RM_OPTION_PRECISION flags). */
 #define RM_PRECISION_QUANTIZED_BVH 1   /* traverse the 8-bit child-quantized tree (rm_bvh_quantize) in the

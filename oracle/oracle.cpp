@@ -1168,7 +1168,10 @@ void main_pixel(const Ctx& C, uint32_t x, uint32_t y, float* a, float* ga, float
 
 // ---------------------------------------------------------------- light tracing
 // src/shaders.mjs:1425-1631, 1630-1831
-struct LightBuffers { uint32_t* splats; uint32_t* heads; float* photons; };
+// Light tracing's outputs, its light paths a frame (`paths`: photon ids
+// 0 .. paths - 1, the normalisation of each photon's power) and the photon
+// slots there is room for (`capacity`).
+struct LightBuffers { uint32_t* splats; uint32_t* heads; float* photons; uint32_t paths, capacity; };
 
 void splat(const Ctx& C, Rng& g, const LightBuffers& B, const Surface& s, const Mat& m, V3 base, V3 d, V3 beta) {
     const Params& P = *C.P;
@@ -1210,7 +1213,7 @@ inline uint32_t photon_cell_hash(int32_t cx, int32_t cy, int32_t cz) {
 
 void deposit(const Ctx& C, const LightBuffers& B, V3 p, V3 d, V3 beta) {
     uint32_t i = B.heads[PHOTON_TABLE]++;
-    if (i >= PHOTON_CAPACITY) return;
+    if (i >= B.capacity) return;
     V3 q = p / C.P->photon_cell;
     uint32_t h = photon_cell_hash(to_i32(std::floor(q.x)), to_i32(std::floor(q.y)), to_i32(std::floor(q.z)));
     uint32_t next = B.heads[h];
@@ -1256,7 +1259,7 @@ V3 gather(const Ctx& C, const uint32_t* heads, const float* photons, const Surfa
 void light_trace_one(const Ctx& C, const LightBuffers& B, uint32_t id) {
     const Params& P = *C.P;
     const Scene& S = C.S;
-    uint32_t photons = photon_count(P);
+    uint32_t photons = B.paths;
     if (id >= photons) return;
     Rng g;
     g.state = pcg(id ^ pcg(P.sample_offset * 2654435769u + 977u));
@@ -1716,8 +1719,16 @@ void oracle_path_stats(const OracleScene* s, const uint8_t* params, uint32_t str
 
 void oracle_light_trace(const OracleScene* s, const uint8_t* params, const float* emitters, uint32_t first,
                         uint32_t count, uint32_t* splats, uint32_t* photon_heads, float* photons) {
+    const Params& P = *reinterpret_cast<const Params*>(params);
+    oracle_light_trace_budget(s, params, emitters, first, count, photon_count(P), PHOTON_CAPACITY, splats,
+                              photon_heads, photons);
+}
+
+void oracle_light_trace_budget(const OracleScene* s, const uint8_t* params, const float* emitters, uint32_t first,
+                               uint32_t count, uint32_t paths, uint32_t capacity, uint32_t* splats,
+                               uint32_t* photon_heads, float* photons) {
     Ctx C{view(s), reinterpret_cast<const Params*>(params), emitters};
-    LightBuffers B{splats, photon_heads, photons};
+    LightBuffers B{splats, photon_heads, photons, paths, capacity};
     for (uint32_t id = first; id < first + count; ++id) light_trace_one(C, B, id);
 }
 

@@ -70,6 +70,11 @@ void oracle_path_stats(const OracleScene* s, const uint8_t* params, uint32_t str
 void oracle_light_trace(const OracleScene* s, const uint8_t* params, const float* emitters,
                         uint32_t first, uint32_t count, uint32_t* splats, uint32_t* photon_heads,
                         float* photons);
+/* The same with a light-path budget: `paths` light paths a frame (the
+ * normalisation of each photon's power) and room for `capacity` photons. */
+void oracle_light_trace_budget(const OracleScene* s, const uint8_t* params, const float* emitters,
+                               uint32_t first, uint32_t count, uint32_t paths, uint32_t capacity,
+                               uint32_t* splats, uint32_t* photon_heads, float* photons);
 void oracle_resolve_splats(const uint8_t* params, uint32_t* splats, float* accum);
 void oracle_photon_gather(const OracleScene* s, const uint8_t* params, const uint32_t* photon_heads,
                           const float* photons, float* accum, int threads);

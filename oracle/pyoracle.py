@@ -48,6 +48,7 @@ def load() -> C.CDLL:
             "oracle_main": (None, [P, P, P, P, P, U, U, C.c_int]),
             "oracle_path_stats": (None, [P, P, U, P]),
             "oracle_light_trace": (None, [P, P, P, U, U, P, P, P]),
+            "oracle_light_trace_budget": (None, [P, P, P, U, U, U, U, P, P, P]),
             "oracle_resolve_splats": (None, [P, P, P]),
             "oracle_photon_gather": (None, [P, P, P, P, P, C.c_int]),
             "oracle_prepare": (None, [P, P, P, P, P]),
@@ -152,9 +153,16 @@ class Oracle:
         keys = ("closest", "shadow", "nodes", "prims", "surfaces", "smooth", "samples")
         return dict(zip(keys, (int(c) for c in counts)))
 
-    def light_trace(self, params, emitters, first, count, splats, heads, photons):
-        self.lib.oracle_light_trace(self.ref, _p(params), _p(emitters), first, count, _p(splats), _p(heads),
-                                    _p(photons))
+    def light_trace(self, params, emitters, first, count, splats, heads, photons, paths=None, capacity=None):
+        """Light paths [first, first + count) of a frame of `paths` (default:
+        the reference's count), depositing into room for `capacity` photons."""
+        if paths is None:
+            self.lib.oracle_light_trace(self.ref, _p(params), _p(emitters), first, count, _p(splats), _p(heads),
+                                        _p(photons))
+        else:
+            self.lib.oracle_light_trace_budget(self.ref, _p(params), _p(emitters), first, count, paths,
+                                               paths if capacity is None else capacity, _p(splats), _p(heads),
+                                               _p(photons))
 
     def resolve_splats(self, params, splats, accum):
         self.lib.oracle_resolve_splats(_p(params), _p(splats), _p(accum))

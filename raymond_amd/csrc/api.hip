@@ -293,6 +293,8 @@ int rm_set_option(rm_context* c, int option, int64_t value) {
         return fail(RM_ERR_INVALID, "RM_OPTION_WAVES is 4 or 5 (0: the default, 4).");
     if (option == RM_OPTION_BATCH && (value < 0 || value > 1))
         return fail(RM_ERR_INVALID, "RM_OPTION_BATCH is 0 or 1.");
+    if (option == RM_OPTION_LIGHT_PATHS && (value < 0 || value > (1 << 28)))
+        return fail(RM_ERR_INVALID, "RM_OPTION_LIGHT_PATHS is 0 (the reference's count) to 2^28 a frame.");
     if (option == RM_OPTION_AUTOTUNE && (value < 0 || value > 64))
         return fail(RM_ERR_INVALID, "RM_OPTION_AUTOTUNE is 0 (off) to 64 frames per candidate.");
     if (option == RM_OPTION_PRECISION &&
@@ -498,7 +500,7 @@ struct Frame {
     uint32_t pair = 0;   // the splat and gather pair of a frame with light tracing
     // Set by frame_alloc: light tracing's photons and spill lanes; the path
     // pass's spill lanes, persistent grid, staging records, wavefront state.
-    uint32_t photons = 0, side_slots = 0;
+    uint32_t photons = 0, capacity = 0, side_slots = 0;
     uint32_t slots = 0, persistent = 0;
     float4* staging = nullptr;
     bool wavefront = false;
@@ -558,11 +560,14 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
            s->dispersive, tree_of(c, s)};
     const uint32_t px = F.px = c->width * c->height;
     if (F.caustics) {
-        uint32_t count = rm::PHOTON_CAPACITY;
-        if ((uint64_t)P.width * P.height * P.samples_per_frame < rm::PHOTON_CAPACITY)
-            count = P.width * P.height * P.samples_per_frame;
+        // Light paths and photon slots: the reference's count and fixed
+        // capacity, or the caller's budget for both.
+        const int64_t budget = c->options[RM_OPTION_LIGHT_PATHS];
+        const uint32_t count = budget > 0 ? (uint32_t)budget : rm::photon_count(P);
         F.photons = count;
-        F.side_slots = count > px ? count : px;
+        F.capacity = budget > 0 ? count : rm::PHOTON_CAPACITY;
+        const uint32_t light_lanes = count < rm::LIGHT_LANES ? count : rm::LIGHT_LANES;
+        F.side_slots = light_lanes > px ? light_lanes : px;
         const uint32_t pair = F.pair = c->pair;
         const size_t emitter_bytes = (size_t)emitter_vec4s * 16;
         const size_t spill_bytes = (size_t)F.side_slots * rm::SPILL_ENTRIES * sizeof(uint2);
@@ -574,7 +579,8 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
             c->gathered[pair].bytes < (size_t)px * 16 || c->side_spill.bytes < spill_bytes)
             RM_HIP(hipStreamSynchronize(c->side), "waiting for light tracing");
         RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
-        RM_HIP(c->photons.ensure((size_t)rm::PHOTON_CAPACITY * 48), "allocating photons");
+        if (c->photons.bytes < (size_t)F.capacity * 48) RM_HIP(hipStreamSynchronize(c->side), "waiting for light tracing");
+        RM_HIP(c->photons.ensure((size_t)F.capacity * 48), "allocating photons");
         RM_HIP(c->gathered[pair].ensure((size_t)px * 16), "allocating the gathered light");
         RM_HIP(c->side_spill.ensure(spill_bytes), "allocating the side traversal spill");
         if (!same_emitters) {
@@ -654,7 +660,7 @@ static int frame_begin(rm_context* c, Frame& F) {
             Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
             rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(),
                                    c->heads.as<uint32_t>(), c->photons.as<float4>(), c->side_spill.as<uint32_t>(),
-                                   F.side_slots, F.photons, f, c->side);
+                                   F.side_slots, F.photons, F.capacity, f, c->side);
         }
         {
             Timed t(c, RM_STAGE_GATHER, c->side);

@@ -112,9 +112,20 @@ void launch_wavefront(const LaunchWavefront& L, const Features& f, hipStream_t s
 int main_resident_blocks_per_cu(const Features& f);
 void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, int tree,
                     uint32_t* spill, uint32_t slots, uint2* counters, hipStream_t stream);
+// The reference's light paths (and photon slots) a frame: one a sample, at
+// most PHOTON_CAPACITY (src/shaders.mjs:1495-1497).
+__host__ __device__ inline uint32_t photon_count(const Params& P) {
+    const uint64_t n = (uint64_t)P.width * P.height * P.samples_per_frame;
+    return n < PHOTON_CAPACITY ? (uint32_t)n : PHOTON_CAPACITY;
+}
+
+// Light tracing: count light paths, photon ids 0 .. count - 1, each depositing
+// at most one photon into `photons` (room for `capacity`); the grid is
+// at most LIGHT_LANES threads, and `spill` has room for that many lanes.
+constexpr uint32_t LIGHT_LANES = 1u << 20;
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
-                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
-                        hipStream_t stream);
+                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, uint32_t capacity,
+                        const Features& f, hipStream_t stream);
 void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, float4* comp,
                     hipStream_t stream);
 // sum + error of the compensated sums, into out: planes accum, galb, gnorm of W·H float4.

@@ -5,9 +5,6 @@
 namespace rm {
 
 // ------------------------------------------------------------------ light tracing
-RM_D uint32_t photon_count(const Params& P) {
-    return min(P.width * P.height * P.samples_per_frame, PHOTON_CAPACITY);
-}
 RM_D uint32_t photon_cell_hash(int32_t cx, int32_t cy, int32_t cz) {
     return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & (PHOTON_TABLE - 1u);
 }
@@ -45,177 +42,187 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
     atomicAdd(&splats[index + 2u], to_u32(fx.z));
 }
 
-// One photon per thread (src/shaders.mjs:1630-1814). Deposits happen after
-// the bounce loop with one counter atomic per wave (ballot + prefix count):
-// photon slots are claimed in a different order than one atomic per lane
-// would give, which the reference leaves unspecified too.
+// Light paths (src/shaders.mjs:1630-1814), photons_n of them a frame: the
+// reference's photon_count(P) by default, or a budget the caller sets. A
+// block takes photon ids first, first + 1, ... with first advancing by the
+// grid's size, so a grid smaller than the budget covers it and the traversal
+// spill stays sized by the grid; photon id seeds its own PCG stream, so the
+// grid does not change a photon. Deposits happen after the bounce loop with
+// one counter atomic per wave (ballot + prefix count): photon slots are
+// claimed in a different order than one atomic per lane would give, which
+// the reference leaves unspecified too; slots at or past `capacity` are
+// dropped, as the reference drops them past its fixed capacity.
 template <bool TEXTURES, bool DISPERSION, int TREE>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Params P, const float4* __restrict__ E,
                                                                uint32_t* splats, uint32_t* heads,
                                                                float4* __restrict__ photons, uint32_t* spill,
-                                                               uint32_t slots) {
+                                                               uint32_t slots, uint32_t photons_n, uint32_t capacity) {
     __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     const Ctx C{S, &P};
-    const uint32_t id = blockIdx.x * BLOCK_THREADS + threadIdx.x;
-    const uint32_t photons_n = photon_count(P);
-    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, id, slots);
-    bool pending = false;
-    V3 dep_p = {0, 0, 0}, dep_d = {0, 0, 0}, dep_beta = {0, 0, 0};
+    const uint32_t lane = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, lane, slots);
+    // `first` is the same for every thread of a block, so whole waves take
+    // each pass and reach the deposit's ballot together.
+    for (uint32_t first = blockIdx.x * BLOCK_THREADS; first < photons_n; first += gridDim.x * BLOCK_THREADS) {
+        const uint32_t id = first + threadIdx.x;
+        bool pending = false;
+        V3 dep_p = {0, 0, 0}, dep_d = {0, 0, 0}, dep_beta = {0, 0, 0};
 
-    if (id < photons_n) {
-        Rng g;
-        g.qmc = false;
-        g.state = pcg(id ^ pcg(P.sample_offset * 2654435769u + 977u));
-        const float4 header = E[0];
-        const uint32_t sources = __float_as_uint(header.x), targets = __float_as_uint(header.y);
-        float xi = g.rand();
-        uint32_t k = 0u;
-        while (!(k + 1u >= sources || E[1u + k].x > xi)) k++;
-        const float4 entry = E[1u + k];
-        const float chance = entry.z;
-        const uint32_t prim_index = __float_as_uint(entry.y);
-        const uint32_t first_target = 1u + sources;
-        bool alive = chance > 0.0f;
-        V3 o = {0, 0, 0}, d = {0, 0, 0}, beta = {0, 0, 0};
-        if (alive && prim_index == NONE) {
-            d = -sample_cone(g, pv3(P.sun_dir), P.sun_omc);
-            float pick = g.rand() * header.w / PI;
-            uint32_t j = 0u;
-            for (;;) {
-                float r = E[first_target + j].w;
-                if (j + 1u >= targets || pick < r * r) break;
-                pick -= r * r;
-                j++;
-            }
-            float4 circle = E[first_target + j];
-            Frame f = frame(d);
-            V2 disk = concentric_disk(g);
-            disk = {disk.x * circle.w, disk.y * circle.w};
-            o = xyz(circle) - d * header.z + f.c0 * disk.x + f.c1 * disk.y;
-            float through = 1.0f;
-            for (uint32_t i = 0u; i < targets; i++) {
-                float4 c = E[first_target + i];
-                V3 q = o - xyz(c);
-                V3 across = q - d * dot(q, d);
-                through += (i != j && dot(across, across) < c.w * c.w) ? 1.0f : 0.0f;
-            }
-            V3 irradiance = pv3(P.sun_radiance) * (TWO_PI * P.sun_omc);
-            beta = irradiance * header.w / (through * chance * (float)photons_n);
-        } else if (alive) {
-            const float4* pr = S.prims + (size_t)prim_index * 4;
-            float4 hd = pr[0], p0 = pr[1], p1 = pr[2], p2 = pr[3];
-            Mat m = load_mat(S, __float_as_uint(hd.y));
-            V3 n;
-            float area;
-            V2 uv;
-            if (__float_as_uint(hd.x) == QUAD) {
-                float a = g.rand();
-                float b = g.rand();
-                o = xyz(p0) + xyz(p1) * a + xyz(p2) * b;
-                V3 c = cross(xyz(p1), xyz(p2));
-                area = length(c);
-                n = c / area;
-                uv = {a, 1.0f - b};
-            } else {
-                float z = 1.0f - 2.0f * g.rand();
-                float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-                float phi = TWO_PI * g.rand();
-                n = {r * cosf(phi), z, r * sinf(phi)};
-                o = xyz(p0) + n * p0.w;
-                area = FOUR_PI * p0.w * p0.w;
-                uv = sphere_uv(n, p1, p2);
-            }
-            float cones = 0.0f;
-            bool inside = false;
-            for (uint32_t i = 0u; i < targets; i++) {
-                float4 aim = E[first_target + i];
-                V3 to = xyz(aim) - o;
-                float d2 = dot(to, to);
-                inside = inside || d2 <= aim.w * aim.w;
-                cones += sphere_omc(aim.w, fmaxf(d2, aim.w * aim.w));
-            }
-            if (!inside) {
-                float pick = g.rand() * cones;
+        if (id < photons_n) {
+            Rng g;
+            g.qmc = false;
+            g.state = pcg(id ^ pcg(P.sample_offset * 2654435769u + 977u));
+            const float4 header = E[0];
+            const uint32_t sources = __float_as_uint(header.x), targets = __float_as_uint(header.y);
+            float xi = g.rand();
+            uint32_t k = 0u;
+            while (!(k + 1u >= sources || E[1u + k].x > xi)) k++;
+            const float4 entry = E[1u + k];
+            const float chance = entry.z;
+            const uint32_t prim_index = __float_as_uint(entry.y);
+            const uint32_t first_target = 1u + sources;
+            bool alive = chance > 0.0f;
+            V3 o = {0, 0, 0}, d = {0, 0, 0}, beta = {0, 0, 0};
+            if (alive && prim_index == NONE) {
+                d = -sample_cone(g, pv3(P.sun_dir), P.sun_omc);
+                float pick = g.rand() * header.w / PI;
                 uint32_t j = 0u;
                 for (;;) {
-                    float4 aim = E[first_target + j];
-                    V3 to = xyz(aim) - o;
-                    float omc = sphere_omc(aim.w, dot(to, to));
-                    if (j + 1u >= targets || pick < omc) {
-                        d = sample_cone(g, to / length(to), omc);
-                        break;
-                    }
-                    pick -= omc;
+                    float r = E[first_target + j].w;
+                    if (j + 1u >= targets || pick < r * r) break;
+                    pick -= r * r;
                     j++;
                 }
-                float within = 1.0f;
+                float4 circle = E[first_target + j];
+                Frame f = frame(d);
+                V2 disk = concentric_disk(g);
+                disk = {disk.x * circle.w, disk.y * circle.w};
+                o = xyz(circle) - d * header.z + f.c0 * disk.x + f.c1 * disk.y;
+                float through = 1.0f;
+                for (uint32_t i = 0u; i < targets; i++) {
+                    float4 c = E[first_target + i];
+                    V3 q = o - xyz(c);
+                    V3 across = q - d * dot(q, d);
+                    through += (i != j && dot(across, across) < c.w * c.w) ? 1.0f : 0.0f;
+                }
+                V3 irradiance = pv3(P.sun_radiance) * (TWO_PI * P.sun_omc);
+                beta = irradiance * header.w / (through * chance * (float)photons_n);
+            } else if (alive) {
+                const float4* pr = S.prims + (size_t)prim_index * 4;
+                float4 hd = pr[0], p0 = pr[1], p1 = pr[2], p2 = pr[3];
+                Mat m = load_mat(S, __float_as_uint(hd.y));
+                V3 n;
+                float area;
+                V2 uv;
+                if (__float_as_uint(hd.x) == QUAD) {
+                    float a = g.rand();
+                    float b = g.rand();
+                    o = xyz(p0) + xyz(p1) * a + xyz(p2) * b;
+                    V3 c = cross(xyz(p1), xyz(p2));
+                    area = length(c);
+                    n = c / area;
+                    uv = {a, 1.0f - b};
+                } else {
+                    float z = 1.0f - 2.0f * g.rand();
+                    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                    float phi = TWO_PI * g.rand();
+                    n = {r * cosf(phi), z, r * sinf(phi)};
+                    o = xyz(p0) + n * p0.w;
+                    area = FOUR_PI * p0.w * p0.w;
+                    uv = sphere_uv(n, p1, p2);
+                }
+                float cones = 0.0f;
+                bool inside = false;
                 for (uint32_t i = 0u; i < targets; i++) {
                     float4 aim = E[first_target + i];
                     V3 to = xyz(aim) - o;
-                    float omc = sphere_omc(aim.w, dot(to, to));
-                    within += (i != j && 1.0f - dot(d, to / length(to)) <= omc) ? 1.0f : 0.0f;
+                    float d2 = dot(to, to);
+                    inside = inside || d2 <= aim.w * aim.w;
+                    cones += sphere_omc(aim.w, fmaxf(d2, aim.w * aim.w));
                 }
-                float cos_out = dot(d, n);
-                if (cos_out <= 0.0f) alive = false;
-                else beta = emitted<TEXTURES>(S, m, uv) * (cos_out * area * 2.0f * PI * cones) /
-                            (within * chance * (float)photons_n);
-            } else {
-                d = cosine_hemisphere(g, n);
-                beta = emitted<TEXTURES>(S, m, uv) * (PI * area) / (chance * (float)photons_n);
+                if (!inside) {
+                    float pick = g.rand() * cones;
+                    uint32_t j = 0u;
+                    for (;;) {
+                        float4 aim = E[first_target + j];
+                        V3 to = xyz(aim) - o;
+                        float omc = sphere_omc(aim.w, dot(to, to));
+                        if (j + 1u >= targets || pick < omc) {
+                            d = sample_cone(g, to / length(to), omc);
+                            break;
+                        }
+                        pick -= omc;
+                        j++;
+                    }
+                    float within = 1.0f;
+                    for (uint32_t i = 0u; i < targets; i++) {
+                        float4 aim = E[first_target + i];
+                        V3 to = xyz(aim) - o;
+                        float omc = sphere_omc(aim.w, dot(to, to));
+                        within += (i != j && 1.0f - dot(d, to / length(to)) <= omc) ? 1.0f : 0.0f;
+                    }
+                    float cos_out = dot(d, n);
+                    if (cos_out <= 0.0f) alive = false;
+                    else beta = emitted<TEXTURES>(S, m, uv) * (cos_out * area * 2.0f * PI * cones) /
+                                (within * chance * (float)photons_n);
+                } else {
+                    d = cosine_hemisphere(g, n);
+                    beta = emitted<TEXTURES>(S, m, uv) * (PI * area) / (chance * (float)photons_n);
+                }
+                o = o + n * (1e-4f * fmaxf(1.0f, fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)))));
             }
-            o = o + n * (1e-4f * fmaxf(1.0f, fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)))));
+
+            float wavelength = 0.0f;
+            uint32_t bounces = 0u;
+            V3 carried = {1.0f, 1.0f, 1.0f};
+            NoCount nc;
+            for (uint32_t depth = 0u; alive && depth < P.max_bounces; depth++) {
+                Hit hit = trace<false>(S, o, d, INF, stack, nc);
+                if (hit.prim == NONE) break;
+                Surface s = surface<TEXTURES>(S, o, d, hit);
+                Mat m = load_mat(S, s.material);
+                if (m.kind == LIGHT) break;
+                V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
+                if (!is_delta(m)) {
+                    if (bounces > 0u) {
+                        splat(C, g, stack, splats, s, m, base, d, beta * carried);
+                        pending = true;
+                        dep_p = s.p;
+                        dep_d = d;
+                        dep_beta = beta * carried;
+                    }
+                    break;
+                }
+                float u = g.rand();
+                m.ior = dispersed_ior<DISPERSION>(m, wavelength, carried, u);
+                BsdfSample bs = sample_bsdf(g, m, base, s, -d);
+                if (!bs.ok) break;
+                carried = carried * bs.weight;
+                bounces++;
+                o = offset(s, bs.wi);
+                d = bs.wi;
+                if (!roulette(g, carried, depth)) break;
+            }
         }
 
-        float wavelength = 0.0f;
-        uint32_t bounces = 0u;
-        V3 carried = {1.0f, 1.0f, 1.0f};
-        NoCount nc;
-        for (uint32_t depth = 0u; alive && depth < P.max_bounces; depth++) {
-            Hit hit = trace<false>(S, o, d, INF, stack, nc);
-            if (hit.prim == NONE) break;
-            Surface s = surface<TEXTURES>(S, o, d, hit);
-            Mat m = load_mat(S, s.material);
-            if (m.kind == LIGHT) break;
-            V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
-            if (!is_delta(m)) {
-                if (bounces > 0u) {
-                    splat(C, g, stack, splats, s, m, base, d, beta * carried);
-                    pending = true;
-                    dep_p = s.p;
-                    dep_d = d;
-                    dep_beta = beta * carried;
-                }
-                break;
-            }
-            float u = g.rand();
-            m.ior = dispersed_ior<DISPERSION>(m, wavelength, carried, u);
-            BsdfSample bs = sample_bsdf(g, m, base, s, -d);
-            if (!bs.ok) break;
-            carried = carried * bs.weight;
-            bounces++;
-            o = offset(s, bs.wi);
-            d = bs.wi;
-            if (!roulette(g, carried, depth)) break;
-        }
+        // deposit() (src/shaders.mjs:1518-1528), one counter atomic per wave.
+        uint64_t mask = __ballot(pending);
+        if (mask == 0) continue;
+        uint32_t wave_lane = __lane_id();
+        uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        uint32_t base = 0;
+        if (wave_lane == leader) base = atomicAdd(&heads[PHOTON_TABLE], (uint32_t)__popcll(mask));
+        base = __shfl(base, (int)leader);
+        if (!pending) continue;
+        uint32_t i = base + (uint32_t)__popcll(mask & ((1ull << wave_lane) - 1ull));
+        if (i >= capacity) continue;
+        V3 q = dep_p / P.photon_cell;
+        uint32_t h = photon_cell_hash(to_i32(floorf(q.x)), to_i32(floorf(q.y)), to_i32(floorf(q.z)));
+        uint32_t next = atomicExch(&heads[h], i + 1u);
+        photons[(size_t)i * 3] = make_float4(dep_p.x, dep_p.y, dep_p.z, __uint_as_float(next));
+        photons[(size_t)i * 3 + 1] = make_float4(dep_beta.x, dep_beta.y, dep_beta.z, 0.0f);
+        photons[(size_t)i * 3 + 2] = make_float4(-dep_d.x, -dep_d.y, -dep_d.z, 0.0f);
     }
-
-    // deposit() (src/shaders.mjs:1518-1528), one counter atomic per wave.
-    uint64_t mask = __ballot(pending);
-    if (mask == 0) return;
-    uint32_t lane = __lane_id();
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&heads[PHOTON_TABLE], (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    if (!pending) return;
-    uint32_t i = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (i >= PHOTON_CAPACITY) return;
-    V3 q = dep_p / P.photon_cell;
-    uint32_t h = photon_cell_hash(to_i32(floorf(q.x)), to_i32(floorf(q.y)), to_i32(floorf(q.z)));
-    uint32_t next = atomicExch(&heads[h], i + 1u);
-    photons[(size_t)i * 3] = make_float4(dep_p.x, dep_p.y, dep_p.z, __uint_as_float(next));
-    photons[(size_t)i * 3 + 1] = make_float4(dep_beta.x, dep_beta.y, dep_beta.z, 0.0f);
-    photons[(size_t)i * 3 + 2] = make_float4(-dep_d.x, -dep_d.y, -dep_d.z, 0.0f);
 }
 
 // resolve_splats() then photon_gather()'s sum into the accumulation
@@ -339,23 +346,29 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Par
 template <bool T, bool D>
 static void launch_light_trace_v(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
                                  uint32_t* heads, float4* photons, uint32_t* spill, uint32_t slots, dim3 grid,
-                                 int tree, hipStream_t stream) {
+                                 uint32_t count, uint32_t capacity, int tree, hipStream_t stream) {
     with_tree(tree, [&](auto t) {
         hipLaunchKernelGGL((k_light_trace<T, D, decltype(t)::value>), grid, dim3(BLOCK_THREADS), 0, stream, S, P,
-                           emitters, splats, heads, photons, spill, slots);
+                           emitters, splats, heads, photons, spill, slots, count, capacity);
     });
 }
 
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
-                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
-                        hipStream_t stream) {
-    dim3 grid((count + BLOCK_THREADS - 1) / BLOCK_THREADS);
+                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, uint32_t capacity,
+                        const Features& f, hipStream_t stream) {
+    // One photon a thread up to LIGHT_LANES, then each thread takes several.
+    const uint32_t lanes = count < LIGHT_LANES ? count : LIGHT_LANES;
+    dim3 grid((lanes + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    if (grid.x == 0) return;
     if (f.textures)
-        launch_light_trace_v<true, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.tree, stream);
+        launch_light_trace_v<true, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, count, capacity,
+                                         f.tree, stream);
     else if (f.dispersion)
-        launch_light_trace_v<false, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.tree, stream);
+        launch_light_trace_v<false, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, count,
+                                          capacity, f.tree, stream);
     else
-        launch_light_trace_v<false, false>(S, P, emitters, splats, heads, photons, spill, slots, grid, f.tree, stream);
+        launch_light_trace_v<false, false>(S, P, emitters, splats, heads, photons, spill, slots, grid, count,
+                                           capacity, f.tree, stream);
 }
 
 void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, float4* comp,

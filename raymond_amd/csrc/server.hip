@@ -95,10 +95,12 @@ uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options
     if (tuned || schedule == RM_SCHEDULE_COMPACT) b += tiled * 48 + (tiled > px ? tiled - px : 0) * rm::SPILL_ENTRIES * 8;
     if (tuned || schedule == RM_SCHEDULE_WAVEFRONT) b += rm::wavefront_bytes(px);
     if (caustics) {
-        const uint64_t photons = rm::PHOTON_CAPACITY;
+        const uint64_t budget = (uint64_t)options[RM_OPTION_LIGHT_PATHS];
+        const uint64_t photons = budget > rm::PHOTON_CAPACITY ? budget : rm::PHOTON_CAPACITY;
+        const uint64_t lanes = photons < rm::LIGHT_LANES ? photons : rm::LIGHT_LANES;
         b += px * 2 * 16;   // the two gathered-light buffers
-        b += (px > photons ? px : photons) * rm::SPILL_ENTRIES * 8;
-        b += (uint64_t)(rm::PHOTON_TABLE + 1) * 4 + (uint64_t)rm::PHOTON_CAPACITY * 48;
+        b += (px > lanes ? px : lanes) * rm::SPILL_ENTRIES * 8;
+        b += (uint64_t)(rm::PHOTON_TABLE + 1) * 4 + photons * 48;
     }
     return b;
 }

@@ -37,6 +37,9 @@ export function precisionFlags({ quantizedBvh = false, fp16Guides = false, compe
 /** RM_OPTION_WAVES: waves per SIMD of the one-thread-per-pixel path pass, 4 or 5. */
 export const OPTION_WAVES = 5;
 
+/** RM_OPTION_LIGHT_PATHS: light paths a frame with caustics on (0: the reference's count). */
+export const OPTION_LIGHT_PATHS = 6;
+
 /** RM_OPTION_SCHEDULE and its values. */
 export const OPTION_SCHEDULE = 0;
 export const SCHEDULES = Object.freeze({ "per-pixel": 0, compact: 1, persistent: 2, wavefront: 3 });

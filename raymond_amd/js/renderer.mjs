@@ -4,7 +4,8 @@
 // the frame itself (trace, light tracing, photon gather, denoise) runs as HIP
 // kernels in libraymond_hip.so, reached through the N-API addon raymond.node.
 // There is no other backend: without the addon or a GPU, create() rejects.
-import { OPTION_PRECISION, OPTION_SCHEDULE, OPTION_WAVES, SCHEDULES, loadAddon, precisionFlags } from "./addon.mjs";
+import { OPTION_LIGHT_PATHS, OPTION_PRECISION, OPTION_SCHEDULE, OPTION_WAVES, SCHEDULES, loadAddon, precisionFlags }
+    from "./addon.mjs";
 import {
     IMAGE_KEY_BYTES, PARAMS_BYTES, VIEWS, deltaObjects, emitters, environmentData,
     sceneTables, textureLayers, writeFrameParams, writeParams,
@@ -57,6 +58,7 @@ export class Renderer {
     #tables = null; #delta = null;
     #uploadedEnvironment = undefined;
     #width = 0; #height = 0;
+    #lightPaths = 0;
     #queue;
     #disposed = false;
 
@@ -132,6 +134,19 @@ export class Renderer {
             throw new RangeError("waves is 4 or 5.");
         this.#addon.setOption(this.#ctx, OPTION_WAVES, v);
         this.#waves = v;
+    }
+    /**
+     * Light paths (and photon slots) a frame when caustics are on: 0 (the
+     * default) for the reference's one a sample, at most 524,288; more
+     * traces more photons a frame, and each carries less power.
+     */
+    get lightPaths() { return this.#lightPaths; }
+    set lightPaths(v) {
+        if (!(Number.isInteger(v) && v >= 0 && v <= 2 ** 28))
+            throw new RangeError("lightPaths is a whole number from 0 to 2^28.");
+        this.#addon.setOption(this.#ctx, OPTION_LIGHT_PATHS, v);
+        this.#lightPaths = v;
+        this.#samples = 0;
     }
     get precision() { return Object.assign({}, this.#precision); }
     set precision(v) {

@@ -37,12 +37,13 @@ def gpu_render(ctx, p, frames: int = 1, params=None, denoised: bool = False):
         scene.release()
 
 
-def oracle_render(p, frames: int = 1, params=None, light=True, rows=None):
+def oracle_render(p, frames: int = 1, params=None, light=True, rows=None, paths=None):
     """The oracle's accumulation after `frames` frames (light tracing and
     photon gather included when the parameters turn caustics on). rows =
     (y0, y1) runs the camera paths of those rows only (light tracing and the
     gather still cover the image, since splats land anywhere): compare that
-    band."""
+    band. paths: a light-path budget (RM_OPTION_LIGHT_PATHS) rather than the
+    reference's count."""
     from oracle.pyoracle import PHOTON_CAPACITY, PHOTON_TABLE, Oracle
     o = Oracle(p)
     shape = (p.height, p.width, 4)
@@ -54,9 +55,13 @@ def oracle_render(p, frames: int = 1, params=None, light=True, rows=None):
         o.main(fp, acc, ga, gn, rows=rows)
         if caustics and light:
             heads = np.zeros(PHOTON_TABLE + 1, np.uint32)
-            photons = np.zeros(PHOTON_CAPACITY * 12, np.float32)
-            count = min(p.width * p.height * get_param(fp, "samples_per_frame"), PHOTON_CAPACITY)
-            o.light_trace(fp, p.emitters, 0, count, splats, heads, photons)
+            if paths is None:
+                photons = np.zeros(PHOTON_CAPACITY * 12, np.float32)
+                count = min(p.width * p.height * get_param(fp, "samples_per_frame"), PHOTON_CAPACITY)
+                o.light_trace(fp, p.emitters, 0, count, splats, heads, photons)
+            else:
+                photons = np.zeros(paths * 12, np.float32)
+                o.light_trace(fp, p.emitters, 0, paths, splats, heads, photons, paths=paths)
             o.resolve_splats(fp, splats, acc)
             o.photon_gather(fp, heads, photons, acc)
     return acc, ga, gn

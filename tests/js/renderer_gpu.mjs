@@ -55,6 +55,16 @@ async function main() {
         throw new Error("the renderer should take the tuned schedule and start the image again");
     renderer.render();
     if (!(await renderer.readPixels()).every(Number.isFinite)) throw new Error("frames after tuning should be finite");
+    // A light-path budget for the caustics, past the reference's one a pixel.
+    rejected = false;
+    try { renderer.lightPaths = -1; } catch (e) { rejected = e instanceof RangeError; }
+    if (!rejected) throw new Error("a negative light-path budget should be a RangeError");
+    renderer.caustics = true;
+    renderer.lightPaths = 50000;
+    renderer.render();
+    renderer.render();
+    if (renderer.lightPaths != 50000 || !(await renderer.readPixels()).every(Number.isFinite))
+        throw new Error("frames with a light-path budget should be finite");
     renderer.dispose();
 }
 

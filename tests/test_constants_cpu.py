@@ -28,7 +28,7 @@ ROOT = Path(__file__).resolve().parent.parent
 FIXTURE = json.loads((ROOT / "tests" / "golden" / "shader_constants.json").read_text())
 ORACLE = ROOT / "oracle" / "oracle.cpp"
 KERNELS = [ROOT / "raymond_amd" / "csrc" / f for f in ("device.hpp", "shading.hpp", "path.hpp", "light.hip",
-                                                        "kernels.hip")]
+                                                        "kernels.hip", "kernels.hpp")]
 
 CNUM = re.compile(r"(?<![\w.])(0[xX][0-9a-fA-F]+[uUlL]*|(?:\d+\.\d*|\.\d+|\d+)(?:[eE][+-]?\d+)?[fFuUlL]*)(?![\w.])")
 

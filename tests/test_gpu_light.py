@@ -83,3 +83,48 @@ def test_switching_scenes_on_one_context(pack):
         alone = gpu_render(fresh, b, 5, pb)[0]
     assert np.isfinite(switched).all()
     np.testing.assert_allclose(switched, alone, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("paths", [3 * 64 * 64, (1 << 20) + 12345], ids=["3x-pixels", "past-the-grid"])
+def test_light_path_budget_matches_oracle(ctx, pack, paths):
+    """RM_OPTION_LIGHT_PATHS: that many light paths a frame (and photon
+    slots), normalised by their number, against the oracle with the same
+    budget. 2^20 + 12345 paths pass the kernel's grid of 2^20 threads, so
+    threads take several photon ids each."""
+    from raymond_amd import capi
+    p = pack(*CAUSTIC_SCENES[0][:1], **CAUSTIC_SCENES[0][1])
+    params = p.with_params(caustics=1, denoise=0, firefly_clamp=0.0)
+    ctx.set_option(capi.OPTION_LIGHT_PATHS, paths)
+    try:
+        g = gpu_render(ctx, p, 2, params)[0]
+    finally:
+        ctx.set_option(capi.OPTION_LIGHT_PATHS, 0)
+    c = oracle_render(p, 2, params, paths=paths)[0]
+    plain = gpu_render(ctx, p, 2, params)[0]
+    frac = agreement(g, c, rel=1e-3, abs_=1e-4)
+    print(f"\n{paths} light paths a frame: {frac:.5f} of pixels agree")
+    assert frac >= 0.999
+    assert abs(g[..., :3].sum() - c[..., :3].sum()) <= 1e-3 * abs(c[..., :3].sum())
+    assert not np.array_equal(g, plain), "a budget past the reference's count traces more photons"
+
+
+def test_light_path_budget_converges_like_path_tracing(ctx, pack):
+    """64 times the reference's light paths a frame over 128 frames, normalised by their
+    number: the floor still converges to what path tracing alone finds
+    (whole floor within 4%, brightest decile within 8%, the reference's
+    thresholds)."""
+    from raymond_amd import capi
+    from test_gpu_closed_form import average, centre_hits, fixture, interior
+    p = fixture(pack, "causticLamp", 48)
+    ctx.set_option(capi.OPTION_LIGHT_PATHS, 64 * 48 * 48)
+    try:
+        on = gpu_render(ctx, p, 128, p.with_params(caustics=1, denoise=0, firefly_clamp=0.0))[0] / 128
+    finally:
+        ctx.set_option(capi.OPTION_LIGHT_PATHS, 0)
+    off = average(ctx, p, 16384, caustics=0)
+    owner, _, _ = centre_hits(p)
+    mask = interior(owner, 0)
+    a, b = on[mask][:, :3].mean(1), off[mask][:, :3].mean(1)
+    assert abs(a.mean() / b.mean() - 1) < 0.04, (a.mean(), b.mean())
+    top = b >= np.quantile(b, 0.9)
+    assert abs(a[top].mean() / b[top].mean() - 1) < 0.08, (a[top].mean(), b[top].mean())
