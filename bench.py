@@ -305,7 +305,8 @@ def bench_frames(args, name: str) -> dict:
                    "schedule": args.schedule, "waves_per_simd": tuning["best"][2] if tuned else args.waves,
                    "tuning": tuning},
         "roofline": dict(roofline(trace_avg_s, bytes_per_launch, wl["traffic"]),
-                         kernel={"compact": "k_main_compact", "per-pixel": "k_main",
+                         kernel={"compact": "k_main_compact", "sorted": "k_main_compact (sorting)",
+                                 "per-pixel": "k_main",
                                  "persistent": "k_main_persistent", "wavefront": "the wavefront stages (k_wf_*)"}[
                              tuning["best"][0] if tuned else args.schedule],
                          launch_ms=round(trace_avg_s * 1e3, 4),
@@ -596,7 +597,7 @@ def main() -> int:
     ap.add_argument("--served-precision", default="quantized-bvh,fp16-guides,compensated",
                     help="serving: precision variants of the batched variant run, comma-separated "
                          "(quantized-bvh, fp16-guides, compensated)")
-    ap.add_argument("--schedule", choices=["per-pixel", "compact", "persistent", "wavefront", "tuned"],
+    ap.add_argument("--schedule", choices=["per-pixel", "compact", "sorted", "persistent", "wavefront", "tuned"],
                     default="per-pixel",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
                          "or lanes regenerating paths from a tile queue")

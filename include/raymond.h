@@ -62,7 +62,15 @@ extern "C" {
 #define RM_OPTION_LIGHT_PATHS 6        /* light paths (and photon slots) a frame with light tracing; 0 (default):
                                           the reference's min(W·H·spp, 524288). A budget past that cap changes
                                           the image's light-traced part (more photons, less noise) */
-#define RM_OPTION_COUNT 7
+#define RM_OPTION_STACK_DEPTH 7        /* RM_SCHEDULE_PIXEL at four waves: traversal stack entries kept in LDS,
+                                          8 (or 0, the default) or 16; the same results bit for bit */
+#define RM_OPTION_BLOCK_SHAPE 8        /* RM_SCHEDULE_PIXEL at four waves: RM_BLOCK_SQUARES (0, the default;
+                                          16×16 tiles of 8×8 wave squares) or RM_BLOCK_ROWS (64×4 tiles, a wave
+                                          a 64-pixel row); the same results bit for bit */
+#define RM_OPTION_COUNT 9
+
+#define RM_BLOCK_SQUARES 0
+#define RM_BLOCK_ROWS 1
 
 /* Reduced- and extended-precision variants (RM_OPTION_PRECISION flags). */
 #define RM_PRECISION_QUANTIZED_BVH 1   /* traverse the 8-bit child-quantized tree (rm_bvh_quantize) in the
@@ -79,6 +87,9 @@ extern "C" {
 #define RM_SCHEDULE_PERSISTENT 2       /* persistent lanes that regenerate finished paths from a tile queue */
 #define RM_SCHEDULE_WAVEFRONT 3        /* one kernel per stage (extend, shade, shadow, finish) over queues of
                                           live paths, the paths' state in device memory between stages */
+#define RM_SCHEDULE_SORTED 4           /* as RM_SCHEDULE_COMPACT, and between bounces a block's live paths are
+                                          sorted by ray-direction octant and specular-or-not, so waves trace and
+                                          shade alike paths */
 
 typedef struct rm_context rm_context;
 typedef struct rm_scene rm_scene;
@@ -230,20 +241,24 @@ int rm_set_option(rm_context* ctx, int option, int64_t value);
 /* The autotuner. Renders `frames` whole frames after one warm-up with the
  * caller's parameters — light tracing (given its emitters) and the denoiser
  * included when they turn them on — under each candidate: one thread per
- * pixel at four and at five waves per SIMD, compacted, persistent at two grid
- * sizes, wavefront. It sets the fastest as the context's RM_OPTION_SCHEDULE,
- * RM_OPTION_PERSISTENT_BLOCKS and RM_OPTION_WAVES. Whole frames, because the
- * path pass shares the GPU with light tracing and the previous frame's
- * denoiser: a path pass that holds more waves can be faster alone and still
- * make the frame slower. The candidates give the same sums bit for bit, so
- * tuning changes speed only. The frames render into the context's buffers:
- * begin the image again (sample offset 0) afterwards. On failure the context
- * keeps the options it had. No counterpart in the reference. */
-#define RM_TUNE_CANDIDATES 6
+ * pixel at four and at five waves per SIMD, at four with 16 stack entries in
+ * LDS and with row-shaped blocks; compacted; sorted; persistent at two grid
+ * sizes; wavefront. It sets the fastest as the context's RM_OPTION_SCHEDULE,
+ * RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES, RM_OPTION_STACK_DEPTH and
+ * RM_OPTION_BLOCK_SHAPE. Whole frames, because the path pass shares the GPU
+ * with light tracing and the previous frame's denoiser: a path pass that
+ * holds more waves can be faster alone and still make the frame slower. The
+ * candidates give the same sums bit for bit, so tuning changes speed only.
+ * The frames render into the context's buffers: begin the image again
+ * (sample offset 0) afterwards. On failure the context keeps the options it
+ * had. No counterpart in the reference. */
+#define RM_TUNE_CANDIDATES 9
 typedef struct rm_tune_result {
     int32_t schedule[RM_TUNE_CANDIDATES];            /* each candidate's RM_SCHEDULE_* */
     uint32_t persistent_blocks[RM_TUNE_CANDIDATES];  /* and its persistent grid (0: fill the GPU) */
     uint32_t waves[RM_TUNE_CANDIDATES];              /* and its RM_OPTION_WAVES */
+    uint32_t stack_depth[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_STACK_DEPTH */
+    uint32_t block_shape[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_BLOCK_SHAPE */
     double ms[RM_TUNE_CANDIDATES];                   /* mean ms per whole frame */
     uint32_t count;                                  /* candidates tried */
     uint32_t best;                                   /* the one now set */

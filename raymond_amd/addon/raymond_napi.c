@@ -228,6 +228,10 @@ static napi_value candidate_object(napi_env env, const rm_tune_result* res, uint
     CHECK(napi_set_named_property(env, o, "persistentBlocks", v));
     CHECK(napi_create_uint32(env, res->waves[k], &v));
     CHECK(napi_set_named_property(env, o, "waves", v));
+    CHECK(napi_create_uint32(env, res->stack_depth[k], &v));
+    CHECK(napi_set_named_property(env, o, "stackDepth", v));
+    CHECK(napi_create_uint32(env, res->block_shape[k], &v));
+    CHECK(napi_set_named_property(env, o, "blockShape", v));
     CHECK(napi_create_double(env, res->ms[k], &v));
     CHECK(napi_set_named_property(env, o, "ms", v));
     return o;

@@ -284,13 +284,17 @@ int rm_open(int device, rm_context** out) {
 int rm_set_option(rm_context* c, int option, int64_t value) {
     if (!c) return fail(RM_ERR_INVALID, "rm_set_option needs a context.");
     if (option < 0 || option >= RM_OPTION_COUNT) return fail(RM_ERR_INVALID, "Unknown option " + std::to_string(option) + ".");
-    if (option == RM_OPTION_SCHEDULE && (value < RM_SCHEDULE_PIXEL || value > RM_SCHEDULE_WAVEFRONT))
+    if (option == RM_OPTION_SCHEDULE && (value < RM_SCHEDULE_PIXEL || value > RM_SCHEDULE_SORTED))
         return fail(RM_ERR_INVALID,
-                    "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT, _PERSISTENT or _WAVEFRONT.");
+                    "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT, _PERSISTENT, _WAVEFRONT or _SORTED.");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
     if (option == RM_OPTION_WAVES && value != 0 && value != 4 && value != 5)
         return fail(RM_ERR_INVALID, "RM_OPTION_WAVES is 4 or 5 (0: the default, 4).");
+    if (option == RM_OPTION_STACK_DEPTH && value != 0 && value != 8 && value != 16)
+        return fail(RM_ERR_INVALID, "RM_OPTION_STACK_DEPTH is 8 or 16 (0: the default, 8).");
+    if (option == RM_OPTION_BLOCK_SHAPE && value != RM_BLOCK_SQUARES && value != RM_BLOCK_ROWS)
+        return fail(RM_ERR_INVALID, "RM_OPTION_BLOCK_SHAPE is RM_BLOCK_SQUARES or RM_BLOCK_ROWS.");
     if (option == RM_OPTION_BATCH && (value < 0 || value > 1))
         return fail(RM_ERR_INVALID, "RM_OPTION_BATCH is 0 or 1.");
     if (option == RM_OPTION_LIGHT_PATHS && (value < 0 || value > (1 << 28)))
@@ -503,7 +507,7 @@ struct Frame {
     uint32_t photons = 0, capacity = 0, side_slots = 0;
     uint32_t slots = 0, persistent = 0;
     float4* staging = nullptr;
-    bool wavefront = false;
+    bool wavefront = false, sorted = false;
 };
 
 // The checks of rm_frame's arguments, before anything is enqueued.
@@ -603,6 +607,7 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
     F.persistent = 0;
     F.staging = nullptr;
     F.wavefront = false;
+    F.sorted = false;
     const int64_t schedule = path_buffers ? c->options[RM_OPTION_SCHEDULE] : RM_SCHEDULE_PIXEL;
     if (schedule == RM_SCHEDULE_PERSISTENT && P.view == 0u) {
         int64_t blocks = c->options[RM_OPTION_PERSISTENT_BLOCKS];
@@ -613,7 +618,8 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
         int r = ensure_spill(c, threads > px ? threads : px);
         if (r != RM_OK) return r;
         F.slots = threads;
-    } else if (schedule == RM_SCHEDULE_COMPACT && P.view == 0u) {
+    } else if ((schedule == RM_SCHEDULE_COMPACT || schedule == RM_SCHEDULE_SORTED) && P.view == 0u) {
+        F.sorted = schedule == RM_SCHEDULE_SORTED;
         const uint32_t threads = ((c->width + 15u) / 16u) * ((c->height + 15u) / 16u) * rm::BLOCK_THREADS;
         int r = ensure_spill(c, threads);
         if (r != RM_OK) return r;
@@ -717,7 +723,8 @@ static int frame_path(rm_context* c, Frame& F) {
         Timed t(c, RM_STAGE_TRACE);
         rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), F.comp,
                          c->spill.as<uint32_t>(), F.slots, F.persistent, c->work.as<uint32_t>(), F.staging, f.tree,
-                         c->options[RM_OPTION_WAVES] == 5 ? 5 : 4};
+                         c->options[RM_OPTION_WAVES] == 5 ? 5 : 4, F.sorted,
+                         c->options[RM_OPTION_STACK_DEPTH] == 16 ? 16 : 8, (int)c->options[RM_OPTION_BLOCK_SHAPE]};
         rm::launch_main(L, f, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the trace");
@@ -999,14 +1006,21 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
     std::memcpy(&P, params, sizeof P);
     const uint32_t spp = P.samples_per_frame > 0 ? P.samples_per_frame : 1u;
     const uint32_t cus = (uint32_t)(c->compute_units > 0 ? c->compute_units : 1);
-    const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_PIXEL,      RM_SCHEDULE_PIXEL,
-                                                  RM_SCHEDULE_COMPACT,    RM_SCHEDULE_PERSISTENT,
-                                                  RM_SCHEDULE_PERSISTENT, RM_SCHEDULE_WAVEFRONT};
-    const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 0u, 2u * cus, 0u};
-    const uint32_t waves[RM_TUNE_CANDIDATES] = {4u, 5u, 4u, 4u, 4u, 4u};
+    // The candidates: schedule, persistent blocks, waves, LDS stack depth, block shape.
+    const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_PIXEL,   RM_SCHEDULE_PIXEL,      RM_SCHEDULE_PIXEL,
+                                                  RM_SCHEDULE_PIXEL,   RM_SCHEDULE_COMPACT,    RM_SCHEDULE_SORTED,
+                                                  RM_SCHEDULE_PERSISTENT, RM_SCHEDULE_PERSISTENT, RM_SCHEDULE_WAVEFRONT};
+    const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 2u * cus, 0u};
+    const uint32_t waves[RM_TUNE_CANDIDATES] = {4u, 5u, 4u, 4u, 4u, 4u, 4u, 4u, 4u};
+    const uint32_t depths[RM_TUNE_CANDIDATES] = {8u, 8u, 16u, 8u, 8u, 8u, 8u, 8u, 8u};
+    const uint32_t shapes[RM_TUNE_CANDIDATES] = {RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_ROWS,
+                                                RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES,
+                                                RM_BLOCK_SQUARES, RM_BLOCK_SQUARES};
     rm_tune_result res = {};
-    const int64_t before[3] = {c->options[RM_OPTION_SCHEDULE], c->options[RM_OPTION_PERSISTENT_BLOCKS],
-                               c->options[RM_OPTION_WAVES]};
+    const int tuned[5] = {RM_OPTION_SCHEDULE, RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES, RM_OPTION_STACK_DEPTH,
+                          RM_OPTION_BLOCK_SHAPE};
+    int64_t before[5];
+    for (int k = 0; k < 5; ++k) before[k] = c->options[tuned[k]];
     auto finish_all = [&]() -> int {
         RM_HIP(hipStreamSynchronize(c->stream), "tuning");
         RM_HIP(hipStreamSynchronize(c->side), "tuning");
@@ -1019,6 +1033,8 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         c->options[RM_OPTION_SCHEDULE] = schedules[k];
         c->options[RM_OPTION_PERSISTENT_BLOCKS] = blocks[k];
         c->options[RM_OPTION_WAVES] = waves[k];
+        c->options[RM_OPTION_STACK_DEPTH] = depths[k];
+        c->options[RM_OPTION_BLOCK_SHAPE] = shapes[k];
         std::chrono::steady_clock::time_point t0;
         for (uint32_t f = 0; f <= frames && r == RM_OK; ++f) {
             if (f == 1) {
@@ -1035,6 +1051,8 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         res.schedule[k] = schedules[k];
         res.persistent_blocks[k] = blocks[k];
         res.waves[k] = waves[k];
+        res.stack_depth[k] = depths[k];
+        res.block_shape[k] = shapes[k];
         res.ms[k] = ms / frames;
         res.count = k + 1;
         if (k == 0 || res.ms[k] < best_ms) {
@@ -1043,9 +1061,9 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         }
     }
     const uint32_t pick = res.best;
-    c->options[RM_OPTION_SCHEDULE] = r == RM_OK ? res.schedule[pick] : before[0];
-    c->options[RM_OPTION_PERSISTENT_BLOCKS] = r == RM_OK ? res.persistent_blocks[pick] : before[1];
-    c->options[RM_OPTION_WAVES] = r == RM_OK ? res.waves[pick] : before[2];
+    const int64_t picked[5] = {res.schedule[pick], res.persistent_blocks[pick], res.waves[pick], res.stack_depth[pick],
+                               res.block_shape[pick]};
+    for (int k = 0; k < 5; ++k) c->options[tuned[k]] = r == RM_OK ? picked[k] : before[k];
     if (r != RM_OK) return r;
     if (out) *out = res;
     return RM_OK;

@@ -40,11 +40,17 @@ struct LaunchMain {
     float4* staging;             // non-null: k_main_compact, 3 float4 per thread of its grid
     int tree;                    // the tree the per-pixel pass walks
     int waves;                   // waves per SIMD the one-thread-per-pixel pass is compiled for: 4 or 5
+    bool sorted = false;         // with staging: k_main_compact sorting live paths by key between bounces
+    int stack_depth = 8;         // k_main at four waves: traversal stack entries in LDS, 8 or 16
+    int block_shape = 0;         // k_main at four waves: BLOCK_SQUARES (16×16) or BLOCK_ROWS (64×4)
 };
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
 // The one-thread-per-pixel pass compiled for five waves per SIMD (main5.hip).
 void launch_main_waves5(const LaunchMain& L, const Features& f, hipStream_t stream);
+// k_main at four waves with 16 stack entries in LDS, or with 64×4 row blocks (main_shapes.hip).
+void launch_main_stack16(const LaunchMain& L, const Features& f, hipStream_t stream);
+void launch_main_rows(const LaunchMain& L, const Features& f, hipStream_t stream);
 // The persistent and compacted schedules' launches (launch_main picks them).
 void launch_main_persistent(const LaunchMain& L, const Features& f, hipStream_t stream);
 void launch_main_compact(const LaunchMain& L, const Features& f, hipStream_t stream);

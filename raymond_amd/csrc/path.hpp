@@ -19,6 +19,26 @@ RM_D void tile_pixel_of(uint32_t t, uint32_t& x, uint32_t& y) {
 }
 RM_D void tile_pixel(uint32_t& x, uint32_t& y) { tile_pixel_of(threadIdx.x, x, y); }
 
+// Block shapes of the per-pixel pass (RM_OPTION_BLOCK_SHAPE): a 16×16 tile
+// of four 8×8 wave squares (BLOCK_SQUARES, the default: a wave's primary
+// rays stay close together), or a 64×4 tile of four 64-pixel wave rows
+// (BLOCK_ROWS: a wave's per-pixel loads and stores are contiguous).
+constexpr int BLOCK_SQUARES = 0, BLOCK_ROWS = 1;
+template <int SHAPE>
+RM_D void shaped_pixel(uint32_t& x, uint32_t& y) {
+    if constexpr (SHAPE == BLOCK_ROWS) {
+        x = blockIdx.x * 64u + (threadIdx.x & 63u);
+        y = blockIdx.y * 4u + (threadIdx.x >> 6);
+    } else {
+        tile_pixel(x, y);
+    }
+}
+template <int SHAPE>
+inline dim3 shaped_grid(uint32_t width, uint32_t height) {
+    if constexpr (SHAPE == BLOCK_ROWS) return dim3((width + 63) / 64, (height + 3) / 4);
+    return dim3((width + 15) / 16, (height + 15) / 16);
+}
+
 // Stencil passes (the denoiser's prepare) read a neighbourhood of pixels.
 // Workgroups are dealt to the 8 XCDs round-robin by linear block id, and each
 // XCD has its own L2, so a plain row-major grid makes every L2 fetch nearly
@@ -395,13 +415,15 @@ struct PixelSums {
 };
 
 // One pixel's samples of a frame: k_main's body, shared with k_main_batch.
-template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, int TREE>
+// LDS_DEPTH: traversal stack entries kept in LDS (lds has room for that many
+// per thread of the block).
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, int TREE, int LDS_DEPTH = MAIN_LDS_DEPTH>
 RM_D void main_pixel(const DevScene& S, const Params& P, uint32_t x, uint32_t y, float4* accum, float4* guide_albedo,
                      float4* guide_normal, float4* comp, uint32_t* spill, uint32_t slots, uint2* lds) {
     if (x >= P.width || y >= P.height) return;
     const Ctx C{S, &P};
     const uint32_t index = y * P.width + x;
-    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, index, slots);
+    auto stack = BlockStack<LDS_DEPTH, TREE>::make(lds, spill, index, slots);
     PixelSums sums;
     sums.clear();
     for (uint32_t k = 0; k < P.samples_per_frame; k++) {

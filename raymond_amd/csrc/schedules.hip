@@ -98,7 +98,24 @@ RM_D void slot_load_wf(const SlotStore& st, uint32_t slot, uint32_t sample, Path
     reseed(g, ps, pix, sample);
 }
 
-template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+// Sorting (SORT, RM_SCHEDULE_SORTED): between bounces, while more than one
+// wave's worth of paths is alive, the block's live paths are reordered
+// through the same LDS slots by a key: the octant of the next ray's
+// direction and whether the bounce that made it was specular (glass and
+// mirrors lead into chains of refraction, diffuse and glossy bounces into
+// light sampling). A wave then traces rays heading the same way, which visit
+// the tree's children in the same order, and shades surfaces reached the
+// same way. Lanes take slots by an LDS counting sort (a count per key, then
+// each key's run in key order); which lane a path lands on changes none of
+// its arithmetic, since its RNG state is re-derived from its pixel and
+// sample at every bounce, so the sums stay bit-identical.
+constexpr uint32_t SORT_KEYS = 16;
+RM_D uint32_t sort_key(const PathState& ps) {
+    return (ps.d.x < 0.0f ? 1u : 0u) | (ps.d.y < 0.0f ? 2u : 0u) | (ps.d.z < 0.0f ? 4u : 0u) |
+           (ps.last_delta ? 8u : 0u);
+}
+
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, bool SORT>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_main_compact(
     DevScene S, Params P, float4* __restrict__ accum, float4* __restrict__ guide_albedo,
     float4* __restrict__ guide_normal, float4* __restrict__ comp, uint32_t* spill, uint32_t slots,
@@ -106,6 +123,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
     __shared__ uint2 lds[COMPACT_LDS_DEPTH * BLOCK_THREADS];
     __shared__ uint32_t slot_words[SLOT_WORDS * BLOCK_THREADS];
     __shared__ uint32_t wave_live[BLOCK_THREADS / 64];
+    __shared__ uint32_t key_count[SORT_KEYS];
     const Ctx C{S, &P};
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t gid = (blockIdx.y * gridDim.x + blockIdx.x) * BLOCK_THREADS + t;
@@ -144,7 +162,20 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
                 nonempty += n > 0u ? 1u : 0u;
             }
             if (total == 0u) break;
-            if ((total + 63u) / 64u < nonempty) {
+            if (SORT && total > 64u) {
+                // Counting sort of the live paths by key into slots 0 .. total-1.
+                if (t < SORT_KEYS) key_count[t] = 0u;
+                __syncthreads();
+                const uint32_t key = live ? sort_key(ps) : 0u;
+                const uint32_t rank = live ? atomicAdd(&key_count[key], 1u) : 0u;
+                __syncthreads();
+                uint32_t start = 0;
+                for (uint32_t k2 = 0; k2 < SORT_KEYS; k2++) start += k2 < key ? key_count[k2] : 0u;
+                if (live) slot_save(st, start + rank, ps, gd, pix);
+                __syncthreads();
+                live = t < total;
+                if (live) slot_load(st, t, P, sample, ps, gd, g, pix);
+            } else if ((total + 63u) / 64u < nonempty) {
                 // Pack the live paths into lanes 0 .. total-1, in lane order.
                 if (live) slot_save(st, before + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)), ps, gd, pix);
                 __syncthreads();
@@ -470,8 +501,12 @@ void launch_main_persistent(const LaunchMain& L, const Features& f, hipStream_t 
 template <bool V, bool T, bool E, bool D>
 static void launch_compact_v(const LaunchMain& L, hipStream_t stream) {
     dim3 grid((L.P.width + 15) / 16, (L.P.height + 15) / 16);
-    hipLaunchKernelGGL((k_main_compact<V, T, E, D>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum, L.galb,
-                       L.gnorm, L.comp, L.spill, L.slots, L.staging);
+    if (L.sorted)
+        hipLaunchKernelGGL((k_main_compact<V, T, E, D, true>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P, L.accum,
+                           L.galb, L.gnorm, L.comp, L.spill, L.slots, L.staging);
+    else
+        hipLaunchKernelGGL((k_main_compact<V, T, E, D, false>), grid, dim3(BLOCK_THREADS), 0, stream, L.S, L.P,
+                           L.accum, L.galb, L.gnorm, L.comp, L.spill, L.slots, L.staging);
 }
 
 void launch_main_compact(const LaunchMain& L, const Features& f, hipStream_t stream) {

@@ -92,7 +92,7 @@ uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options
     // The compacted schedule's staging records (16×16 tiles rounded up) and
     // the wavefront schedule's path state and queues.
     const uint64_t tiled = (uint64_t)((w + 15) / 16) * ((h + 15) / 16) * rm::BLOCK_THREADS;
-    if (tuned || schedule == RM_SCHEDULE_COMPACT) b += tiled * 48 + (tiled > px ? tiled - px : 0) * rm::SPILL_ENTRIES * 8;
+    if (tuned || schedule == RM_SCHEDULE_COMPACT || schedule == RM_SCHEDULE_SORTED) b += tiled * 48 + (tiled > px ? tiled - px : 0) * rm::SPILL_ENTRIES * 8;
     if (tuned || schedule == RM_SCHEDULE_WAVEFRONT) b += rm::wavefront_bytes(px);
     if (caustics) {
         const uint64_t budget = (uint64_t)options[RM_OPTION_LIGHT_PATHS];
@@ -113,8 +113,8 @@ struct rm_server {
     int64_t options[RM_OPTION_COUNT] = {};   // given to jobs submitted from now on
     // The schedule rm_tune picked for each (scene, width, height, samples per
     // frame), with RM_OPTION_AUTOTUNE; used by the scheduler thread only.
-    // The options rm_tune set: schedule, persistent blocks, waves.
-    std::map<std::tuple<const rm_scene*, uint32_t, uint32_t, uint32_t>, std::array<int64_t, 3>> tuned;
+    // The options rm_tune set: schedule, persistent blocks, waves, stack depth, block shape.
+    std::map<std::tuple<const rm_scene*, uint32_t, uint32_t, uint32_t>, std::array<int64_t, 5>> tuned;
     rm_context* home = nullptr;
     std::vector<Slot> slots;
     std::mutex m;
@@ -246,12 +246,15 @@ bool rm_server::admit() {
                             j->caustics ? (uint32_t)(j->emitters.size() / 4) : 0u,
                             (uint32_t)j->options[RM_OPTION_AUTOTUNE], &res);
                 if (r == RM_OK)
-                    it = tuned.emplace(key, std::array<int64_t, 3>{res.schedule[res.best], res.persistent_blocks[res.best],
-                                                                   res.waves[res.best]}).first;
+                    it = tuned.emplace(key, std::array<int64_t, 5>{res.schedule[res.best], res.persistent_blocks[res.best],
+                                                                   res.waves[res.best], res.stack_depth[res.best],
+                                                                   res.block_shape[res.best]}).first;
             }
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_SCHEDULE, it->second[0]);
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_PERSISTENT_BLOCKS, it->second[1]);
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_WAVES, it->second[2]);
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_STACK_DEPTH, it->second[3]);
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_BLOCK_SHAPE, it->second[4]);
         }
         if (r == RM_OK) {
             // The job is charged what its slot now holds, if that is more

@@ -244,7 +244,8 @@ export class Renderer {
         const names = Object.keys(SCHEDULES);
         const candidates = result.candidates.map((c) => ({
             schedule: names.find((k) => SCHEDULES[k] == c.schedule), waves: c.waves,
-            persistentBlocks: c.persistentBlocks, ms: c.ms,
+            persistentBlocks: c.persistentBlocks, stackDepth: c.stackDepth,
+            blockShape: c.blockShape == 1 ? "rows" : "squares", ms: c.ms,
         }));
         const best = candidates[result.best];
         this.#schedule = best.schedule;

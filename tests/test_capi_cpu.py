@@ -124,9 +124,12 @@ def test_python_and_js_constants_match_the_header(tmp_path):
     header = (root / "include" / "raymond.h").read_text()
     defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define (RM_\w+) \(?(-?\d+)\)?", header)}
     py = {f"RM_OPTION_{n}": getattr(capi, f"OPTION_{n}")
-          for n in ("SCHEDULE", "PERSISTENT_BLOCKS", "PRECISION", "AUTOTUNE", "BATCH", "WAVES")}
+          for n in ("SCHEDULE", "PERSISTENT_BLOCKS", "PRECISION", "AUTOTUNE", "BATCH", "WAVES", "LIGHT_PATHS",
+                    "STACK_DEPTH", "BLOCK_SHAPE")}
     py.update({f"RM_SCHEDULE_{n}": getattr(capi, f"SCHEDULE_{n}") for n in ("PIXEL", "COMPACT", "PERSISTENT",
-                                                                            "WAVEFRONT")})
+                                                                            "WAVEFRONT", "SORTED")})
+    py.update({f"RM_BLOCK_{n}": getattr(capi, f"BLOCK_{n}") for n in ("SQUARES", "ROWS")})
+    py["RM_ERR_CANCELLED"] = capi.ERR_CANCELLED
     py.update({f"RM_PRECISION_{n}": getattr(capi, f"PRECISION_{n}") for n in ("QUANTIZED_BVH", "FP16_GUIDES",
                                                                               "COMPENSATED")})
     py.update({f"RM_BUFFER_{n}": getattr(capi, f"BUFFER_{n}") for n in ("ACCUM", "GUIDE_ALBEDO", "GUIDE_NORMAL",
@@ -138,7 +141,7 @@ def test_python_and_js_constants_match_the_header(tmp_path):
     assert max(v for k, v in defs.items() if k.startswith("RM_OPTION_") and k != "RM_OPTION_COUNT") + 1 == \
         defs["RM_OPTION_COUNT"]
     js = (root / "raymond_amd" / "js" / "addon.mjs").read_text()
-    for name in ("OPTION_PRECISION", "OPTION_SCHEDULE", "OPTION_WAVES"):
+    for name in ("OPTION_PRECISION", "OPTION_SCHEDULE", "OPTION_WAVES", "OPTION_LIGHT_PATHS"):
         assert int(re.search(rf"export const {name} = (\d+);", js).group(1)) == defs["RM_" + name]
     if not shutil.which("gcc"):
         pytest.skip("no C compiler")

@@ -99,17 +99,24 @@ def test_schedules_are_bit_identical(ctx, pack, config, opts, spp):
     try:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         plain = gpu_render(ctx, p, 3, params)
-        runs = [(capi.SCHEDULE_PIXEL, 0, 5), (capi.SCHEDULE_COMPACT, 0, 4), (capi.SCHEDULE_PERSISTENT, 0, 4),
-                (capi.SCHEDULE_PERSISTENT, 7, 4), (capi.SCHEDULE_WAVEFRONT, 0, 4)]
-        for schedule, blocks, waves in runs:
+        runs = [(capi.SCHEDULE_PIXEL, 0, 5, 8, 0), (capi.SCHEDULE_PIXEL, 0, 4, 16, 0), (capi.SCHEDULE_PIXEL, 0, 4, 8, 1),
+                (capi.SCHEDULE_COMPACT, 0, 4, 8, 0), (capi.SCHEDULE_SORTED, 0, 4, 8, 0),
+                (capi.SCHEDULE_PERSISTENT, 0, 4, 8, 0), (capi.SCHEDULE_PERSISTENT, 7, 4, 8, 0),
+                (capi.SCHEDULE_WAVEFRONT, 0, 4, 8, 0)]
+        for schedule, blocks, waves, depth, shape in runs:
             ctx.set_option(capi.OPTION_SCHEDULE, schedule)
             ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, blocks)
             ctx.set_option(capi.OPTION_WAVES, waves)
+            ctx.set_option(capi.OPTION_STACK_DEPTH, depth)
+            ctx.set_option(capi.OPTION_BLOCK_SHAPE, shape)
             other = gpu_render(ctx, p, 3, params)
             for name, a, b in zip(["accum", "albedo", "normal"], plain, other):
                 differ = int(np.any(a.view(np.uint32) != b.view(np.uint32), axis=-1).sum())
-                assert differ == 0, f"schedule {schedule} (blocks {blocks}, waves {waves}): {differ} pixels of {name} differ"
+                assert differ == 0, (f"schedule {schedule} (blocks {blocks}, waves {waves}, stack {depth}, "
+                                     f"shape {shape}): {differ} pixels of {name} differ")
     finally:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
         ctx.set_option(capi.OPTION_WAVES, 0)
+        ctx.set_option(capi.OPTION_STACK_DEPTH, 0)
+        ctx.set_option(capi.OPTION_BLOCK_SHAPE, 0)
