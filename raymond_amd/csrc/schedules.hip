@@ -98,21 +98,28 @@ RM_D void slot_load_wf(const SlotStore& st, uint32_t slot, uint32_t sample, Path
     reseed(g, ps, pix, sample);
 }
 
-// Sorting (SORT, RM_SCHEDULE_SORTED): between bounces, while more than one
-// wave's worth of paths is alive, the block's live paths are reordered
-// through the same LDS slots by a key: the octant of the next ray's
-// direction and whether the bounce that made it was specular (glass and
-// mirrors lead into chains of refraction, diffuse and glossy bounces into
-// light sampling). A wave then traces rays heading the same way, which visit
-// the tree's children in the same order, and shades surfaces reached the
-// same way. Lanes take slots by an LDS counting sort (a count per key, then
-// each key's run in key order); which lane a path lands on changes none of
-// its arithmetic, since its RNG state is re-derived from its pixel and
-// sample at every bounce, so the sums stay bit-identical.
+// Sorting (SORT, RM_SCHEDULE_SORTED): each bounce is split in two at its
+// closest hit, and while more than one wave's worth of paths is alive the
+// block's live paths are reordered through the LDS slots twice a bounce:
+// after tracing, by the kind of material hit (or none: the ray escaped), so
+// a wave shades one kind of surface (diffuse, metal, plastic, glass, light)
+// rather than branching between them; after shading, by the octant of the
+// next ray's direction and whether the bounce was specular, so a wave
+// traces rays heading the same way, which visit the tree's children in the
+// same order. Lanes take slots by an LDS counting sort (a count per key,
+// then each key's run in key order). Which lane a path lands on changes
+// none of its arithmetic: its RNG is re-derived from its pixel and sample,
+// and shading restarts the RNG's dimension from the bounce, so the sums
+// stay bit-identical.
 constexpr uint32_t SORT_KEYS = 16;
-RM_D uint32_t sort_key(const PathState& ps) {
+RM_D uint32_t direction_key(const PathState& ps) {
     return (ps.d.x < 0.0f ? 1u : 0u) | (ps.d.y < 0.0f ? 2u : 0u) | (ps.d.z < 0.0f ? 4u : 0u) |
            (ps.last_delta ? 8u : 0u);
+}
+RM_D uint32_t material_key(const DevScene& S, const Hit& h) {
+    if (h.prim == NONE) return 7u;
+    const uint32_t m = __float_as_uint(S.prims[(size_t)h.prim * 4].y);
+    return __float_as_uint(S.mats[(size_t)m * 6].w) & 7u;
 }
 
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION, bool SORT>
@@ -124,6 +131,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
     __shared__ uint32_t slot_words[SLOT_WORDS * BLOCK_THREADS];
     __shared__ uint32_t wave_live[BLOCK_THREADS / 64];
     __shared__ uint32_t key_count[SORT_KEYS];
+    __shared__ uint4 hit_slots[SORT ? BLOCK_THREADS : 1];   // a sorted path's closest hit rides with it
     const Ctx C{S, &P};
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t gid = (blockIdx.y * gridDim.x + blockIdx.x) * BLOCK_THREADS + t;
@@ -144,6 +152,73 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
         uint32_t pix = t;
         bool live = valid;
         if (live) path_begin(P, g, ps, gd, x, y, index, sample);
+        // The number of live paths in the block (and in the waves before this one).
+        auto count_live = [&](uint64_t& mask, uint32_t& before, uint32_t& nonempty) {
+            mask = __ballot(live);
+            if (lane == 0) wave_live[wave] = (uint32_t)__popcll(mask);
+            __syncthreads();
+            uint32_t total = 0;
+            before = nonempty = 0;
+            for (uint32_t w = 0; w < BLOCK_THREADS / 64; w++) {
+                const uint32_t n = wave_live[w];
+                before += w < wave ? n : 0u;
+                total += n;
+                nonempty += n > 0u ? 1u : 0u;
+            }
+            return total;
+        };
+        // Moves the live paths (and their hits) to slots 0 .. total-1 in key order.
+        auto regroup = [&](uint32_t key, uint32_t total, Hit& hit) {
+            if (t < SORT_KEYS) key_count[t] = 0u;
+            __syncthreads();
+            const uint32_t rank = live ? atomicAdd(&key_count[key], 1u) : 0u;
+            __syncthreads();
+            uint32_t start = 0;
+            for (uint32_t k2 = 0; k2 < SORT_KEYS; k2++) start += k2 < key ? key_count[k2] : 0u;
+            if (live) {
+                slot_save(st, start + rank, ps, gd, pix);
+                hit_slots[SORT ? start + rank : 0] =
+                    make_uint4(__float_as_uint(hit.t), hit.prim, __float_as_uint(hit.u), __float_as_uint(hit.v));
+            }
+            __syncthreads();
+            live = t < total;
+            if (live) {
+                slot_load(st, t, P, sample, ps, gd, g, pix);
+                const uint4 h = hit_slots[SORT ? t : 0];
+                hit = {__uint_as_float(h.x), h.y, __uint_as_float(h.z), __uint_as_float(h.w)};
+            }
+            __syncthreads();
+        };
+        if constexpr (SORT) {
+            for (;;) {
+                uint64_t mask;
+                uint32_t before, nonempty;
+                NoCount nc;
+                Hit hit = {INF, NONE, 0.0f, 0.0f};
+                if (live) hit = trace<false>(C.S, ps.o, ps.d, INF, stack, nc);
+                uint32_t total = count_live(mask, before, nonempty);
+                if (total == 0u) break;
+                // (Either way a barrier: no wave rewrites wave_live while another still reads it.)
+                if (total > 64u) regroup(live ? material_key(C.S, hit) : 0u, total, hit);
+                else __syncthreads();
+                if (live) {
+                    PendingNee pn;
+                    const bool ended = path_shade<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, ps, gd, hit, pn);
+                    if (pn.active && trace<true>(C.S, pn.o, pn.d, pn.reach, stack, nc).prim == NONE)
+                        apply_nee<VOLUMES>(C, ps, pn);
+                    if (ended) {
+                        stage[pix * 3] = make_float4(ps.result.x, ps.result.y, ps.result.z, 0.0f);
+                        stage[pix * 3 + 1] = make_float4(gd.albedo.x, gd.albedo.y, gd.albedo.z, 0.0f);
+                        stage[pix * 3 + 2] = gd.normal;
+                        live = false;
+                    }
+                }
+                total = count_live(mask, before, nonempty);
+                if (total == 0u) break;
+                if (total > 64u) regroup(live ? direction_key(ps) : 0u, total, hit);
+                else __syncthreads();
+            }
+        } else
             for (;;) {
             if (live && path_step<VOLUMES, TEXTURES, ENV, DISPERSION>(C, g, stack, ps, gd)) {
                 stage[pix * 3] = make_float4(ps.result.x, ps.result.y, ps.result.z, 0.0f);
@@ -162,20 +237,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(4
                 nonempty += n > 0u ? 1u : 0u;
             }
             if (total == 0u) break;
-            if (SORT && total > 64u) {
-                // Counting sort of the live paths by key into slots 0 .. total-1.
-                if (t < SORT_KEYS) key_count[t] = 0u;
-                __syncthreads();
-                const uint32_t key = live ? sort_key(ps) : 0u;
-                const uint32_t rank = live ? atomicAdd(&key_count[key], 1u) : 0u;
-                __syncthreads();
-                uint32_t start = 0;
-                for (uint32_t k2 = 0; k2 < SORT_KEYS; k2++) start += k2 < key ? key_count[k2] : 0u;
-                if (live) slot_save(st, start + rank, ps, gd, pix);
-                __syncthreads();
-                live = t < total;
-                if (live) slot_load(st, t, P, sample, ps, gd, g, pix);
-            } else if ((total + 63u) / 64u < nonempty) {
+            if ((total + 63u) / 64u < nonempty) {
                 // Pack the live paths into lanes 0 .. total-1, in lane order.
                 if (live) slot_save(st, before + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)), ps, gd, pix);
                 __syncthreads();
