@@ -31,6 +31,9 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 # from before the assignment, although the optimised IR selects the new one
 # (a <4 x float> phi across a divergent switch, lowered wrongly for gfx950);
 # without it the kernel matches the one-thread-per-pixel kernel bit for bit.
+# The fault is not confined to that kernel: built with SLP, main.hip's
+# k_main for volumes gives other guide normals than the oracle on 1.8% of the
+# fog-and-lamp fixture's pixels. So it stays off for every file.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
