@@ -525,6 +525,10 @@ def served_vs_plain(args) -> dict:
                                                    max(busy_plain["Msamples_per_s"], 1e-9), 3)}}
 
 
+SERVING_VARIANTS = ("served", "served_batched", "served_tuned", "served_batched_with_precision_variants",
+                    "served_one_slot", "plain")
+
+
 def bench_serving(args) -> dict:
     """A Poisson trace of jobs over the scene pool, served with concurrent
     slots and, on the same trace, one job at a time (the plain path)."""
@@ -533,12 +537,21 @@ def bench_serving(args) -> dict:
     trace = serving_trace(args.rate, args.duration, args.spp_min, args.spp_max)
     budget = 200 << 30
     flags = precision_flags(args.served_precision)
-    served = run_trace(packs, trace, args.slots, budget)
-    served_batched = run_trace(packs, trace, args.slots, budget, batch=True)
-    served_tuned = run_trace(packs, trace, args.slots, budget, autotune=1)
-    served_described = run_trace(packs, trace, args.slots, budget, flags, batch=True)
-    one_slot = run_trace(packs, trace, 1, budget)
-    plain = run_plain(packs, trace)
+    wanted = set(args.variants.split(","))
+    unknown = wanted - set(SERVING_VARIANTS)
+    if unknown or "served" not in wanted:
+        raise SystemExit(f"--variants takes 'served' and any of {', '.join(SERVING_VARIANTS)}; got {args.variants}")
+    runs = {
+        "served": lambda: run_trace(packs, trace, args.slots, budget),
+        "served_batched": lambda: run_trace(packs, trace, args.slots, budget, batch=True),
+        "served_tuned": lambda: run_trace(packs, trace, args.slots, budget, autotune=1),
+        "served_batched_with_precision_variants": lambda: run_trace(packs, trace, args.slots, budget, flags,
+                                                                    batch=True),
+        "served_one_slot": lambda: run_trace(packs, trace, 1, budget),
+        "plain": lambda: run_plain(packs, trace),
+    }
+    results = {k: runs[k]() for k in SERVING_VARIANTS if k in wanted}
+    served, plain = results["served"], results.get("plain")
     with capi.Context(0) as ctx:
         device, cus = ctx.device_info()
     return {
@@ -565,13 +578,8 @@ def bench_serving(args) -> dict:
                            "the running jobs' path passes batched into one launch, autotuned per scene and size, and "
                            "batched with the precision variants named); plain: one context, one job at a time, FP32, "
                            "every frame denoised"},
-        "served": served,
-        "served_batched": served_batched,
-        "served_tuned": served_tuned,
-        "served_batched_with_precision_variants": served_described,
-        "served_one_slot": one_slot,
-        "plain": plain,
-        "speedup_vs_plain": round(served["Msamples_per_s"] / max(plain["Msamples_per_s"], 1e-9), 3),
+        **results,
+        "speedup_vs_plain": round(served["Msamples_per_s"] / max(plain["Msamples_per_s"], 1e-9), 3) if plain else None,
         "cpu_baseline": None,
     }
 
@@ -594,6 +602,8 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the served-vs-plain trace")
     ap.add_argument("--capacity-rate", type=float, default=60.0,
                     help="default workload: arrival rate (jobs/s) of the saturating served-vs-plain trace")
+    ap.add_argument("--variants", default=",".join(SERVING_VARIANTS),
+                    help="serving: the runs to make, comma-separated (served is always one of them)")
     ap.add_argument("--served-precision", default="quantized-bvh,fp16-guides,compensated",
                     help="serving: precision variants of the batched variant run, comma-separated "
                          "(quantized-bvh, fp16-guides, compensated)")

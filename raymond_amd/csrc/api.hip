@@ -56,6 +56,152 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+namespace {
+// One device's block cache: released blocks, oldest first, each with the
+// events that pass once the work issued before its release is done.
+struct Block {
+    void* ptr;
+    size_t bytes;
+    hipEvent_t ready[3];
+    const Owner* owner;   // whose streams the events are on; null once they have passed
+};
+struct BlockCache {
+    std::mutex lock;
+    std::vector<Block> blocks;
+    uint64_t bytes = 0;
+};
+BlockCache g_cache[MAX_DEVICES];
+
+bool block_ready(const Block& b) {
+    for (hipEvent_t e : b.ready)
+        if (e && hipEventQuery(e) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+    return true;
+}
+
+// Waits for a block's events, destroys them and frees its memory.
+void block_free(const Block& b) {
+    for (hipEvent_t e : b.ready)
+        if (e) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
+    (void)hipFree(b.ptr);
+}
+
+// Frees the oldest blocks until the cache holds at most `keep` bytes; the
+// caller holds the lock.
+void shrink_to(BlockCache& cache, uint64_t keep) {
+    size_t n = 0;
+    while (n < cache.blocks.size() && cache.bytes > keep) {
+        block_free(cache.blocks[n]);
+        cache.bytes -= cache.blocks[n].bytes;
+        ++n;
+    }
+    cache.blocks.erase(cache.blocks.begin(), cache.blocks.begin() + (std::ptrdiff_t)n);
+    (void)hipGetLastError();
+}
+}  // namespace
+
+hipError_t cache_take(int device, size_t need, void** ptr, size_t* bytes) {
+    *ptr = nullptr;
+    *bytes = 0;
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    BlockCache& cache = g_cache[device];
+    {
+        std::lock_guard<std::mutex> guard(cache.lock);
+        // The smallest ready block of at least `need` and at most twice it.
+        size_t best = cache.blocks.size();
+        for (size_t i = 0; i < cache.blocks.size(); ++i) {
+            const Block& b = cache.blocks[i];
+            if (b.bytes < need || b.bytes / 2 > need) continue;
+            if (best < cache.blocks.size() && cache.blocks[best].bytes <= b.bytes) continue;
+            if (block_ready(b)) best = i;
+        }
+        if (best < cache.blocks.size()) {
+            const Block b = cache.blocks[best];
+            for (hipEvent_t e : b.ready)
+                if (e) (void)hipEventDestroy(e);
+            cache.blocks.erase(cache.blocks.begin() + (std::ptrdiff_t)best);
+            cache.bytes -= b.bytes;
+            *ptr = b.ptr;
+            *bytes = b.bytes;
+            return hipSuccess;
+        }
+    }
+    hipError_t e = hipMalloc(ptr, need);
+    if (e == hipErrorOutOfMemory) {
+        // The cache's blocks may be what the device lacks: free them all, then try again.
+        (void)hipGetLastError();
+        cache_flush(device);
+        e = hipMalloc(ptr, need);
+    }
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return e;
+    }
+    *bytes = need;
+    return hipSuccess;
+}
+
+void cache_put(const Owner& owner, void* ptr, size_t bytes) {
+    if (owner.device < 0 || owner.device >= MAX_DEVICES) return;
+    Block b{ptr, bytes, {nullptr, nullptr, nullptr}, &owner};
+    bool recorded = true;
+    for (int k = 0; k < 3; ++k) {
+        if (!owner.streams[k]) continue;
+        if (hipEventCreateWithFlags(&b.ready[k], hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(b.ready[k], owner.streams[k]) != hipSuccess) {
+            recorded = false;
+            break;
+        }
+    }
+    if (!recorded) {
+        // Without its events the block can't be handed on safely: free it
+        // (hipFree waits for the device).
+        (void)hipGetLastError();
+        for (hipEvent_t e : b.ready)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipFree(ptr);
+        return;
+    }
+    BlockCache& cache = g_cache[owner.device];
+    std::lock_guard<std::mutex> guard(cache.lock);
+    cache.blocks.push_back(b);
+    cache.bytes += bytes;
+    if (cache.bytes > BLOCK_CACHE_BYTES) shrink_to(cache, BLOCK_CACHE_BYTES);
+}
+
+void cache_forget(const Owner& owner) {
+    if (owner.device < 0 || owner.device >= MAX_DEVICES) return;
+    BlockCache& cache = g_cache[owner.device];
+    std::lock_guard<std::mutex> guard(cache.lock);
+    for (Block& b : cache.blocks) {
+        if (b.owner != &owner) continue;
+        for (hipEvent_t& e : b.ready)
+            if (e) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+        b.owner = nullptr;
+    }
+}
+
+uint64_t cache_bytes(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return 0;
+    std::lock_guard<std::mutex> guard(g_cache[device].lock);
+    return g_cache[device].bytes;
+}
+
+void cache_flush(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return;
+    std::lock_guard<std::mutex> guard(g_cache[device].lock);
+    shrink_to(g_cache[device], 0);
+}
+
 }  // namespace rmi
 
 using rmi::DevBuf;
@@ -245,6 +391,12 @@ int rm_open(int device, rm_context** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->prepared_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->denoised_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming);
+    c->owner.device = device;
+    c->owner.streams[0] = c->stream;
+    c->owner.streams[1] = c->side;
+    c->owner.streams[2] = c->post;
+    for (DevBuf* b : working_buffers(c)) b->owner = &c->owner;
+    for (DevBuf* b : {&c->env, &c->env_table, &c->work}) b->owner = &c->owner;
     if (e != hipSuccess) {
         if (c->batch_ev) (void)hipEventDestroy(c->batch_ev);
         if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
@@ -323,6 +475,9 @@ int rm_close(rm_context* c) {
                       &c->gathered[0], &c->gathered[1], &c->env, &c->env_table, &c->rays, &c->hits, &c->counters,
                       &c->work, &c->staging, &c->display, &c->comp, &c->folded, &c->wavefront})
         b->release();
+    // The streams are idle: the blocks just given back need no events, which
+    // must not outlive the streams they were recorded on.
+    rmi::cache_forget(c->owner);
     for (int s = 0; s < RM_STAGE_COUNT; ++s)
         for (auto& p : c->timer.pending[s]) {
             (void)hipEventDestroy(p.first);

@@ -20,28 +20,59 @@ int hip_fail(hipError_t e, const char* what);
         if (e_ != hipSuccess) return ::rmi::hip_fail(e_, what); \
     } while (0)
 
-// A growable device allocation.
+// The streams a context's buffers are used on, for the block cache.
+struct Owner {
+    int device = 0;
+    hipStream_t streams[3] = {nullptr, nullptr, nullptr};
+};
+
+// The device's block cache. hipFree waits for every stream of the device, so
+// a context that gave back a buffer (to grow it, or to trim a job server's
+// slot) would drain the whole GPU, all other contexts' frames with it. A
+// context's buffers are given back to this cache instead, with events
+// recorded on the context's streams; cache_take hands a block on only once
+// those events have passed, so no work still using it can see it reused.
+// Blocks are reused for requests of at least half their size. The cache holds
+// at most BLOCK_CACHE_BYTES a device; past that, and when an allocation fails,
+// it frees its oldest blocks (after their events).
+constexpr uint64_t BLOCK_CACHE_BYTES = 16ull << 30;
+hipError_t cache_take(int device, size_t need, void** ptr, size_t* bytes);
+void cache_put(const Owner& owner, void* ptr, size_t bytes);
+uint64_t cache_bytes(int device);
+// Called by a context that is closing, once its streams are idle: drops the
+// events its released blocks carry (they have passed), before the streams go.
+void cache_forget(const Owner& owner);
+// Frees every cached block of the device (waiting for their events).
+void cache_flush(int device);
+
+// A growable device allocation. With an owner (a context's buffer), memory
+// comes from and goes back to the device's block cache; without one (a
+// scene's buffers, read by every context), straight from hipMalloc and to
+// hipFree.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    const Owner* owner = nullptr;
     hipError_t ensure(size_t need) {
         if (need <= bytes) return hipSuccess;
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&ptr, need);
+        release();
+        hipError_t e = owner ? cache_take(owner->device, need, &ptr, &bytes) : hipMalloc(&ptr, need);
         if (e == hipSuccess) {
-            bytes = need;
+            if (!owner) bytes = need;
         } else {
             // The failure is returned here; left as the thread's last error,
             // it would be reported again by the next launch's check.
             ptr = nullptr;
+            bytes = 0;
             (void)hipGetLastError();
         }
         return e;
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            if (owner) cache_put(*owner, ptr, bytes);
+            else (void)hipFree(ptr);
+        }
         ptr = nullptr;
         bytes = 0;
     }
@@ -74,6 +105,7 @@ struct rm_scene {
 
 struct rm_context {
     int device = 0;
+    rmi::Owner owner;   // the device and streams its buffers are used on
     hipStream_t stream = nullptr;
     // Light tracing and photon gather run on `side`, alongside the path pass
     // on `stream`; join orders a frame's resolve after them.

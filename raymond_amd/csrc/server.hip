@@ -207,7 +207,10 @@ bool rm_server::admit() {
             if (queue.empty()) return any;
             bool busy = false;
             for (Slot& o : slots) busy = busy || (bool)o.job;
-            // A job larger than the whole budget still runs, alone.
+            // A job larger than the whole budget still runs, alone. Blocks
+            // the slots gave back to the device's cache count against the
+            // budget too: when they would push it over, they are freed.
+            if (used + queue.front()->bytes + rmi::cache_bytes(device) > budget) rmi::cache_flush(device);
             if (busy && used + queue.front()->bytes > budget) return any;
             j = queue.front();
             queue.pop_front();

@@ -54,6 +54,28 @@ def test_jobs_match_their_frames_rendered_alone(ctx, pack):
             assert np.array_equal(image.view(np.uint32), expected[j].view(np.uint32)), name
 
 
+def test_released_buffers_are_reused_safely(ctx, pack):
+    """A context's buffers go back to the device's block cache when it closes
+    or grows them, and the next context takes them, holding another job's
+    values: each job, rendered on a fresh context right after another
+    context rendered the previous job and closed, gives the image it gives
+    alone."""
+    packs = [pack(c, **o) for c, o, _, _ in JOBS]
+    params = [p.with_params(firefly_clamp=0.0, **f) for p, (_, _, f, _) in zip(packs, JOBS)]
+    frames = [n for *_, n in JOBS]
+    expected = [alone(ctx, p, q, n) for p, q, n in zip(packs, params, frames)]
+    for j in range(len(JOBS)):
+        k = (j + 1) % len(JOBS)
+        with capi.Context(0) as before:
+            alone(before, packs[k], params[k], frames[k])
+        with capi.Context(0) as after:
+            image = alone(after, packs[j], params[j], frames[j])
+        if params[j].view(np.uint32)[47]:
+            assert agreement(image, expected[j], rel=1e-3, abs_=1e-4) >= 0.999, JOBS[j][0]
+        else:
+            assert np.array_equal(image.view(np.uint32), expected[j].view(np.uint32)), JOBS[j][0]
+
+
 def test_budget_serialises_jobs(pack):
     """A budget smaller than any job admits one at a time, and they still finish."""
     p = pack("spheres", scene={"width": 64, "height": 36}, width=64, height=36)
