@@ -405,7 +405,8 @@ def precision_flags(names: str) -> int:
     return flags
 
 
-def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotune: int = 0, batch: bool = False) -> dict:
+def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotune: int = 0, batch: bool = False,
+              schedule: int = 0) -> dict:
     """Submits the trace's jobs at their arrival times to a server with `slots`
     slots and RM_OPTION_PRECISION flags `precision`; returns throughput and
     latency percentiles."""
@@ -414,6 +415,7 @@ def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotun
         server.set_option(capi.OPTION_PRECISION, precision)
         server.set_option(capi.OPTION_AUTOTUNE, autotune)
         server.set_option(capi.OPTION_BATCH, int(batch))
+        server.set_option(capi.OPTION_SCHEDULE, schedule)
         scenes = [server.context.upload_scene(p) for p in packs]
         jobs = []
         for arrival, k, (w, h), spp, bounces in trace:
@@ -525,8 +527,8 @@ def served_vs_plain(args) -> dict:
                                                    max(busy_plain["Msamples_per_s"], 1e-9), 3)}}
 
 
-SERVING_VARIANTS = ("served", "served_batched", "served_tuned", "served_batched_with_precision_variants",
-                    "served_one_slot", "plain")
+SERVING_VARIANTS = ("served", "served_batched", "served_batched_wavefront", "served_tuned",
+                    "served_batched_with_precision_variants", "served_one_slot", "plain")
 
 
 def bench_serving(args) -> dict:
@@ -544,6 +546,8 @@ def bench_serving(args) -> dict:
     runs = {
         "served": lambda: run_trace(packs, trace, args.slots, budget),
         "served_batched": lambda: run_trace(packs, trace, args.slots, budget, batch=True),
+        "served_batched_wavefront": lambda: run_trace(packs, trace, args.slots, budget, batch=True,
+                                                      schedule=capi.SCHEDULE_WAVEFRONT),
         "served_tuned": lambda: run_trace(packs, trace, args.slots, budget, autotune=1),
         "served_batched_with_precision_variants": lambda: run_trace(packs, trace, args.slots, budget, flags,
                                                                     batch=True),
@@ -575,9 +579,10 @@ def bench_serving(args) -> dict:
                    "memory_budget_bytes": budget, "device": device, "compute_units": cus,
                    "served_precision": args.served_precision,
                    "path": "served: concurrent slots, FP32, a job's first and last frames denoised (also reported: "
-                           "the running jobs' path passes batched into one launch, autotuned per scene and size, and "
-                           "batched with the precision variants named); plain: one context, one job at a time, FP32, "
-                           "every frame denoised"},
+                           "the running jobs' path passes batched into one launch; batched on the wavefront schedule, "
+                           "each stage one launch for all running jobs, replayed from a hipGraph; autotuned per scene "
+                           "and size; and batched with the precision variants named); plain: one context, one job at "
+                           "a time, FP32, every frame denoised"},
         **results,
         "speedup_vs_plain": round(served["Msamples_per_s"] / max(plain["Msamples_per_s"], 1e-9), 3) if plain else None,
         "cpu_baseline": None,

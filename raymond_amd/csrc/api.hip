@@ -1,5 +1,6 @@
 // The C ABI (include/raymond.h) over the kernels: device memory, the frame's
 // dispatch order (src/Renderer.mjs:441-489), readback and stage timing.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -847,6 +848,27 @@ static int frame_begin(rm_context* c, Frame& F) {
     return RM_OK;
 }
 
+// The wavefront path state of a frame of px pixels, carved out of the
+// context's wavefront buffer (frame_alloc sized it).
+static rm::Wavefront carve_wavefront(rm_context* c, uint32_t px) {
+    rm::Wavefront W;
+    char* base = c->wavefront.as<char>();
+    auto carve = [&](size_t bytes) {
+        char* p = base;
+        base += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    W.n = px;
+    W.state[0] = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_STATE_WORDS * 4));
+    W.state[1] = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_STATE_WORDS * 4));
+    W.hits = reinterpret_cast<uint4*>(carve((size_t)px * 16));
+    W.nee = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_NEE_WORDS * 4));
+    W.done = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_DONE_WORDS * 4));
+    W.sums = reinterpret_cast<float4*>(carve((size_t)px * 48));
+    W.counts = reinterpret_cast<uint32_t*>(carve(16));
+    return W;
+}
+
 // The path pass under the context's schedule, into the buffers frame_alloc sized.
 static int frame_path(rm_context* c, Frame& F) {
     const rm::Params& P = F.P;
@@ -855,21 +877,7 @@ static int frame_path(rm_context* c, Frame& F) {
     const uint32_t px = F.px;
     if (F.persistent) rm::launch_fill_u32(c->work.as<uint32_t>(), 0u, 1, c->stream);
     if (F.wavefront) {
-        rm::Wavefront W;
-        char* base = c->wavefront.as<char>();
-        auto carve = [&](size_t bytes) {
-            char* p = base;
-            base += (bytes + 255) & ~(size_t)255;
-            return p;
-        };
-        W.n = px;
-        W.state[0] = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_STATE_WORDS * 4));
-        W.state[1] = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_STATE_WORDS * 4));
-        W.hits = reinterpret_cast<uint4*>(carve((size_t)px * 16));
-        W.nee = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_NEE_WORDS * 4));
-        W.done = reinterpret_cast<uint32_t*>(carve((size_t)px * rm::WF_DONE_WORDS * 4));
-        W.sums = reinterpret_cast<float4*>(carve((size_t)px * 48));
-        W.counts = reinterpret_cast<uint32_t*>(carve(16));
+        const rm::Wavefront W = carve_wavefront(c, px);
         Timed t(c, RM_STAGE_TRACE);
         rm::LaunchWavefront L{S, P, W, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), F.comp,
                               c->spill.as<uint32_t>(), F.slots, F.slots / rm::BLOCK_THREADS, f.tree};
@@ -961,6 +969,101 @@ int rm_frame(rm_context* c, const rm_scene* s, const void* params, const float* 
     return r;
 }
 
+namespace {
+// The cross-job wavefront pass's launch sequences, captured in hipGraphs, one
+// for each shape (device, features, tree, samples, bounce rounds, grid), with
+// the device job table its launches read and an event that passes once the
+// last replay has read it.
+struct WfGraph {
+    int device;
+    uint64_t key;
+    hipGraphExec_t exec;
+    rm::WfTable* table;
+    hipEvent_t done;
+};
+std::mutex g_graph_lock;
+std::vector<WfGraph> g_graphs;
+constexpr size_t MAX_GRAPHS = 32;
+
+void drop_graph(const WfGraph& g) {
+    (void)hipEventSynchronize(g.done);
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipFree(g.table);
+    (void)hipEventDestroy(g.done);
+}
+
+// Captures the launch sequence of shape B on `stream` into a new graph.
+int capture_graph(hipStream_t stream, int device, uint64_t key, const rm::WfBatchShape& B, const rm::Features& u,
+                  WfGraph& out) {
+    out = WfGraph{device, key, nullptr, nullptr, nullptr};
+    RM_HIP(hipMalloc((void**)&out.table, sizeof(rm::WfTable)), "allocating a batch's job table");
+    hipError_t e = hipEventCreateWithFlags(&out.done, hipEventDisableTiming);
+    hipGraph_t graph = nullptr;
+    if (e == hipSuccess) e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rm::launch_wavefront_batch(out.table, B, u, stream);
+        hipError_t launched = hipGetLastError();
+        e = hipStreamEndCapture(stream, &graph);
+        if (e == hipSuccess) e = launched;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&out.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        if (out.exec) (void)hipGraphExecDestroy(out.exec);
+        if (out.done) (void)hipEventDestroy(out.done);
+        (void)hipFree(out.table);
+        out = WfGraph{};
+        return hip_fail(e, "capturing the batched wavefront pass");
+    }
+    return RM_OK;
+}
+
+// The batch's wavefront path passes: its job table uploaded on the first
+// context's stream, then the graph of its shape replayed there.
+int wavefront_batch(rm_context* run, rm_context* const* ctxs, const Frame* F, uint32_t n, const rm::Features& u) {
+    rm::WfTable T;
+    std::memset(&T, 0, sizeof T);
+    rm::WfBatchShape B{0u, 0u, (uint32_t)(run->compute_units > 0 ? run->compute_units : 1) * 8u, u.tree};
+    uint32_t chunks = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        rm_context* c = ctxs[i];
+        T.job[i] = rm::WfJob{F[i].S, F[i].P, carve_wavefront(c, F[i].px), c->accum.as<float4>(), c->galb.as<float4>(),
+                             c->gnorm.as<float4>(), F[i].comp, c->spill.as<uint32_t>(), F[i].slots};
+        T.begin[i] = chunks;
+        chunks += (F[i].px + rm::BLOCK_THREADS - 1) / rm::BLOCK_THREADS;
+        B.samples = std::max(B.samples, F[i].P.samples_per_frame);
+        B.bounces = std::max(B.bounces, F[i].P.max_bounces + 1u);
+    }
+    T.begin[n] = chunks;
+    T.n = n;
+    const uint64_t features = (u.volumes ? 8u : 0u) | (u.textures ? 4u : 0u) | (u.environment ? 2u : 0u) |
+                              (u.dispersion ? 1u : 0u);
+    const uint64_t key = features | (uint64_t)(B.tree & 3) << 4 | (uint64_t)(B.samples & 0xffffu) << 8 |
+                         (uint64_t)(B.bounces & 0xffffu) << 24 | (uint64_t)B.blocks << 40;
+    std::lock_guard<std::mutex> guard(g_graph_lock);
+    WfGraph* g = nullptr;
+    for (WfGraph& x : g_graphs)
+        if (x.device == run->device && x.key == key) g = &x;
+    if (!g) {
+        if (g_graphs.size() >= MAX_GRAPHS) {
+            drop_graph(g_graphs.front());
+            g_graphs.erase(g_graphs.begin());
+        }
+        WfGraph made;
+        int r = capture_graph(run->stream, run->device, key, B, u, made);
+        if (r != RM_OK) return r;
+        g_graphs.push_back(made);
+        g = &g_graphs.back();
+    }
+    // The last replay of this graph, on whichever stream, has read the table.
+    RM_HIP(hipStreamWaitEvent(run->stream, g->done, 0), "ordering a batch's job table");
+    RM_HIP(hipMemcpyAsync(g->table, &T, sizeof T, hipMemcpyHostToDevice, run->stream), "uploading a batch's job table");
+    RM_HIP(hipGraphLaunch(g->exec, run->stream), "launching the batched wavefront pass");
+    RM_HIP(hipEventRecord(g->done, run->stream), "ordering a batch's job table");
+    return RM_OK;
+}
+}  // namespace
+
 int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const void* const* params,
                    const float* const* emitters, const uint32_t* emitter_vec4s, uint32_t n) {
     if (!ctxs || !scenes || !params || n == 0 || n > rm::MAX_BATCH)
@@ -979,9 +1082,15 @@ int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const
             return fail(RM_ERR_INVALID, "A batch's frames walk the same kind of tree (RM_PRECISION_QUANTIZED_BVH "
                                         "the same for all, and quantizable scenes).");
     }
+    // Every context on the wavefront schedule (and rendering colour): the
+    // jobs' wavefront passes, each stage one launch for all of them, replayed
+    // from a hipGraph; otherwise one launch of the per-pixel pass.
+    bool wavefront = true;
+    for (uint32_t i = 0; i < n; ++i)
+        wavefront = wavefront && ctxs[i]->options[RM_OPTION_SCHEDULE] == RM_SCHEDULE_WAVEFRONT && F[i].P.view == 0u;
     for (uint32_t i = 0; i < n; ++i) {
         int r = frame_alloc(ctxs[i], emitters ? emitters[i] : nullptr, emitter_vec4s ? emitter_vec4s[i] : 0u, F[i],
-                            false);
+                            wavefront);
         if (r != RM_OK) return r;
     }
     for (uint32_t i = 0; i < n; ++i) {
@@ -1014,7 +1123,11 @@ int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const
         RM_HIP(hipEventRecord(ctxs[i]->batch_ev, ctxs[i]->stream), "ordering a batch");
         RM_HIP(hipStreamWaitEvent(run->stream, ctxs[i]->batch_ev, 0), "ordering a batch");
     }
-    {
+    if (wavefront) {
+        Timed t(run, RM_STAGE_TRACE);
+        int r = wavefront_batch(run, ctxs, F, n, u);
+        if (r != RM_OK) return r;
+    } else {
         Timed t(run, RM_STAGE_TRACE);
         rm::launch_main_batch(T, u, run->stream);
     }

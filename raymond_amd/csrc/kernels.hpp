@@ -114,6 +114,38 @@ struct LaunchWavefront {
     int tree;
 };
 void launch_wavefront(const LaunchWavefront& L, const Features& f, hipStream_t stream);
+
+// Several jobs' wavefront path passes, each stage one launch for all of them
+// (rm_frame_batch with every context on RM_SCHEDULE_WAVEFRONT). The table is
+// in device memory; job j's queue entries are its 256-entry chunks
+// [begin[j], begin[j + 1]).
+struct WfJob {
+    DevScene S;
+    Params P;
+    Wavefront W;
+    float4* accum;
+    float4* galb;
+    float4* gnorm;
+    float4* comp;       // non-null: compensated sums
+    uint32_t* spill;    // traversal spill for `slots` entries (at least W.n)
+    uint32_t slots;
+};
+struct WfTable {
+    WfJob job[MAX_BATCH];
+    uint32_t begin[MAX_BATCH + 1];
+    uint32_t n;
+};
+// The launch sequence's shape: samples per frame and bounce stages (the
+// most any job has), the persistent grid, the tree. Captured in a hipGraph,
+// one sequence serves every batch of this shape.
+struct WfBatchShape {
+    uint32_t samples;
+    uint32_t bounces;   // extend/shade/shadow/finish rounds: max_bounces + 1 of the deepest job
+    uint32_t blocks;
+    int tree;
+};
+// f: the union of the jobs' features.
+void launch_wavefront_batch(const WfTable* table, const WfBatchShape& B, const Features& f, hipStream_t stream);
 // How many 256-thread blocks of the persistent path kernel fit on one CU.
 int main_resident_blocks_per_cu(const Features& f);
 void launch_raycast(const DevScene& S, const float4* rays, uint32_t n, uint4* hits, bool any_hit, int tree,

@@ -389,58 +389,38 @@ RM_D void wf_put_sums(const Wavefront& W, uint32_t i, const PixelSums& s) {
     W.sums[(size_t)2 * W.n + i] = s.normal;
 }
 
-// One thread per queue entry: every stage's grid covers all n paths, and the
-// threads past the queue's length (read on the device) return at once, so
-// finished waves give way to new ones as in a kernel launched per ray.
-#define WF_EACH(i, count) if (const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x; i < (count))
+// The stages' work for one queue entry i, shared by the one-frame kernels
+// below and the cross-job kernels (k_wfb_*) after them.
 
 // Every pixel's sample `sample`: its camera path, in pixel order in live
 // state 0; the first sample of the frame also clears the pixel's sums.
-__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_begin(Params P, Wavefront W, uint32_t sample, uint32_t first) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        W.counts[0] = W.n;
-        W.counts[1] = W.counts[2] = W.counts[3] = 0u;
-    }
-    WF_EACH(i, W.n) {
-        Rng g;
-        PathState ps;
-        Guides gd;
-        path_begin(P, g, ps, gd, i % P.width, i / P.width, i, sample);
-        SlotStore st = wf_state(W, 0);
-        slot_save(st, i, ps, gd, i);
-        if (first) {
-            PixelSums s;
-            s.clear();
-            wf_put_sums(W, i, s);
-        }
+RM_D void wf_begin_one(const Params& P, const Wavefront& W, uint32_t i, uint32_t sample, bool first) {
+    Rng g;
+    PathState ps;
+    Guides gd;
+    path_begin(P, g, ps, gd, i % P.width, i / P.width, i, sample);
+    SlotStore st = wf_state(W, 0);
+    slot_save(st, i, ps, gd, i);
+    if (first) {
+        PixelSums s;
+        s.clear();
+        wf_put_sums(W, i, s);
     }
 }
 
-template <int TREE>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_extend(DevScene S, Wavefront W, uint32_t cur, uint32_t* spill,
-                                                             uint32_t slots) {
-    __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
-    // Nothing reads these until the shade kernel after this one.
-    if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[cur ^ 1u] = W.counts[2] = W.counts[3] = 0u;
-    WF_EACH(i, W.counts[cur]) {
-        auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, spill, i, slots);
-        const SlotStore st = wf_state(W, cur);
-        NoCount nc;
-        const Hit h = trace<false>(S, st.get3(i, 0), st.get3(i, 3), INF, stack, nc);
-        W.hits[i] = make_uint4(__float_as_uint(h.t), h.prim, __float_as_uint(h.u), __float_as_uint(h.v));
-    }
+template <class STACK>
+RM_D void wf_extend_one(const DevScene& S, const Wavefront& W, uint32_t cur, uint32_t i, STACK& stack) {
+    const SlotStore st = wf_state(W, cur);
+    NoCount nc;
+    const Hit h = trace<false>(S, st.get3(i, 0), st.get3(i, 3), INF, stack, nc);
+    W.hits[i] = make_uint4(__float_as_uint(h.t), h.prim, __float_as_uint(h.u), __float_as_uint(h.v));
 }
 
+// Shades entry i when live (i below the live count); every lane of the wave
+// calls it, live or not, so the slot ballots see the whole wave.
 template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_shade(DevScene S, Params P, Wavefront W, uint32_t cur,
-                                                            uint32_t sample) {
-    const Ctx C{S, &P};
+RM_D void wf_shade_one(const Ctx& C, const Wavefront& W, uint32_t cur, uint32_t sample, uint32_t i, bool live) {
     const uint32_t next = cur ^ 1u;
-    const uint32_t count = W.counts[cur];
-    // Whole waves reach the slot atomics together, so their ballots see every lane.
-    if (blockIdx.x * BLOCK_THREADS >= count) return;
-    const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
-    const bool live = i < count;
     bool ended = false;
     PendingNee pn;
     pn.active = false;
@@ -487,6 +467,79 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_wf_shade(DevScene S, Params P
     }
 }
 
+template <bool VOLUMES, class STACK>
+RM_D void wf_shadow_one(const Ctx& C, const Wavefront& W, uint32_t next, uint32_t i, STACK& stack) {
+    const SlotStore nz{W.nee, W.n};
+    PendingNee pn;
+    pn.o = nz.get3(i, 0);
+    pn.d = nz.get3(i, 3);
+    pn.vec = nz.get3(i, 6);
+    pn.reach = nz.get(i, 9);
+    pn.phase = nz.get(i, 10);
+    pn.mis = nz.get(i, 11);
+    pn.pdf = nz.get(i, 12);
+    pn.medium = nz.getu(i, 13) != 0u;
+    pn.active = true;
+    NoCount nc;
+    if (trace<true>(C.S, pn.o, pn.d, pn.reach, stack, nc).prim == NONE) {
+        const uint32_t to = nz.getu(i, 14);
+        SlotStore dst = (to & 0x80000000u) ? SlotStore{W.done, W.n} : wf_state(W, next);
+        const uint32_t slot = to & 0x7fffffffu;
+        const int word = (to & 0x80000000u) ? 0 : 9;   // the result's first word
+        PathState ps = {};
+        ps.result = dst.get3(slot, word);
+        apply_nee<VOLUMES>(C, ps, pn);
+        dst.put3(slot, word, ps.result);
+    }
+}
+
+RM_D void wf_finish_one(const Params& P, const Wavefront& W, uint32_t i) {
+    const SlotStore dn{W.done, W.n};
+    Guides gd;
+    gd.albedo = dn.get3(i, 3);
+    gd.normal = make_float4(dn.get(i, 6), dn.get(i, 7), dn.get(i, 8), dn.get(i, 9));
+    const uint32_t pix = dn.getu(i, 10);
+    PixelSums s = wf_sums(W, pix);
+    s.add(P, dn.get3(i, 0), gd, true);
+    wf_put_sums(W, pix, s);
+}
+
+// One thread per queue entry: every stage's grid covers all n paths, and the
+// threads past the queue's length (read on the device) return at once, so
+// finished waves give way to new ones as in a kernel launched per ray.
+#define WF_EACH(i, count) if (const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x; i < (count))
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_begin(Params P, Wavefront W, uint32_t sample, uint32_t first) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        W.counts[0] = W.n;
+        W.counts[1] = W.counts[2] = W.counts[3] = 0u;
+    }
+    WF_EACH(i, W.n) wf_begin_one(P, W, i, sample, first != 0u);
+}
+
+template <int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_extend(DevScene S, Wavefront W, uint32_t cur, uint32_t* spill,
+                                                             uint32_t slots) {
+    __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
+    // Nothing reads these until the shade kernel after this one.
+    if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[cur ^ 1u] = W.counts[2] = W.counts[3] = 0u;
+    WF_EACH(i, W.counts[cur]) {
+        auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, spill, i, slots);
+        wf_extend_one(S, W, cur, i, stack);
+    }
+}
+
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wf_shade(DevScene S, Params P, Wavefront W, uint32_t cur,
+                                                            uint32_t sample) {
+    const Ctx C{S, &P};
+    const uint32_t count = W.counts[cur];
+    // Whole waves reach the slot atomics together, so their ballots see every lane.
+    if (blockIdx.x * BLOCK_THREADS >= count) return;
+    const uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    wf_shade_one<VOLUMES, TEXTURES, ENV, DISPERSION>(C, W, cur, sample, i, i < count);
+}
+
 template <bool VOLUMES, int TREE>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_wf_shadow(DevScene S, Params P, Wavefront W, uint32_t next,
                                                              uint32_t* spill, uint32_t slots) {
@@ -494,42 +547,12 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_wf_shadow(DevScene S, Params 
     const Ctx C{S, &P};
     WF_EACH(i, W.counts[2]) {
         auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, spill, i, slots);
-        const SlotStore nz{W.nee, W.n};
-        PendingNee pn;
-        pn.o = nz.get3(i, 0);
-        pn.d = nz.get3(i, 3);
-        pn.vec = nz.get3(i, 6);
-        pn.reach = nz.get(i, 9);
-        pn.phase = nz.get(i, 10);
-        pn.mis = nz.get(i, 11);
-        pn.pdf = nz.get(i, 12);
-        pn.medium = nz.getu(i, 13) != 0u;
-        pn.active = true;
-        NoCount nc;
-        if (trace<true>(S, pn.o, pn.d, pn.reach, stack, nc).prim == NONE) {
-            const uint32_t to = nz.getu(i, 14);
-            SlotStore dst = (to & 0x80000000u) ? SlotStore{W.done, W.n} : wf_state(W, next);
-            const uint32_t slot = to & 0x7fffffffu;
-            const int word = (to & 0x80000000u) ? 0 : 9;   // the result's first word
-            PathState ps = {};
-            ps.result = dst.get3(slot, word);
-            apply_nee<VOLUMES>(C, ps, pn);
-            dst.put3(slot, word, ps.result);
-        }
+        wf_shadow_one<VOLUMES>(C, W, next, i, stack);
     }
 }
 
 __global__ __launch_bounds__(BLOCK_THREADS) void k_wf_finish(Params P, Wavefront W) {
-    WF_EACH(i, W.counts[3]) {
-        const SlotStore dn{W.done, W.n};
-        Guides gd;
-        gd.albedo = dn.get3(i, 3);
-        gd.normal = make_float4(dn.get(i, 6), dn.get(i, 7), dn.get(i, 8), dn.get(i, 9));
-        const uint32_t pix = dn.getu(i, 10);
-        PixelSums s = wf_sums(W, pix);
-        s.add(P, dn.get3(i, 0), gd, true);
-        wf_put_sums(W, pix, s);
-    }
+    WF_EACH(i, W.counts[3]) wf_finish_one(P, W, i);
 }
 
 __global__ __launch_bounds__(BLOCK_THREADS) void k_wf_store(Params P, Wavefront W, float4* __restrict__ accum,
@@ -538,6 +561,99 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_wf_store(Params P, Wavefront 
     WF_EACH(i, W.n) wf_sums(W, i).store(P, i, accum, galb, gnorm, comp);
 }
 
+// ------------------------------------------------------- cross-job wavefront
+// Several jobs' wavefront path passes (rm_frame_batch, every context on
+// RM_SCHEDULE_WAVEFRONT), each stage one launch for all of them. The job
+// table lives in device memory, so the launches' arguments don't change from
+// frame to frame and the whole sequence is captured once in a hipGraph and
+// replayed. The grids are persistent: a fixed number of blocks takes the
+// jobs' 256-entry chunks in turn (job j's are [begin[j], begin[j + 1])); a
+// chunk lies within one job, so its job and Params are uniform over the
+// block, read by scalar loads. Each job keeps its own queues and counters, so
+// its paths are shaded in the same order, with the same functions, as by the
+// one-job kernels above: the images are the same, bit for bit.
+RM_D uint32_t wfb_job(const WfTable& T, uint32_t chunk) {
+    uint32_t j = 0;
+    while (j + 1u < T.n && chunk >= T.begin[j + 1u]) ++j;
+    return j;
+}
+
+#define WFB_EACH_CHUNK(T, J, base)                                                        \
+    for (uint32_t chunk_ = blockIdx.x; chunk_ < (T).begin[(T).n]; chunk_ += gridDim.x)    \
+        if (const uint32_t j_ = wfb_job((T), chunk_), base = (chunk_ - (T).begin[j_]) * BLOCK_THREADS; true) \
+            if (const WfJob& J = (T).job[j_]; true)
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wfb_begin(const WfTable* __restrict__ table, uint32_t k) {
+    const WfTable& T = *table;
+    WFB_EACH_CHUNK(T, J, base) {
+        const bool on = k < J.P.samples_per_frame;
+        if (base == 0u && threadIdx.x == 0) {
+            J.W.counts[0] = on ? J.W.n : 0u;
+            J.W.counts[1] = J.W.counts[2] = J.W.counts[3] = 0u;
+        }
+        const uint32_t i = base + threadIdx.x;
+        if (on && i < J.W.n) wf_begin_one(J.P, J.W, i, J.P.sample_offset + k, k == 0u);
+    }
+}
+
+template <int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wfb_extend(const WfTable* __restrict__ table, uint32_t cur) {
+    __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
+    const WfTable& T = *table;
+    WFB_EACH_CHUNK(T, J, base) {
+        if (base == 0u && threadIdx.x == 0) J.W.counts[cur ^ 1u] = J.W.counts[2] = J.W.counts[3] = 0u;
+        const uint32_t i = base + threadIdx.x;
+        if (i < J.W.counts[cur]) {
+            auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, J.spill, i, J.slots);
+            wf_extend_one(J.S, J.W, cur, i, stack);
+        }
+    }
+}
+
+template <bool VOLUMES, bool TEXTURES, bool ENV, bool DISPERSION>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wfb_shade(const WfTable* __restrict__ table, uint32_t cur,
+                                                             uint32_t k) {
+    const WfTable& T = *table;
+    WFB_EACH_CHUNK(T, J, base) {
+        const uint32_t count = J.W.counts[cur];
+        if (base >= count) continue;   // the whole block: whole waves reach the slot ballots
+        const Ctx C{J.S, &J.P};
+        const uint32_t i = base + threadIdx.x;
+        wf_shade_one<VOLUMES, TEXTURES, ENV, DISPERSION>(C, J.W, cur, J.P.sample_offset + k, i, i < count);
+    }
+}
+
+template <bool VOLUMES, int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wfb_shadow(const WfTable* __restrict__ table, uint32_t next) {
+    __shared__ uint2 lds[TRACE_LDS_DEPTH * BLOCK_THREADS];
+    const WfTable& T = *table;
+    WFB_EACH_CHUNK(T, J, base) {
+        const uint32_t i = base + threadIdx.x;
+        if (i < J.W.counts[2]) {
+            const Ctx C{J.S, &J.P};
+            auto stack = BlockStack<TRACE_LDS_DEPTH, TREE>::make(lds, J.spill, i, J.slots);
+            wf_shadow_one<VOLUMES>(C, J.W, next, i, stack);
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wfb_finish(const WfTable* __restrict__ table) {
+    const WfTable& T = *table;
+    WFB_EACH_CHUNK(T, J, base) {
+        const uint32_t i = base + threadIdx.x;
+        if (i < J.W.counts[3]) wf_finish_one(J.P, J.W, i);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK_THREADS) void k_wfb_store(const WfTable* __restrict__ table) {
+    const WfTable& T = *table;
+    WFB_EACH_CHUNK(T, J, base) {
+        const uint32_t i = base + threadIdx.x;
+        if (i < J.W.n) wf_sums(J.W, i).store(J.P, i, J.accum, J.galb, J.gnorm, J.comp);
+    }
+}
+
+#undef WFB_EACH_CHUNK
 #undef WF_EACH
 
 // ------------------------------------------------------------------ launchers
@@ -632,6 +748,39 @@ static void launch_wavefront_v(const LaunchWavefront& L, hipStream_t stream) {
         }
     }
     hipLaunchKernelGGL(k_wf_store, grid, block, 0, stream, L.P, L.W, L.accum, L.galb, L.gnorm, L.comp);
+}
+
+template <bool V, bool T, bool E, bool D>
+static void launch_wavefront_batch_v(const WfTable* table, const WfBatchShape& B, hipStream_t stream) {
+    const dim3 grid(B.blocks), block(BLOCK_THREADS);
+    for (uint32_t k = 0; k < B.samples; ++k) {
+        hipLaunchKernelGGL(k_wfb_begin, grid, block, 0, stream, table, k);
+        for (uint32_t b = 0; b < B.bounces; ++b) {
+            const uint32_t cur = b & 1u;
+            with_tree(B.tree, [&](auto t) {
+                hipLaunchKernelGGL((k_wfb_extend<decltype(t)::value>), grid, block, 0, stream, table, cur);
+            });
+            hipLaunchKernelGGL((k_wfb_shade<V, T, E, D>), grid, block, 0, stream, table, cur, k);
+            with_tree(B.tree, [&](auto t) {
+                hipLaunchKernelGGL((k_wfb_shadow<V, decltype(t)::value>), grid, block, 0, stream, table, cur ^ 1u);
+            });
+            hipLaunchKernelGGL(k_wfb_finish, grid, block, 0, stream, table);
+        }
+    }
+    hipLaunchKernelGGL(k_wfb_store, grid, block, 0, stream, table);
+}
+
+void launch_wavefront_batch(const WfTable* table, const WfBatchShape& B, const Features& f, hipStream_t stream) {
+    int key = (f.volumes ? 8 : 0) | (f.textures ? 4 : 0) | (f.environment ? 2 : 0) | (f.dispersion ? 1 : 0);
+    switch (key) {
+    case 0: launch_wavefront_batch_v<false, false, false, false>(table, B, stream); break;
+    case 1: launch_wavefront_batch_v<false, false, false, true>(table, B, stream); break;
+    case 2: launch_wavefront_batch_v<false, false, true, false>(table, B, stream); break;
+    case 3: launch_wavefront_batch_v<false, false, true, true>(table, B, stream); break;
+    case 8: launch_wavefront_batch_v<true, false, false, false>(table, B, stream); break;
+    case 10: launch_wavefront_batch_v<true, false, true, false>(table, B, stream); break;
+    default: launch_wavefront_batch_v<true, true, true, true>(table, B, stream); break;
+    }
 }
 
 void launch_wavefront(const LaunchWavefront& L, const Features& f, hipStream_t stream) {

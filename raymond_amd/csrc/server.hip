@@ -383,10 +383,15 @@ bool rm_server::enqueue_batched() {
             if (s.job && s.in_flight < 2 && s.job->frames_enqueued < s.job->frames_total && s.job->cancel == 0)
                 ready.push_back(&s);
         if (ready.empty()) return changed;
+        // A batch walks one kind of tree, and runs the per-pixel pass or,
+        // when all its jobs are on the wavefront schedule, the wavefront one.
+        auto wavefront = [](const Slot* s) { return s->ctx->options[RM_OPTION_SCHEDULE] == RM_SCHEDULE_WAVEFRONT; };
         const int tree = rmi::tree_of(ready[0]->ctx, ready[0]->job->scene);
+        const bool wf = wavefront(ready[0]);
         std::vector<Slot*> group;
         for (Slot* s : ready)
-            if (group.size() < rm::MAX_BATCH && rmi::tree_of(s->ctx, s->job->scene) == tree) group.push_back(s);
+            if (group.size() < rm::MAX_BATCH && rmi::tree_of(s->ctx, s->job->scene) == tree && wavefront(s) == wf)
+                group.push_back(s);
         const uint32_t n = (uint32_t)group.size();
         rm_context* ctxs[rm::MAX_BATCH];
         const rm_scene* scenes[rm::MAX_BATCH];

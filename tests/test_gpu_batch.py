@@ -1,8 +1,13 @@
 """Cross-job batching (rm_frame_batch, and the job server under
 RM_OPTION_BATCH): several contexts' path passes in one launch of the
-per-pixel kernel. Each job's frame is computed by the same code on the same
+per-pixel kernel, or, with every context on the wavefront schedule, each
+wavefront stage one launch for all the jobs, the launch sequence replayed
+from a hipGraph. Each job's frame is computed by the same code on the same
 inputs as alone, so its sums are the same bit for bit, even where the batch's
-kernel carries features (volumes, environment, textures) the job lacks."""
+kernel carries features (volumes, environment, textures) the job lacks; the
+wavefront batch is checked after every frame, so every replay of its graph is
+compared with frames rendered alone, with the program's own launches and
+read-backs between the replays."""
 import numpy as np
 import pytest
 
@@ -22,10 +27,23 @@ JOBS = [
 FRAMES = 3
 
 
-def test_batched_frames_match_frames_rendered_alone(ctx, pack):
+def check_against_alone(got, alone, when):
+    for (name, _, f), a, g in zip(JOBS, alone, got):
+        if f.get("caustics"):
+            assert agreement(g[0], a[0], rel=1e-3, abs_=1e-4) > 0.97, f"{name} {when}"
+            continue
+        for what, x, y in zip(("accum", "albedo", "normal", "denoised"), g, a):
+            if what == "denoised" and not f.get("denoise"):
+                continue
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{name}: {what} {when}"
+
+
+@pytest.mark.parametrize("schedule", ["per-pixel", "wavefront"])
+def test_batched_frames_match_frames_rendered_alone(ctx, pack, schedule):
     packs = [pack(c, **o) for c, o, _ in JOBS]
     params = [p.with_params(firefly_clamp=0.0, **f) for p, (_, _, f) in zip(packs, JOBS)]
-    alone = [gpu_render(ctx, p, FRAMES, q, denoised=True) for p, q in zip(packs, params)]
+    # Alone: the per-pixel pass (the wavefront pass gives the same sums).
+    alone = [[gpu_render(ctx, p, k + 1, q, denoised=True) for k in range(FRAMES)] for p, q in zip(packs, params)]
     contexts = [capi.Context(0) for _ in JOBS]
     scenes = []
     try:
@@ -33,25 +51,21 @@ def test_batched_frames_match_frames_rendered_alone(ctx, pack):
             scenes.append(c.upload_scene(p))
             c.upload_environment(p)
             c.resize(p.width, p.height)
+            if schedule == "wavefront":
+                c.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_WAVEFRONT)
         for k in range(FRAMES):
             fp = [frame_params(p, k, q) for p, q in zip(packs, params)]
             em = [p.emitters if scenefile.get_param(q, "caustics") else None for p, q in zip(packs, params)]
             capi.frame_batch(contexts, scenes, fp, em)
-        got = [[c.read(b) for b in (capi.BUFFER_ACCUM, capi.BUFFER_GUIDE_ALBEDO, capi.BUFFER_GUIDE_NORMAL,
-                                    capi.BUFFER_DENOISED)] for c in contexts]
+            if schedule == "wavefront" or k == FRAMES - 1:
+                got = [[c.read(b) for b in (capi.BUFFER_ACCUM, capi.BUFFER_GUIDE_ALBEDO, capi.BUFFER_GUIDE_NORMAL,
+                                            capi.BUFFER_DENOISED)] for c in contexts]
+                check_against_alone(got, [a[k] for a in alone], f"after frame {k + 1}")
     finally:
         for c, s in zip(contexts, scenes):
             s.release()
         for c in contexts:
             c.close()
-    for (name, _, f), a, g in zip(JOBS, alone, got):
-        if f.get("caustics"):
-            assert agreement(g[0], a[0], rel=1e-3, abs_=1e-4) > 0.97, name
-            continue
-        for what, x, y in zip(("accum", "albedo", "normal", "denoised"), g, a):
-            if what == "denoised" and not f.get("denoise"):
-                continue
-            assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{name}: {what}"
 
 
 def test_batch_arguments_are_checked(ctx, pack):
@@ -74,7 +88,8 @@ def test_batch_arguments_are_checked(ctx, pack):
         other.close()
 
 
-def test_server_batches_jobs_that_match_their_frames_alone(ctx, pack):
+@pytest.mark.parametrize("schedule", ["per-pixel", "wavefront"])
+def test_server_batches_jobs_that_match_their_frames_alone(ctx, pack, schedule):
     chosen = [j for j in JOBS if not j[2].get("caustics")]
     packs = [pack(c, **o) for c, o, _ in chosen]
     params = [p.with_params(firefly_clamp=0.0, **f) for p, (_, _, f) in zip(packs, chosen)]
@@ -84,6 +99,8 @@ def test_server_batches_jobs_that_match_their_frames_alone(ctx, pack):
         expected.append(out[3] if f.get("denoise") else out[0])
     with capi.Server(slots=4, memory_budget=8 << 30) as server:
         server.set_option(capi.OPTION_BATCH, 1)
+        if schedule == "wavefront":
+            server.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_WAVEFRONT)
         scenes = [server.context.upload_scene(p) for p in packs]
         spp = [int(q.view(np.uint32)[43]) for q in params]
         ids = [server.submit(s, q, FRAMES * k, packed=p) for p, q, k, s in zip(packs, params, spp, scenes)]
