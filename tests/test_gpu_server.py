@@ -5,6 +5,8 @@ Without light tracing a frame's arithmetic does not depend on scheduling, so
 the images agree bit for bit. With light tracing, photons land in the hash
 grid in atomic order, so gathered sums agree to float rounding; those jobs
 are compared as the light-tracing tests compare against the oracle."""
+import time
+
 import numpy as np
 import pytest
 
@@ -82,15 +84,25 @@ def test_budget_serialises_jobs(pack):
     q = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
     with capi.Server(slots=4, memory_budget=1) as server:
         scene = server.context.upload_scene(p)
-        ids = [server.submit(scene, q, 3) for _ in range(4)]
+        ids, submitted = [], []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            ids.append(server.submit(scene, q, 3))
+            submitted.append((t0, time.perf_counter()))
         results = [server.wait(i) for i in ids]
     first = results[0][0]
     for image, stats in results:
         assert stats["samples"] == 3
         assert np.array_equal(image.view(np.uint32), first.view(np.uint32))
-    # Admitted one after another: each later job waited in the queue.
-    queued = [s["queued_ms"] for _, s in results]
-    assert queued == sorted(queued)
+    # Admitted one after another: job k enters its slot only after job k - 1
+    # is done. Stats are times since each job's own submit, so job k's
+    # admission, measured from job k - 1's submit, is its queued time plus
+    # the gap between the two submits, which lies within the host-side
+    # bracket of the two calls.
+    for k in range(1, len(results)):
+        gap_ms = (submitted[k][1] - submitted[k - 1][0]) * 1e3
+        admitted_ms = results[k][1]["queued_ms"] + gap_ms
+        assert admitted_ms >= results[k - 1][1]["done_ms"], (k, admitted_ms, results[k - 1][1])
 
 
 def test_precision_options_reach_the_jobs(ctx, pack):
