@@ -2,6 +2,7 @@
 // dispatch order (src/Renderer.mjs:441-489), readback and stage timing.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -285,6 +286,8 @@ rm::DevScene dev_scene(const rm_context* c, const rm_scene* s) {
     S.nmap_w = s->nmap_w;
     S.nmap_h = s->nmap_h;
     S.nmap_layers = s->nmap_layers;
+    S.delta_boxes = s->delta.as<const float4>();
+    S.delta_count = s->delta_count;
     return S;
 }
 
@@ -314,7 +317,7 @@ namespace {
 std::vector<DevBuf*> working_buffers(rm_context* c) {
     return {&c->accum,     &c->galb,        &c->gnorm,       &c->light_a,     &c->light_b,   &c->denoised,
             &c->guides,    &c->albedo_frame, &c->splats[0],  &c->splats[1],   &c->gathered[0], &c->gathered[1],
-            &c->heads,     &c->photons,     &c->emitters,    &c->spill,       &c->side_spill, &c->rays,
+            &c->photon_counts, &c->photon_starts, &c->photons, &c->gather_queue, &c->emitters, &c->spill, &c->side_spill, &c->rays,
             &c->hits,      &c->counters,    &c->staging,     &c->display,     &c->comp,      &c->folded,
             &c->wavefront};
 }
@@ -472,7 +475,7 @@ int rm_close(rm_context* c) {
         delete s;
     }
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats[0], &c->splats[1],
-                      &c->guides, &c->albedo_frame, &c->heads, &c->photons, &c->emitters, &c->spill, &c->side_spill,
+                      &c->guides, &c->albedo_frame, &c->photon_counts, &c->photon_starts, &c->photons, &c->gather_queue, &c->emitters, &c->spill, &c->side_spill,
                       &c->gathered[0], &c->gathered[1], &c->env, &c->env_table, &c->rays, &c->hits, &c->counters,
                       &c->work, &c->staging, &c->display, &c->comp, &c->folded, &c->wavefront})
         b->release();
@@ -496,6 +499,97 @@ int rm_close(rm_context* c) {
     (void)hipStreamDestroy(c->stream);
     delete c;
     return RM_OK;
+}
+
+// The delta cut of a packed tree: at most DELTA_CUT_BOXES boxes whose union
+// covers the box of every leaf holding a delta primitive (glass, or metal
+// with roughness under 0.01, as rm::is_delta decides), each box the FP32
+// min/max union of such leaf boxes, so exact. The slab test's entry distance
+// only falls and its exit only grows as a box grows, so a ray that misses
+// every cut box misses every such leaf's box, and the FP32 walk, which
+// slab-tests a leaf's box before its primitives, never tests a delta
+// primitive for it: its closest hit is not a delta surface. The photon
+// gather's camera paths end at a first hit that isn't one, so it skips
+// tracing them. The cut starts at the root's box and splits the box of
+// largest area into its children's, while it has room. False (no cut) when
+// the nodes do not form a tree or a delta leaf's box isn't finite.
+constexpr size_t DELTA_CUT_BOXES = 16;
+static bool delta_cut(const uint32_t* nw, uint32_t node_count, const uint32_t* pw, const uint32_t* mw,
+                      std::vector<float>& out) {
+    struct Box { float lo[3], hi[3]; bool any = false; };
+    std::vector<Box> box(node_count);
+    std::vector<uint8_t> seen(node_count, 0);
+    auto node_box = [&](uint32_t i, Box& b) {
+        for (int a = 0; a < 3; ++a) {
+            std::memcpy(&b.lo[a], &nw[i * 8 + a], 4);
+            std::memcpy(&b.hi[a], &nw[i * 8 + 4 + a], 4);
+        }
+    };
+    auto is_delta = [&](uint32_t prim) {
+        const uint32_t m = pw[prim * 16 + 1];
+        const uint32_t kind = mw[m * 24 + 3];
+        float roughness;
+        std::memcpy(&roughness, &mw[m * 24 + 7], 4);
+        return kind == rm::GLASS || (kind == rm::METAL && roughness < 0.01f);
+    };
+    // Children before parents: a node is finished when popped the second time.
+    std::vector<std::pair<uint32_t, bool>> stack{{0u, false}};
+    while (!stack.empty()) {
+        const auto [i, children_done] = stack.back();
+        stack.pop_back();
+        const uint32_t first = nw[i * 8 + 3], count = nw[i * 8 + 7];
+        Box& b = box[i];
+        if (!children_done) {
+            if (seen[i]++) return false;
+            if (count == 0) {
+                stack.push_back({i, true});
+                stack.push_back({first, false});
+                stack.push_back({first + 1, false});
+                continue;
+            }
+            for (uint32_t k = first; k < first + count && !b.any; ++k) b.any = pw[k * 16] <= rm::TRIANGLE && is_delta(k);
+            if (b.any) {
+                node_box(i, b);
+                for (int a = 0; a < 3; ++a)
+                    if (!std::isfinite(b.lo[a]) || !std::isfinite(b.hi[a])) return false;
+            }
+            continue;
+        }
+        for (uint32_t child : {first, first + 1}) {
+            const Box& c = box[child];
+            if (!c.any) continue;
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = b.any ? std::fmin(b.lo[a], c.lo[a]) : c.lo[a];
+                b.hi[a] = b.any ? std::fmax(b.hi[a], c.hi[a]) : c.hi[a];
+            }
+            b.any = true;
+        }
+    }
+    std::vector<uint32_t> cut;
+    if (box[0].any) cut.push_back(0);
+    auto area = [&](uint32_t i) {
+        const Box& b = box[i];
+        const double x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+        return x * y + y * z + z * x;
+    };
+    for (;;) {
+        size_t split = cut.size();
+        for (size_t k = 0; k < cut.size(); ++k)
+            if (nw[cut[k] * 8 + 7] == 0 && (split == cut.size() || area(cut[k]) > area(cut[split]))) split = k;
+        if (split == cut.size()) break;
+        const uint32_t first = nw[cut[split] * 8 + 3];
+        const size_t grows = (box[first].any ? 1 : 0) + (box[first + 1].any ? 1 : 0) - 1;
+        if (cut.size() + grows > DELTA_CUT_BOXES) break;
+        cut.erase(cut.begin() + split);
+        for (uint32_t child : {first, first + 1})
+            if (box[child].any) cut.push_back(child);
+    }
+    out.clear();
+    for (uint32_t i : cut) {
+        const Box& b = box[i];
+        out.insert(out.end(), {b.lo[0], b.lo[1], b.lo[2], 0.0f, b.hi[0], b.hi[1], b.hi[2], 0.0f});
+    }
+    return true;
 }
 
 int rm_scene_upload(rm_context* c, const void* primitives, uint32_t primitive_slots, uint32_t primitive_count,
@@ -549,6 +643,15 @@ int rm_scene_upload(rm_context* c, const void* primitives, uint32_t primitive_sl
             r = upload(c, s->wnodes, w.data(), (size_t)wide * 64, "uploading the wide quantized BVH");
         if (r == RM_OK) fail(RM_OK, before);
         if (r == RM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(RM_ERR_DEVICE, "uploading a scene");
+    }
+    if (r == RM_OK) {
+        std::vector<float> cut;
+        if (delta_cut(nw, node_count, pw, static_cast<const uint32_t*>(materials), cut)) {
+            const uint32_t boxes = (uint32_t)(cut.size() / 8);
+            if (cut.empty()) cut.assign(8, 0.0f);   // an address for an empty cut
+            r = upload(c, s->delta, cut.data(), cut.size() * 4, "uploading the delta cut");
+            if (r == RM_OK) s->delta_count = boxes;
+        }
     }
     if (r == RM_OK) r = upload(c, s->mats, materials, (size_t)material_count * 96, "uploading materials");
     if (r == RM_OK) r = upload(c, s->attr, attributes, (size_t)attribute_vec4s * 16, "uploading attributes");
@@ -666,6 +769,9 @@ struct Frame {
     bool wavefront = false, sorted = false;
 };
 
+// The photon gather's queue for px pixels: a 96-byte slot each, then its length.
+static size_t gather_queue_bytes(size_t px) { return px * rm::GATHER_SLOT_VEC4S * 16 + 256; }
+
 // The checks of rm_frame's arguments, before anything is enqueued.
 static int frame_check(rm_context* c, const rm_scene* s, const void* params, const float* emitters,
                        uint32_t emitter_vec4s, Frame& F) {
@@ -726,6 +832,8 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
         const uint32_t count = budget > 0 ? (uint32_t)budget : rm::photon_count(P);
         F.photons = count;
         F.capacity = budget > 0 ? count : rm::PHOTON_CAPACITY;
+        // Traversal spill lanes: light tracing's, or the gather's (a lane a
+        // pixel), which runs after it on the same stream.
         const uint32_t light_lanes = count < rm::LIGHT_LANES ? count : rm::LIGHT_LANES;
         F.side_slots = light_lanes > px ? light_lanes : px;
         const uint32_t pair = F.pair = c->pair;
@@ -735,12 +843,16 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
         // old: earlier frames' light tracing may still be using either.
         const bool same_emitters = c->emitters_host.size() == emitter_bytes &&
                                    std::memcmp(c->emitters_host.data(), emitters, emitter_bytes) == 0;
-        if (!same_emitters || c->heads.bytes == 0 || c->photons.bytes == 0 ||
-            c->gathered[pair].bytes < (size_t)px * 16 || c->side_spill.bytes < spill_bytes)
+        const size_t queue_bytes = gather_queue_bytes(px);
+        if (!same_emitters || c->photon_counts.bytes == 0 || c->photon_starts.bytes == 0 ||
+            c->photons.bytes < (size_t)F.capacity * 96 || c->gather_queue.bytes < queue_bytes ||
+            c->gathered[pair].bytes < (size_t)px * 16 || c->side_spill.bytes < spill_bytes) {
             RM_HIP(hipStreamSynchronize(c->side), "waiting for light tracing");
-        RM_HIP(c->heads.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
-        if (c->photons.bytes < (size_t)F.capacity * 48) RM_HIP(hipStreamSynchronize(c->side), "waiting for light tracing");
-        RM_HIP(c->photons.ensure((size_t)F.capacity * 48), "allocating photons");
+        }
+        RM_HIP(c->gather_queue.ensure(queue_bytes), "allocating the gather queue");
+        RM_HIP(c->photon_counts.ensure((size_t)(rm::PHOTON_TABLE + 1) * 4), "allocating the photon grid");
+        RM_HIP(c->photon_starts.ensure((size_t)(rm::PHOTON_TABLE + 2 * rm::PHOTON_GROUPS) * 4), "allocating the photon grid");
+        RM_HIP(c->photons.ensure((size_t)F.capacity * 96), "allocating photons");
         RM_HIP(c->gathered[pair].ensure((size_t)px * 16), "allocating the gathered light");
         RM_HIP(c->side_spill.ensure(spill_bytes), "allocating the side traversal spill");
         if (!same_emitters) {
@@ -794,6 +906,19 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
     return RM_OK;
 }
 
+// The context's photon map for a frame of `capacity` photon slots (frame_alloc
+// sized its buffers): the deposits, then their sorted positions and powers.
+static rm::PhotonMap photon_map(rm_context* c, uint32_t capacity) {
+    rm::PhotonMap M;
+    M.counts = c->photon_counts.as<uint32_t>();
+    M.starts = c->photon_starts.as<uint32_t>();
+    M.deposits = c->photons.as<float4>();
+    M.positions = M.deposits + (size_t)capacity * 3;
+    M.powers = M.positions + capacity;
+    M.capacity = capacity;
+    return M;
+}
+
 // Everything before the path pass: light tracing and the photon gather on
 // the side stream, the compensation terms.
 static int frame_begin(rm_context* c, Frame& F) {
@@ -813,22 +938,25 @@ static int frame_begin(rm_context* c, Frame& F) {
     if (F.caustics) {
         const uint32_t pair = F.pair;
         c->pair = pair ^ 1u;
+        const rm::PhotonMap M = photon_map(c, F.capacity);
+        const rm::GatherQueue Q{c->gather_queue.as<float4>(),
+                                reinterpret_cast<uint32_t*>(c->gather_queue.as<char>() + gather_queue_bytes(F.px) - 256),
+                                F.px};
+        float4* gathered = c->gathered[pair].as<float4>();
         RM_HIP(hipStreamWaitEvent(c->side, c->resolved_ev[pair], 0), "ordering light tracing");
         {
             Timed t(c, RM_STAGE_CLEAR, c->side);
-            rm::launch_fill_u32(c->heads.as<uint32_t>(), 0u, rm::PHOTON_TABLE + 1, c->side);
+            rm::launch_fill_u32(M.counts, 0u, rm::PHOTON_TABLE + 1, c->side);
         }
         {
             Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
-            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(),
-                                   c->heads.as<uint32_t>(), c->photons.as<float4>(), c->side_spill.as<uint32_t>(),
-                                   F.side_slots, F.photons, F.capacity, f, c->side);
+            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(), M,
+                                   c->side_spill.as<uint32_t>(), F.side_slots, F.photons, f, c->side);
         }
         {
             Timed t(c, RM_STAGE_GATHER, c->side);
-            rm::launch_photon_gather(S, P, c->heads.as<const uint32_t>(), c->photons.as<const float4>(),
-                                     c->gathered[pair].as<float4>(), c->side_spill.as<uint32_t>(), F.side_slots, f,
-                                     c->side);
+            rm::launch_photon_sort(M, c->side);
+            rm::launch_photon_gather(S, P, M, Q, gathered, c->side_spill.as<uint32_t>(), F.side_slots, f, c->side);
         }
         RM_HIP(hipEventRecord(c->join, c->side), "joining the side stream");
         RM_HIP(hipGetLastError(), "launching light tracing");

@@ -92,13 +92,17 @@ struct rm_scene {
     rmi::DevBuf prims, nodes, mats, attr, lights, textures, nmaps;
     rmi::DevBuf qnodes;   // the 8-bit child-quantized tree; empty when the tree can't be quantized
     rmi::DevBuf wnodes;   // the 4-wide quantized tree; empty when the tree can't be folded
+    // The delta cut (rm::DevScene::delta_boxes): delta_count box pairs, or
+    // rm::NO_DELTA_CUT when the nodes gave none (see delta_cut in api.hip).
+    rmi::DevBuf delta;
+    uint32_t delta_count = rm::NO_DELTA_CUT;
     uint32_t primitive_slots = 0, primitive_count = 0, node_count = 0, material_count = 0;
     uint32_t tex_w = 0, tex_h = 0, tex_layers = 0, nmap_w = 0, nmap_h = 0, nmap_layers = 0;
     bool dispersive = false;
     std::vector<uint32_t> lights_host;   // the lights table, for checking parameters against
     std::vector<uint8_t> kinds_host;     // each primitive slot's kind, for checking emitters against
     void release() {
-        for (rmi::DevBuf* b : {&prims, &nodes, &mats, &attr, &lights, &textures, &nmaps, &qnodes, &wnodes})
+        for (rmi::DevBuf* b : {&prims, &nodes, &mats, &attr, &lights, &textures, &nmaps, &qnodes, &wnodes, &delta})
             b->release();
     }
 };
@@ -127,7 +131,10 @@ struct rm_context {
     rmi::DevBuf splats[2], gathered[2];
     hipEvent_t resolved_ev[2] = {nullptr, nullptr};
     uint32_t pair = 0;   // the pair the next frame with light tracing uses
-    rmi::DevBuf heads, photons, emitters, spill, side_spill;
+    // The photon map (rm::PhotonMap): cell counts, cell starts, and the
+    // photons' deposits with their sorted copy (96 bytes a photon slot).
+    rmi::DevBuf photon_counts, photon_starts, photons, emitters, spill, side_spill;
+    rmi::DevBuf gather_queue;   // the photon gather's queue (rm::GatherQueue): a slot a pixel, then its length
     std::vector<uint8_t> emitters_host;   // the bytes now in `emitters`
     rmi::DevBuf env, env_table;
     uint32_t env_w = 1, env_h = 1, table_w = 1, table_h = 0;

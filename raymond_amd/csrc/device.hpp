@@ -117,6 +117,8 @@ struct Mat {
     float density, anisotropy;
 };
 
+constexpr uint32_t NO_DELTA_CUT = 0xffffffffu;
+
 // The device-resident scene. Every pointer is device memory.
 struct DevScene {
     const float4* prims;       // 4 per primitive
@@ -134,6 +136,11 @@ struct DevScene {
     uint32_t tex_w, tex_h, tex_layers;
     const uchar4* nmaps;       // RGBA8 layers
     uint32_t nmap_w, nmap_h, nmap_layers;
+    // The delta cut of the FP32 tree: delta_count box pairs (min, max) whose
+    // union covers every leaf box holding a delta primitive (rm_scene_upload
+    // builds it); NO_DELTA_CUT when there is none to rely on.
+    const float4* delta_boxes = nullptr;
+    uint32_t delta_count = NO_DELTA_CUT;
 };
 
 RM_D Mat load_mat(const DevScene& S, uint32_t i) {

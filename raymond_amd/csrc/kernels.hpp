@@ -157,19 +157,55 @@ __host__ __device__ inline uint32_t photon_count(const Params& P) {
     return n < PHOTON_CAPACITY ? (uint32_t)n : PHOTON_CAPACITY;
 }
 
+// A frame's photon map. Light tracing deposits each photon in the slot its
+// wave claims and counts it in its hash cell (the reference's 2^20-cell
+// grid, src/shaders.mjs:1499-1528); launch_photon_sort then lays the photons
+// out cell by cell, so a gather reads each cell's photons as one contiguous
+// run rather than following a linked list one dependent load at a time.
+constexpr uint32_t PHOTON_GROUP_CELLS = 4096;                          // cells one scan block covers
+constexpr uint32_t PHOTON_GROUPS = PHOTON_TABLE / PHOTON_GROUP_CELLS;  // 256
+struct PhotonMap {
+    uint32_t* counts;    // PHOTON_TABLE + 1 words: photons in each cell; [PHOTON_TABLE] counts the slots claimed
+    uint32_t* starts;    // PHOTON_TABLE + 2 · PHOTON_GROUPS words: each cell's first sorted slot within its group
+                         // of PHOTON_GROUP_CELLS cells, each group's photon count, then each group's first slot
+    float4* deposits;    // capacity × 3, slot order: (position, cell), (power, rank within the cell), (to the light, 0)
+    float4* positions;   // capacity, cell order: (position, 0)
+    float4* powers;      // capacity × 2, cell order: (power, 0), (to the light, 0)
+    uint32_t capacity;   // photon slots; deposits past it are dropped, as the reference drops them
+};
+// Bytes of the photon map's buffers other than the counts: starts, and the
+// deposits and their sorted copy for `capacity` photons.
+inline uint64_t photon_map_bytes(uint64_t capacity) {
+    return (uint64_t)(PHOTON_TABLE + 2 * PHOTON_GROUPS) * 4 + capacity * 96;
+}
+
 // Light tracing: count light paths, photon ids 0 .. count - 1, each depositing
-// at most one photon into `photons` (room for `capacity`); the grid is
-// at most LIGHT_LANES threads, and `spill` has room for that many lanes.
+// at most one photon into M (whose counts start at zero); the grid is at most
+// LIGHT_LANES threads, and `spill` has room for that many lanes.
 constexpr uint32_t LIGHT_LANES = 1u << 20;
-void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
-                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, uint32_t capacity,
-                        const Features& f, hipStream_t stream);
+void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
+                        const PhotonMap& M, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
+                        hipStream_t stream);
+// Counting sort of the deposited photons by cell: an exclusive scan of the
+// cell counts, then each photon copied to its cell's run at its rank.
+void launch_photon_sort(const PhotonMap& M, hipStream_t stream);
 void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, float4* comp,
                     hipStream_t stream);
 // sum + error of the compensated sums, into out: planes accum, galb, gnorm of W·H float4.
 void launch_fold(const Params& P, const float4* accum, const float4* galb, const float4* gnorm, const float4* comp,
                  float4* out, hipStream_t stream);
-void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
+// The photon gather's queue of camera paths, each replaced in its slot by
+// its gather point: a slot of GATHER_SLOT_VEC4S float4 for each of px
+// pixels, and the number queued.
+constexpr uint32_t GATHER_SLOT_VEC4S = 6;
+struct GatherQueue {
+    float4* slots;
+    uint32_t* count;
+    uint32_t px;
+};
+// Every pixel's gathered light into `gathered` (w = 1 where it adds any),
+// once the photon map is sorted; spill has room for W·H lanes of `slots`.
+void launch_photon_gather(const DevScene& S, const Params& P, const PhotonMap& M, const GatherQueue& Q,
                           float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream);
 // fp16: the guide buffer holds FP16 (8 bytes a pixel) rather than FP32.
 void launch_guides(const Params& P, const float4* gnorm, void* guides, bool fp16, hipStream_t stream);

@@ -1,5 +1,7 @@
-// Light tracing (k_light_trace), the splat and gather resolve (k_resolve),
-// the photon gather (k_photon_gather), and the fold of compensated sums.
+// Light tracing (k_light_trace), the photon map's counting sort
+// (k_photon_scan, k_photon_sort), the splat and gather
+// resolve (k_resolve), the photon gather (k_gather_begin, k_gather_paths,
+// k_gather_walk), and the fold of compensated sums.
 #include "path.hpp"
 
 namespace rm {
@@ -42,6 +44,12 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
     atomicAdd(&splats[index + 2u], to_u32(fx.z));
 }
 
+// The light-tracing and gather kernels are held to 96 VGPRs (five waves per
+// SIMD): they run beside the path pass, and a wave of theirs then fits where
+// one path-pass wave at five waves per SIMD (96 VGPRs) retires. At more than
+// 96, on the caustics scene, light tracing waited for two to retire and took
+// 2.5 ms a frame beside the path pass instead of 0.64.
+//
 // Light paths (src/shaders.mjs:1630-1814), photons_n of them a frame: the
 // reference's photon_count(P) by default, or a budget the caller sets. A
 // block takes photon ids first, first + 1, ... with first advancing by the
@@ -51,12 +59,13 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
 // one counter atomic per wave (ballot + prefix count): photon slots are
 // claimed in a different order than one atomic per lane would give, which
 // the reference leaves unspecified too; slots at or past `capacity` are
-// dropped, as the reference drops them past its fixed capacity.
+// dropped, as the reference drops them past its fixed capacity. A deposit
+// counts its photon in its cell (the rank it gets orders the cell's photons
+// for the sort) where the reference links it into the cell's list.
 template <bool TEXTURES, bool DISPERSION, int TREE>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Params P, const float4* __restrict__ E,
-                                                               uint32_t* splats, uint32_t* heads,
-                                                               float4* __restrict__ photons, uint32_t* spill,
-                                                               uint32_t slots, uint32_t photons_n, uint32_t capacity) {
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_light_trace(DevScene S, Params P, const float4* __restrict__ E,
+                                                               uint32_t* splats, PhotonMap M, uint32_t* spill,
+                                                               uint32_t slots, uint32_t photons_n) {
     __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
     const Ctx C{S, &P};
     const uint32_t lane = blockIdx.x * BLOCK_THREADS + threadIdx.x;
@@ -211,24 +220,92 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_light_trace(DevScene S, Param
         uint32_t wave_lane = __lane_id();
         uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
         uint32_t base = 0;
-        if (wave_lane == leader) base = atomicAdd(&heads[PHOTON_TABLE], (uint32_t)__popcll(mask));
+        if (wave_lane == leader) base = atomicAdd(&M.counts[PHOTON_TABLE], (uint32_t)__popcll(mask));
         base = __shfl(base, (int)leader);
         if (!pending) continue;
         uint32_t i = base + (uint32_t)__popcll(mask & ((1ull << wave_lane) - 1ull));
-        if (i >= capacity) continue;
+        if (i >= M.capacity) continue;
         V3 q = dep_p / P.photon_cell;
         uint32_t h = photon_cell_hash(to_i32(floorf(q.x)), to_i32(floorf(q.y)), to_i32(floorf(q.z)));
-        uint32_t next = atomicExch(&heads[h], i + 1u);
-        photons[(size_t)i * 3] = make_float4(dep_p.x, dep_p.y, dep_p.z, __uint_as_float(next));
-        photons[(size_t)i * 3 + 1] = make_float4(dep_beta.x, dep_beta.y, dep_beta.z, 0.0f);
-        photons[(size_t)i * 3 + 2] = make_float4(-dep_d.x, -dep_d.y, -dep_d.z, 0.0f);
+        uint32_t rank = atomicAdd(&M.counts[h], 1u);
+        M.deposits[(size_t)i * 3] = make_float4(dep_p.x, dep_p.y, dep_p.z, __uint_as_float(h));
+        M.deposits[(size_t)i * 3 + 1] = make_float4(dep_beta.x, dep_beta.y, dep_beta.z, __uint_as_float(rank));
+        M.deposits[(size_t)i * 3 + 2] = make_float4(-dep_d.x, -dep_d.y, -dep_d.z, 0.0f);
     }
 }
 
-// resolve_splats() then photon_gather()'s sum into the accumulation
-// (src/shaders.mjs:1819-1831, 1610-1624), in that order, for one pixel. The
-// gather pass leaves its light in `gathered` (w = 1 where it adds any), so it
-// can run while the path-tracing pass still writes the accumulation.
+// ------------------------------------------------------------------ photon sort
+// An exclusive scan over a block's 256 threads of one value each; every
+// thread gets the sum of the values before its own, and `total` the block's.
+RM_D uint32_t block_exclusive_scan(uint32_t v, uint32_t* wave_totals, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t off = 1u; off < 64u; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63u) wave_totals[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0u;
+    total = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < BLOCK_THREADS / 64; ++w) {
+        before += w < wave ? wave_totals[w] : 0u;
+        total += wave_totals[w];
+    }
+    return before + incl - v;
+}
+
+// Each block scans one group of PHOTON_GROUP_CELLS cell counts, 16 cells a
+// thread: every cell's first slot within its group, and the group's total.
+static_assert(PHOTON_GROUP_CELLS == BLOCK_THREADS * 16 && PHOTON_GROUPS == BLOCK_THREADS, "scan layout");
+__global__ __launch_bounds__(BLOCK_THREADS) void k_photon_scan(const uint32_t* __restrict__ counts,
+                                                               uint32_t* __restrict__ starts) {
+    __shared__ uint32_t wave_totals[BLOCK_THREADS / 64];
+    const uint32_t base = blockIdx.x * PHOTON_GROUP_CELLS + threadIdx.x * 16u;
+    const uint4* in = reinterpret_cast<const uint4*>(counts + base);
+    uint4 c[4];
+    uint32_t run = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 v = in[k];
+        c[k] = make_uint4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
+        run += v.x + v.y + v.z + v.w;
+    }
+    uint32_t total;
+    const uint32_t before = block_exclusive_scan(run, wave_totals, total);
+    uint4* out = reinterpret_cast<uint4*>(starts + base);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = make_uint4(c[k].x + before, c[k].y + before, c[k].z + before, c[k].w + before);
+    if (threadIdx.x == 0) starts[PHOTON_TABLE + blockIdx.x] = total;
+}
+
+// Every deposited photon to its cell's run, at its rank: the runs together
+// hold exactly the photons in slots below min(claimed, capacity). Each block
+// first scans the groups' counts in LDS for their first slots (block 0 keeps
+// them for the gather's walk).
+__global__ __launch_bounds__(BLOCK_THREADS) void k_photon_sort(PhotonMap M) {
+    __shared__ uint32_t wave_totals[BLOCK_THREADS / 64];
+    __shared__ uint32_t group_first[PHOTON_GROUPS];
+    uint32_t total;
+    const uint32_t first = block_exclusive_scan(M.starts[PHOTON_TABLE + threadIdx.x], wave_totals, total);
+    group_first[threadIdx.x] = first;
+    if (blockIdx.x == 0) M.starts[PHOTON_TABLE + PHOTON_GROUPS + threadIdx.x] = first;
+    __syncthreads();
+    const uint32_t claimed = M.counts[PHOTON_TABLE];
+    const uint32_t n = claimed < M.capacity ? claimed : M.capacity;
+    for (uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x; i < n; i += gridDim.x * BLOCK_THREADS) {
+        const float4 a = M.deposits[(size_t)i * 3], b = M.deposits[(size_t)i * 3 + 1];
+        const float4 d = M.deposits[(size_t)i * 3 + 2];
+        const uint32_t h = __float_as_uint(a.w);
+        const uint32_t at = group_first[h / PHOTON_GROUP_CELLS] + M.starts[h] + __float_as_uint(b.w);
+        M.positions[at] = make_float4(a.x, a.y, a.z, 0.0f);
+        M.powers[(size_t)at * 2] = make_float4(b.x, b.y, b.z, 0.0f);
+        M.powers[(size_t)at * 2 + 1] = d;
+    }
+}
+
 __global__ __launch_bounds__(BLOCK_THREADS) void k_resolve(Params P, uint4* splats, const float4* gathered,
                                                            float4* accum, float4* comp) {
     uint32_t i = blockIdx.x * BLOCK_THREADS + threadIdx.x;
@@ -253,122 +330,328 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_fold(Params P, const float4* 
     }
 }
 
-// The photons within the radius of a surface, times the BSDF, over the
-// disc's area (src/shaders.mjs:1534-1570).
-RM_D V3 gather(const Ctx& C, const uint32_t* heads, const float4* photons, const Surface& s, const Mat& m, V3 base,
-               V3 wo) {
-    const Params& P = *C.P;
-    float r = P.photon_radius, r2 = r * r;
-    V3 here = s.p / P.photon_cell;
-    int32_t cx = to_i32(floorf(here.x)), cy = to_i32(floorf(here.y)), cz = to_i32(floorf(here.z));
-    int32_t sx = (here.x - floorf(here.x)) >= 0.5f ? 1 : -1;
-    int32_t sy = (here.y - floorf(here.y)) >= 0.5f ? 1 : -1;
-    int32_t sz = (here.z - floorf(here.z)) >= 0.5f ? 1 : -1;
+// ------------------------------------------------------------------ photon gather
+// photon_gather() (src/shaders.mjs:1577-1626), once the photon map is sorted:
+//   k_gather_begin  every pixel's camera ray to its first hit. A path whose
+//                   first hit is a delta surface bounces there and goes on.
+//                   A wave where many paths go on carries each to its
+//                   gather point (its first non-delta surface) and gathers
+//                   there, a lane a point, as the reference does. A wave
+//                   where few go on queues them instead, so that they don't
+//                   hold the wave through their bounces and walks while its
+//                   other lanes idle;
+//   k_gather_paths  each queued path on to its gather point, a thread a path,
+//                   the point kept in the path's slot;
+//   k_gather_walk   the queued points' photons, eight lanes to a point.
+// A queue slot is 6 float4. A path: (o, pixel), (d, rng state), (throughput,
+// next depth). A gather point: (p, pixel), (n, material), (ng, 0), (base, 0),
+// (wo, 0), (throughput, 0); a path that ends without one leaves GATHER_DEAD
+// as its pixel. Every pixel's gathered[] is written once.
+constexpr uint32_t GATHER_DEAD = 0xffffffffu;
+// Lanes of a wave whose paths go on past their first hit, from which the
+// wave carries them on itself rather than queueing them.
+constexpr uint32_t GATHER_DENSE_LANES = 48u;
+
+// Whether the ray's slab test passes a box of the scene's delta cut: the
+// same reciprocal direction and slab test as the FP32 walk's, so a ray that
+// passes none is one whose walk tests no delta primitive.
+RM_D bool may_meet_delta(const DevScene& S, V3 o, V3 d) {
+    const V3 sd = safe(d);
+    const V3 inv = {1.0f / sd.x, 1.0f / sd.y, 1.0f / sd.z};
+    for (uint32_t i = 0u; i < S.delta_count; ++i)
+        if (slabs(S.delta_boxes[2 * i], S.delta_boxes[2 * i + 1], o, inv, INF) < INF) return true;
+    return false;
+}
+
+// Where a camera path gathers, with what gather() reads of it.
+struct GatherPoint { V3 p, n, ng, base, wo, throughput; uint32_t material; };
+
+// What a gather path reaches in one bounce.
+enum class GatherStep { End, Bounce, Point };
+
+// One bounce of a gather path at depth `depth` from o along d: at a
+// non-delta surface past the first hit, its gather point q; at a delta
+// surface, its bounce (o, d and throughput carry on).
+template <bool TEXTURES, class STACK>
+RM_D GatherStep gather_bounce(const Ctx& C, Rng& g, STACK& stack, uint32_t depth, V3& o, V3& d, V3& throughput,
+                              GatherPoint& q) {
+    const DevScene& S = C.S;
+    NoCount nc;
+    Hit hit = trace<false>(S, o, d, INF, stack, nc);
+    if (hit.prim == NONE) return GatherStep::End;
+    // The hit primitive's material alone decides whether the path ends
+    // here (a light, or any surface but a delta one at the first hit), so
+    // most camera rays stop before their shading point is built.
+    const uint32_t material = __float_as_uint(S.prims[(size_t)hit.prim * 4].y);
+    Mat m = load_mat(S, material);
+    if (m.kind == LIGHT || (depth == 0u && !is_delta(m))) return GatherStep::End;
+    Surface s = surface<TEXTURES>(S, o, d, hit);
+    V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
+    if (!is_delta(m)) {
+        q = {s.p, s.n, s.ng, base, -d, throughput, material};
+        return GatherStep::Point;
+    }
+    BsdfSample bs = sample_bsdf(g, m, base, s, -d);
+    if (!bs.ok) return GatherStep::End;
+    throughput = throughput * bs.weight;
+    o = offset(s, bs.wi);
+    d = bs.wi;
+    return GatherStep::Bounce;
+}
+
+// A gather path from depth `depth` on to its gather point q; false when it
+// ends without one.
+template <bool TEXTURES, class STACK>
+RM_D bool gather_path(const Ctx& C, Rng& g, STACK& stack, uint32_t depth, V3 o, V3 d, V3 throughput,
+                      GatherPoint& q) {
+    for (; depth < C.P->max_bounces; depth++) {
+        const GatherStep step = gather_bounce<TEXTURES>(C, g, stack, depth, o, d, throughput, q);
+        if (step != GatherStep::Bounce) return step == GatherStep::Point;
+    }
+    return false;
+}
+
+// The eight cells a gather point at p reads (the reference's cell and its
+// nearer neighbour along each axis, in its order): cell k's hash-table
+// entry, for k = 0 .. 7.
+RM_D uint32_t gather_cell(const Params& P, V3 p, uint32_t k) {
+    const V3 here = p / P.photon_cell;
+    const int32_t cx = to_i32(floorf(here.x)), cy = to_i32(floorf(here.y)), cz = to_i32(floorf(here.z));
+    const int32_t sx = (here.x - floorf(here.x)) >= 0.5f ? 1 : -1;
+    const int32_t sy = (here.y - floorf(here.y)) >= 0.5f ? 1 : -1;
+    const int32_t sz = (here.z - floorf(here.z)) >= 0.5f ? 1 : -1;
+    return photon_cell_hash(cx + (int32_t)(k & 1u) * sx, cy + (int32_t)((k >> 1u) & 1u) * sy,
+                            cz + (int32_t)((k >> 2u) & 1u) * sz);
+}
+// The sorted slot of cell h's newest photon (its run ends there), given its count.
+RM_D uint32_t cell_last(const PhotonMap& M, uint32_t h, uint32_t count) {
+    return M.starts[PHOTON_TABLE + PHOTON_GROUPS + h / PHOTON_GROUP_CELLS] + M.starts[h] + count - 1u;
+}
+// One photon's term of gather()'s sum at q (zero when it is out of reach).
+RM_D V3 photon_term(const PhotonMap& M, uint32_t at, const GatherPoint& q, const Mat& m, float r, float r2) {
+    const float4 pos = M.positions[at];
+    V3 off = xyz(pos) - q.p;
+    if (dot(off, off) < r2 && fabsf(dot(off, q.ng)) < 0.25f * r) {
+        const V3 wi = xyz(M.powers[(size_t)at * 2 + 1]);
+        float cos_in = dot(q.n, wi);
+        if (cos_in > 0.0f && dot(q.ng, wi) > 0.0f) {
+            Eval ev = eval_bsdf(m, q.base, q.n, q.wo, wi);
+            return xyz(M.powers[(size_t)at * 2]) * (ev.f / cos_in);
+        }
+    }
+    return {0.0f, 0.0f, 0.0f};
+}
+
+// gather() (src/shaders.mjs:1534-1570) by one lane, in the reference's order:
+// the eight cells in turn, each newest deposit first (the reference's lists
+// are pushed at the head), at most 4096 photons in all. The run's slots are
+// known up front, so unrolled iterations load their photons together.
+RM_D V3 gather_lane(const Params& P, const PhotonMap& M, const GatherPoint& q, const Mat& m) {
+    const float r = P.photon_radius, r2 = r * r;
     V3 sum = {0, 0, 0};
     uint32_t visited = 0u;
-    for (uint32_t k = 0u; k < 8u; k++) {
-        uint32_t index = heads[photon_cell_hash(cx + (int32_t)(k & 1u) * sx, cy + (int32_t)((k >> 1u) & 1u) * sy,
-                                                cz + (int32_t)((k >> 2u) & 1u) * sz)];
-        while (index != 0u && visited < 4096u) {
-            visited++;
-            float4 at = photons[(size_t)(index - 1u) * 3];
-            V3 off = xyz(at) - s.p;
-            if (dot(off, off) < r2 && fabsf(dot(off, s.ng)) < 0.25f * r) {
-                V3 wi = xyz(photons[(size_t)(index - 1u) * 3 + 2]);
-                float cos_in = dot(s.n, wi);
-                if (cos_in > 0.0f && dot(s.ng, wi) > 0.0f) {
-                    Eval e = eval_bsdf(m, base, s.n, wo, wi);
-                    sum = sum + xyz(photons[(size_t)(index - 1u) * 3 + 1]) * (e.f / cos_in);
-                }
-            }
-            index = __float_as_uint(at.w);
-        }
+    for (uint32_t k = 0u; k < 8u && visited < 4096u; k++) {
+        const uint32_t h = gather_cell(P, q.p, k);
+        const uint32_t count = M.counts[h];
+        const uint32_t last = cell_last(M, h, count);
+        const uint32_t take = count < 4096u - visited ? count : 4096u - visited;
+#pragma unroll 4
+        for (uint32_t i = 0u; i < take; i++) sum = sum + photon_term(M, last - i, q, m, r, r2);
+        visited += take;
     }
     return sum / (PI * r2);
 }
 
-// src/shaders.mjs:1577-1626. The pixel's gathered light goes to
-// gathered[index] (w = 1), or zeros when it adds none; k_resolve adds it.
-template <bool TEXTURES, class STACK>
-RM_D float4 gather_pixel(const Ctx& C, const uint32_t* heads, const float4* photons, uint32_t x, uint32_t y,
-                         uint32_t index, STACK& stack) {
-    const Params& P = *C.P;
-    const DevScene& S = C.S;
-    NoCount nc;
-    Rng g;
-    g.qmc = false;
-    g.state = pcg(index ^ pcg(P.sample_offset * 2654435769u + 4099u));
-    float jx = g.rand();
-    float jy = g.rand();
-    Ray ray = camera_ray(P, g, {(float)x + jx, (float)y + jy}, true);
-    V3 o = ray.o, d = ray.d;
-    V3 throughput = {1, 1, 1};
-    for (uint32_t depth = 0u; depth < P.max_bounces; depth++) {
-        Hit hit = trace<false>(S, o, d, INF, stack, nc);
-        if (hit.prim == NONE) break;
-        // The hit primitive's material alone decides whether the path ends
-        // here (a light, or any surface but a delta one at the first hit),
-        // so most camera rays stop before their shading point is built.
-        Mat m = load_mat(S, __float_as_uint(S.prims[(size_t)hit.prim * 4].y));
-        if (m.kind == LIGHT || (depth == 0u && !is_delta(m))) break;
-        Surface s = surface<TEXTURES>(S, o, d, hit);
-        V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
-        if (!is_delta(m)) {
-            V3 light = throughput * gather(C, heads, photons, s, m, base, -d);
-            if (finite3(light)) return make_float4(light.x, light.y, light.z, 1.0f);
-            break;
-        }
-        BsdfSample bs = sample_bsdf(g, m, base, s, -d);
-        if (!bs.ok) break;
-        throughput = throughput * bs.weight;
-        o = offset(s, bs.wi);
-        d = bs.wi;
-    }
-    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// The gathered light a pixel's point adds (w = 1), or zeros.
+RM_D float4 gathered_light(V3 throughput, V3 radiance) {
+    const V3 light = throughput * radiance;
+    return finite3(light) ? make_float4(light.x, light.y, light.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Every pixel's camera ray (the reference's gather stream) to its first hit,
+// and on (see above).
 template <bool TEXTURES, int TREE>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_photon_gather(DevScene S, Params P, const uint32_t* heads,
-                                                                 const float4* __restrict__ photons,
-                                                                 float4* __restrict__ gathered, uint32_t* spill,
-                                                                 uint32_t slots) {
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_begin(DevScene S, Params P, PhotonMap M, GatherQueue Q,
+                                                                float4* __restrict__ gathered, uint32_t* spill,
+                                                                uint32_t slots) {
     __shared__ uint2 lds[GATHER_LDS_DEPTH * BLOCK_THREADS];
     uint32_t x, y;
     tile_pixel(x, y);
-    if (x >= P.width || y >= P.height) return;
+    const bool inside = x < P.width && y < P.height;
     const Ctx C{S, &P};
-    const uint32_t index = y * P.width + x;
+    const uint32_t index = inside ? y * P.width + x : 0u;
     auto stack = BlockStack<GATHER_LDS_DEPTH, TREE>::make(lds, spill, index, slots);
-    gathered[index] = gather_pixel<TEXTURES>(C, heads, photons, x, y, index, stack);
+    bool go = false;
+    Rng g;
+    V3 o = {0, 0, 0}, d = {0, 0, 0}, throughput = {1, 1, 1};
+    GatherPoint q = {};
+    if (inside && P.max_bounces > 0u) {
+        g.qmc = false;
+        g.state = pcg(index ^ pcg(P.sample_offset * 2654435769u + 4099u));
+        float jx = g.rand();
+        float jy = g.rand();
+        Ray ray = camera_ray(P, g, {(float)x + jx, (float)y + jy}, true);
+        o = ray.o;
+        d = ray.d;
+        // A primary ray whose FP32 walk can't meet a delta surface ends at
+        // its first hit, whatever that is: no need to find it.
+        if (TREE != TREE_FP32 || S.delta_count == NO_DELTA_CUT || may_meet_delta(S, o, d))
+            go = gather_bounce<TEXTURES>(C, g, stack, 0u, o, d, throughput, q) == GatherStep::Bounce;
+    }
+    if ((uint32_t)__popcll(__ballot(go)) >= GATHER_DENSE_LANES) {
+        if (go && gather_path<TEXTURES>(C, g, stack, 1u, o, d, throughput, q))
+            gathered[index] = gathered_light(q.throughput, gather_lane(P, M, q, load_mat(S, q.material)));
+        else if (inside)
+            gathered[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const uint32_t at = wave_slot(Q.count, go);
+    if (go) {
+        float4* slot = Q.slots + (size_t)at * GATHER_SLOT_VEC4S;
+        slot[0] = make_float4(o.x, o.y, o.z, __uint_as_float(index));
+        slot[1] = make_float4(d.x, d.y, d.z, __uint_as_float(g.state));
+        slot[2] = make_float4(throughput.x, throughput.y, throughput.z, __uint_as_float(1u));
+    } else if (inside) {
+        gathered[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// The queued paths on to their gather points, a thread a path, each point
+// written over its path's slot.
+template <bool TEXTURES, int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_paths(DevScene S, Params P, GatherQueue Q,
+                                                                float4* __restrict__ gathered, uint32_t* spill,
+                                                                uint32_t slots) {
+    __shared__ uint2 lds[GATHER_LDS_DEPTH * BLOCK_THREADS];
+    const Ctx C{S, &P};
+    const uint32_t queued = *Q.count;
+    const uint32_t lane = blockIdx.x * BLOCK_THREADS + threadIdx.x;
+    auto stack = BlockStack<GATHER_LDS_DEPTH, TREE>::make(lds, spill, lane, slots);
+    for (uint32_t j = lane; j < queued; j += gridDim.x * BLOCK_THREADS) {
+        float4* slot = Q.slots + (size_t)j * GATHER_SLOT_VEC4S;
+        const float4 a = slot[0], b = slot[1], c = slot[2];
+        const uint32_t pixel = __float_as_uint(a.w);
+        Rng g;
+        g.qmc = false;
+        g.state = __float_as_uint(b.w);
+        GatherPoint q;
+        if (gather_path<TEXTURES>(C, g, stack, __float_as_uint(c.w), xyz(a), xyz(b), xyz(c), q)) {
+            slot[0] = make_float4(q.p.x, q.p.y, q.p.z, __uint_as_float(pixel));
+            slot[1] = make_float4(q.n.x, q.n.y, q.n.z, __uint_as_float(q.material));
+            slot[2] = make_float4(q.ng.x, q.ng.y, q.ng.z, 0.0f);
+            slot[3] = make_float4(q.base.x, q.base.y, q.base.z, 0.0f);
+            slot[4] = make_float4(q.wo.x, q.wo.y, q.wo.z, 0.0f);
+            slot[5] = make_float4(q.throughput.x, q.throughput.y, q.throughput.z, 0.0f);
+        } else {
+            slot[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(GATHER_DEAD));
+            gathered[pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
+// gather() for the queued points, eight lanes to a point: the point's eight
+// cells, one a lane, are runs of the sorted photons. The first 4096 photons
+// of the runs in the reference's order are laid end to end and the eight
+// lanes take every eighth, so they load side by side rather than one link at
+// a time. The lanes' partial sums are added in a fixed tree; the photons'
+// order within a cell follows their deposits, which the reference leaves to
+// the order its atomics land in.
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_walk(DevScene S, Params P, PhotonMap M, GatherQueue Q,
+                                                               float4* __restrict__ gathered) {
+    const uint32_t points = *Q.count;
+    const uint32_t sub = __lane_id() & 7u;
+    const uint32_t stride = gridDim.x * BLOCK_THREADS / 8u;
+    const float r = P.photon_radius, r2 = r * r;
+    for (uint32_t v = (blockIdx.x * BLOCK_THREADS + threadIdx.x) / 8u; __ballot(v < points); v += stride) {
+        bool live = false;
+        GatherPoint q = {};
+        uint32_t pixel = GATHER_DEAD;
+        if (v < points) {
+            const float4* slot = Q.slots + (size_t)v * GATHER_SLOT_VEC4S;
+            const float4 a = slot[0];
+            pixel = __float_as_uint(a.w);
+            live = pixel != GATHER_DEAD;
+            if (live) {
+                const float4 b = slot[1], c = slot[2], e = slot[3], f = slot[4], t = slot[5];
+                q = {xyz(a), xyz(b), xyz(c), xyz(e), xyz(f), xyz(t), __float_as_uint(b.w)};
+            }
+        }
+        // This lane's cell of the point, and where its run starts in the runs
+        // laid end to end (an inclusive scan over the group's eight lanes).
+        uint32_t count = 0u, last = 0u;
+        if (live) {
+            const uint32_t h = gather_cell(P, q.p, sub);
+            count = M.counts[h];
+            last = cell_last(M, h, count);
+        }
+        uint32_t end = count;
+#pragma unroll
+        for (uint32_t off = 1u; off < 8u; off <<= 1) {
+            const uint32_t below = __shfl_up(end, off, 8);
+            if (sub >= off) end += below;
+        }
+        uint32_t begins[8], lasts[8];
+#pragma unroll
+        for (uint32_t k = 0u; k < 8u; ++k) {
+            begins[k] = __shfl(end - count, (int)k, 8);
+            lasts[k] = __shfl(last, (int)k, 8);
+        }
+        // The runs laid end to end, cut at the first 4096 photons.
+        const uint32_t all = __shfl(end, 7, 8);
+        const uint32_t total = all < 4096u ? all : 4096u;
+        V3 sum = {0, 0, 0};
+        if (live) {
+            const Mat m = load_mat(S, q.material);
+            for (uint32_t j = sub; j < total; j += 8u) {
+                uint32_t k = 0u;
+#pragma unroll
+                for (uint32_t c = 1u; c < 8u; c++) k += j >= begins[c] ? 1u : 0u;
+                sum = sum + photon_term(M, lasts[k] - (j - begins[k]), q, m, r, r2);
+            }
+        }
+#pragma unroll
+        for (int off = 4; off > 0; off >>= 1) {
+            sum.x += __shfl_xor(sum.x, off, 8);
+            sum.y += __shfl_xor(sum.y, off, 8);
+            sum.z += __shfl_xor(sum.z, off, 8);
+        }
+        if (live && sub == 0u) gathered[pixel] = gathered_light(q.throughput, sum / (PI * r2));
+    }
 }
 
 // ------------------------------------------------------------------ launchers
 template <bool T, bool D>
 static void launch_light_trace_v(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
-                                 uint32_t* heads, float4* photons, uint32_t* spill, uint32_t slots, dim3 grid,
-                                 uint32_t count, uint32_t capacity, int tree, hipStream_t stream) {
+                                 const PhotonMap& M, uint32_t* spill, uint32_t slots, dim3 grid, uint32_t count,
+                                 int tree, hipStream_t stream) {
     with_tree(tree, [&](auto t) {
         hipLaunchKernelGGL((k_light_trace<T, D, decltype(t)::value>), grid, dim3(BLOCK_THREADS), 0, stream, S, P,
-                           emitters, splats, heads, photons, spill, slots, count, capacity);
+                           emitters, splats, M, spill, slots, count);
     });
 }
 
-void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats, uint32_t* heads,
-                        float4* photons, uint32_t* spill, uint32_t slots, uint32_t count, uint32_t capacity,
-                        const Features& f, hipStream_t stream) {
+void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
+                        const PhotonMap& M, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
+                        hipStream_t stream) {
     // One photon a thread up to LIGHT_LANES, then each thread takes several.
     const uint32_t lanes = count < LIGHT_LANES ? count : LIGHT_LANES;
     dim3 grid((lanes + BLOCK_THREADS - 1) / BLOCK_THREADS);
     if (grid.x == 0) return;
     if (f.textures)
-        launch_light_trace_v<true, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, count, capacity,
-                                         f.tree, stream);
+        launch_light_trace_v<true, true>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
     else if (f.dispersion)
-        launch_light_trace_v<false, true>(S, P, emitters, splats, heads, photons, spill, slots, grid, count,
-                                          capacity, f.tree, stream);
+        launch_light_trace_v<false, true>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
     else
-        launch_light_trace_v<false, false>(S, P, emitters, splats, heads, photons, spill, slots, grid, count,
-                                           capacity, f.tree, stream);
+        launch_light_trace_v<false, false>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
+}
+
+void launch_photon_sort(const PhotonMap& M, hipStream_t stream) {
+    hipLaunchKernelGGL(k_photon_scan, dim3(PHOTON_GROUPS), dim3(BLOCK_THREADS), 0, stream, M.counts, M.starts);
+    // The deposit count is known on the device only: a grid for the
+    // capacity, at most 1024 blocks, each thread taking every grid's-worth.
+    uint32_t blocks = (M.capacity + BLOCK_THREADS - 1) / BLOCK_THREADS;
+    blocks = blocks < 1u ? 1u : blocks < 1024u ? blocks : 1024u;
+    hipLaunchKernelGGL(k_photon_sort, dim3(blocks), dim3(BLOCK_THREADS), 0, stream, M);
 }
 
 void launch_resolve(const Params& P, uint4* splats, const float4* gathered, float4* accum, float4* comp,
@@ -385,17 +668,28 @@ void launch_fold(const Params& P, const float4* accum, const float4* galb, const
                        accum, galb, gnorm, comp, out);
 }
 
-void launch_photon_gather(const DevScene& S, const Params& P, const uint32_t* heads, const float4* photons,
+void launch_photon_gather(const DevScene& S, const Params& P, const PhotonMap& M, const GatherQueue& Q,
                           float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream) {
-    dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
-    auto go = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK_THREADS), 0, stream, S, P, heads, photons, gathered, spill, slots);
+    const uint32_t px = P.width * P.height;
+    const dim3 tiles((P.width + 15) / 16, (P.height + 15) / 16);
+    // The queue's length is known on the device only: grids for at most
+    // every pixel, each thread (or group of eight) taking every grid's-worth.
+    uint32_t path_blocks = (px + BLOCK_THREADS - 1) / BLOCK_THREADS;
+    path_blocks = path_blocks < 2048u ? path_blocks : 2048u;
+    uint32_t walk_blocks = (px + BLOCK_THREADS / 8 - 1) / (BLOCK_THREADS / 8);
+    walk_blocks = walk_blocks < 2048u ? walk_blocks : 2048u;
+    launch_fill_u32(Q.count, 0u, 1, stream);
+    auto go = [&](auto begin, auto paths) {
+        hipLaunchKernelGGL(begin, tiles, dim3(BLOCK_THREADS), 0, stream, S, P, M, Q, gathered, spill, slots);
+        hipLaunchKernelGGL(paths, dim3(path_blocks), dim3(BLOCK_THREADS), 0, stream, S, P, Q, gathered, spill,
+                           slots);
     };
     with_tree(f.tree, [&](auto t) {
         constexpr int TREE = decltype(t)::value;
-        if (f.textures) go(k_photon_gather<true, TREE>);
-        else go(k_photon_gather<false, TREE>);
+        if (f.textures) go(k_gather_begin<true, TREE>, k_gather_paths<true, TREE>);
+        else go(k_gather_begin<false, TREE>, k_gather_paths<false, TREE>);
     });
+    hipLaunchKernelGGL(k_gather_walk, dim3(walk_blocks), dim3(BLOCK_THREADS), 0, stream, S, P, M, Q, gathered);
 }
 
 }  // namespace rm

@@ -12,12 +12,26 @@ namespace rm {
 // ------------------------------------------------------------------ thread mapping
 // A 256-thread block covers a 16×16 pixel tile as four 8×8 wave tiles, so a
 // wave's primary rays are spatially coherent.
-RM_D void tile_pixel_of(uint32_t t, uint32_t& x, uint32_t& y) {
+RM_D void tile_pixel_in(uint32_t t, uint32_t tile_x, uint32_t tile_y, uint32_t& x, uint32_t& y) {
     uint32_t w = t >> 6, l = t & 63u;
-    x = blockIdx.x * 16u + (w & 1u) * 8u + (l & 7u);
-    y = blockIdx.y * 16u + (w >> 1) * 8u + (l >> 3);
+    x = tile_x * 16u + (w & 1u) * 8u + (l & 7u);
+    y = tile_y * 16u + (w >> 1) * 8u + (l >> 3);
 }
+RM_D void tile_pixel_of(uint32_t t, uint32_t& x, uint32_t& y) { tile_pixel_in(t, blockIdx.x, blockIdx.y, x, y); }
 RM_D void tile_pixel(uint32_t& x, uint32_t& y) { tile_pixel_of(threadIdx.x, x, y); }
+
+// A slot for each lane with pred, from `count`, one atomic per wave; lanes
+// without pred get NONE. Every lane of the wave calls it.
+RM_D uint32_t wave_slot(uint32_t* count, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0ull) return NONE;
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : NONE;
+}
 
 // Block shapes of the per-pixel pass (RM_OPTION_BLOCK_SHAPE): a 16×16 tile
 // of four 8×8 wave squares (BLOCK_SQUARES, the default: a wave's primary

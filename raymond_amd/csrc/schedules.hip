@@ -358,19 +358,6 @@ constexpr int DONE_WORDS = 11;   // result, guide albedo, guide normal, pixel
 static_assert(WF_STATE_WORDS == SLOT_WORDS && WF_NEE_WORDS == NEE_WORDS + 1 && WF_DONE_WORDS == DONE_WORDS,
               "the host sizes these buffers");
 
-// A slot for each lane with pred, from `count`, one atomic per wave; lanes
-// without pred get NONE.
-RM_D uint32_t wave_slot(uint32_t* count, bool pred) {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0ull) return NONE;
-    const uint32_t lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : NONE;
-}
-
 RM_D SlotStore wf_state(const Wavefront& W, uint32_t which) { return SlotStore{W.state[which], W.n}; }
 
 RM_D PixelSums wf_sums(const Wavefront& W, uint32_t i) {

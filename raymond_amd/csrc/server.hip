@@ -99,8 +99,9 @@ uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options
         const uint64_t photons = budget > rm::PHOTON_CAPACITY ? budget : rm::PHOTON_CAPACITY;
         const uint64_t lanes = photons < rm::LIGHT_LANES ? photons : rm::LIGHT_LANES;
         b += px * 2 * 16;   // the two gathered-light buffers
+        b += px * rm::GATHER_SLOT_VEC4S * 16 + 256;   // the gather queue
         b += (px > lanes ? px : lanes) * rm::SPILL_ENTRIES * 8;
-        b += (uint64_t)(rm::PHOTON_TABLE + 1) * 4 + photons * 48;
+        b += (uint64_t)(rm::PHOTON_TABLE + 1) * 4 + rm::photon_map_bytes(photons);
     }
     return b;
 }

@@ -77,7 +77,8 @@ ORACLE_NAMES = {"aces": ["aces1"], "to_srgb": ["srgb1"], "atrous": ["oracle_atro
                 "qmc_next": ["next"]}
 KERNEL_NAMES = {"aces": ["aces1"], "to_srgb": ["srgb1"], "atrous": ["atrous_pixel"], "prepare": ["k_prepare"],
                 "compose": ["k_compose"], "main": ["main_pixel", "path_begin"], "light_trace": ["k_light_trace"],
-                "photon_gather": ["gather_pixel"], "resolve_splats": ["resolve_add"], "qmc_next": ["next"],
+                "photon_gather": ["k_gather_begin", "gather_bounce"], "gather": ["gather_lane", "gather_cell", "photon_term"],
+                "resolve_splats": ["resolve_add"], "qmc_next": ["next"],
                 "finite": ["finite3"], "radiance": ["path_step", "path_shade", "path_begin"],
                 "deposit": ["k_light_trace"], "dispersed": ["path_shade", "dispersed_ior"],
                 "env_choice": ["light_choices", "env_choice"], "sun_choice": ["light_choices", "sun_choice"],
@@ -99,6 +100,9 @@ EXCEPTIONS = {
     "sobol2": {0, 2147483648},
     # The kernels read a pixel's splats as one uint4 rather than words index·4 + k.
     "resolve_splats": {4, 1, 2},
+    # The reference's photons are 3 vec4s each (index · 3); the kernels' sorted
+    # photon map keeps positions and (power, direction) pairs in arrays of their own.
+    "gather": {3},
     # The stack bound 64 is STACK_LIMIT; the restatements start from named values.
     "trace": {0.0, 0, 1.0, 1, 64},
 }
