@@ -11,6 +11,17 @@ RM_D uint32_t photon_cell_hash(int32_t cx, int32_t cy, int32_t cz) {
     return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & (PHOTON_TABLE - 1u);
 }
 
+// Whether the ray's slab test passes a box of the scene's delta cut: the
+// same reciprocal direction and slab test as the FP32 walk's, so a ray that
+// passes none is one whose walk tests no delta primitive.
+RM_D bool may_meet_delta(const DevScene& S, V3 o, V3 d) {
+    const V3 sd = safe(d);
+    const V3 inv = {1.0f / sd.x, 1.0f / sd.y, 1.0f / sd.z};
+    for (uint32_t i = 0u; i < S.delta_count; ++i)
+        if (slabs(S.delta_boxes[2 * i], S.delta_boxes[2 * i + 1], o, inv, INF) < INF) return true;
+    return false;
+}
+
 // Adds a photon's light to the pixel that sees surface s (src/shaders.mjs:1444-1491).
 template <class STACK>
 RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surface& s, const Mat& m, V3 base, V3 d,
@@ -181,6 +192,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
                 o = o + n * (1e-4f * fmaxf(1.0f, fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)))));
             }
 
+            // A light path deposits and splats only past a delta bounce: one
+            // whose first ray can't meet a delta surface in the FP32 walk
+            // ends at its first hit with nothing to show, so it isn't traced.
+            if (TREE == TREE_FP32 && alive && S.delta_count != NO_DELTA_CUT && !may_meet_delta(S, o, d))
+                alive = false;
             float wavelength = 0.0f;
             uint32_t bounces = 0u;
             V3 carried = {1.0f, 1.0f, 1.0f};
@@ -351,17 +367,6 @@ constexpr uint32_t GATHER_DEAD = 0xffffffffu;
 // Lanes of a wave whose paths go on past their first hit, from which the
 // wave carries them on itself rather than queueing them.
 constexpr uint32_t GATHER_DENSE_LANES = 48u;
-
-// Whether the ray's slab test passes a box of the scene's delta cut: the
-// same reciprocal direction and slab test as the FP32 walk's, so a ray that
-// passes none is one whose walk tests no delta primitive.
-RM_D bool may_meet_delta(const DevScene& S, V3 o, V3 d) {
-    const V3 sd = safe(d);
-    const V3 inv = {1.0f / sd.x, 1.0f / sd.y, 1.0f / sd.z};
-    for (uint32_t i = 0u; i < S.delta_count; ++i)
-        if (slabs(S.delta_boxes[2 * i], S.delta_boxes[2 * i + 1], o, inv, INF) < INF) return true;
-    return false;
-}
 
 // Where a camera path gathers, with what gather() reads of it.
 struct GatherPoint { V3 p, n, ng, base, wo, throughput; uint32_t material; };
