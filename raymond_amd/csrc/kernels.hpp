@@ -194,9 +194,10 @@ void launch_resolve(const Params& P, uint4* splats, const float4* gathered, floa
 // sum + error of the compensated sums, into out: planes accum, galb, gnorm of W·H float4.
 void launch_fold(const Params& P, const float4* accum, const float4* galb, const float4* gnorm, const float4* comp,
                  float4* out, hipStream_t stream);
-// The photon gather's queue of camera paths, each replaced in its slot by
-// its gather point: a slot of GATHER_SLOT_VEC4S float4 for each of px
-// pixels, and the number queued.
+// The photon gather's queue of gather points (from the front) and camera
+// paths (from the back, each replaced in its slot by its gather point): a
+// slot of GATHER_SLOT_VEC4S float4 for each of px pixels; count[0] points
+// and count[1] paths queued.
 constexpr uint32_t GATHER_SLOT_VEC4S = 6;
 struct GatherQueue {
     float4* slots;

@@ -351,22 +351,29 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_fold(Params P, const float4* 
 //   k_gather_begin  every pixel's camera ray to its first hit. A path whose
 //                   first hit is a delta surface bounces there and goes on.
 //                   A wave where many paths go on carries each to its
-//                   gather point (its first non-delta surface) and gathers
-//                   there, a lane a point, as the reference does. A wave
-//                   where few go on queues them instead, so that they don't
-//                   hold the wave through their bounces and walks while its
-//                   other lanes idle;
+//                   gather point (its first non-delta surface), and when
+//                   every lane's photons are few, gathers there, a lane a
+//                   point, as the reference does; when some lane has many,
+//                   its points are queued. A wave where few paths go on
+//                   queues them instead, so that they don't hold the wave
+//                   through their bounces and walks while its other lanes
+//                   idle;
 //   k_gather_paths  each queued path on to its gather point, a thread a path,
 //                   the point kept in the path's slot;
 //   k_gather_walk   the queued points' photons, eight lanes to a point.
-// A queue slot is 6 float4. A path: (o, pixel), (d, rng state), (throughput,
-// next depth). A gather point: (p, pixel), (n, material), (ng, 0), (base, 0),
-// (wo, 0), (throughput, 0); a path that ends without one leaves GATHER_DEAD
-// as its pixel. Every pixel's gathered[] is written once.
+// The queue has a 6-float4 slot a pixel: points from the front (count[0] of
+// them) and paths from the back (count[1], slot W·H - 1 - j for the j-th).
+// A path: (o, pixel), (d, rng state), (throughput, next depth). A gather
+// point: (p, pixel), (n, material), (ng, 0), (base, 0), (wo, 0), (throughput,
+// 0); a path that ends without one leaves GATHER_DEAD as its pixel. Every
+// pixel's gathered[] is written once.
 constexpr uint32_t GATHER_DEAD = 0xffffffffu;
 // Lanes of a wave whose paths go on past their first hit, from which the
 // wave carries them on itself rather than queueing them.
 constexpr uint32_t GATHER_DENSE_LANES = 48u;
+// The most photons any lane of such a wave may have to visit for the wave
+// to gather a lane a point; past it, the eight-lane walk does.
+constexpr uint32_t GATHER_LANE_PHOTONS = 256u;
 
 // Where a camera path gathers, with what gather() reads of it.
 struct GatherPoint { V3 p, n, ng, base, wo, throughput; uint32_t material; };
@@ -447,21 +454,35 @@ RM_D V3 photon_term(const PhotonMap& M, uint32_t at, const GatherPoint& q, const
     return {0.0f, 0.0f, 0.0f};
 }
 
+// The eight cells of a gather point: each run's newest sorted slot and
+// length, and how many photons the reference visits in all (at most 4096).
+struct PointCells { uint32_t last[8], count[8], total; };
+RM_D PointCells point_cells(const Params& P, const PhotonMap& M, V3 p) {
+    PointCells c;
+    uint32_t all = 0u;
+#pragma unroll
+    for (uint32_t k = 0u; k < 8u; k++) {
+        const uint32_t h = gather_cell(P, p, k);
+        c.count[k] = M.counts[h];
+        c.last[k] = cell_last(M, h, c.count[k]);
+        all += c.count[k];
+    }
+    c.total = all < 4096u ? all : 4096u;
+    return c;
+}
+
 // gather() (src/shaders.mjs:1534-1570) by one lane, in the reference's order:
 // the eight cells in turn, each newest deposit first (the reference's lists
 // are pushed at the head), at most 4096 photons in all. The run's slots are
 // known up front, so unrolled iterations load their photons together.
-RM_D V3 gather_lane(const Params& P, const PhotonMap& M, const GatherPoint& q, const Mat& m) {
+RM_D V3 gather_lane(const Params& P, const PhotonMap& M, const GatherPoint& q, const PointCells& c, const Mat& m) {
     const float r = P.photon_radius, r2 = r * r;
     V3 sum = {0, 0, 0};
     uint32_t visited = 0u;
     for (uint32_t k = 0u; k < 8u && visited < 4096u; k++) {
-        const uint32_t h = gather_cell(P, q.p, k);
-        const uint32_t count = M.counts[h];
-        const uint32_t last = cell_last(M, h, count);
-        const uint32_t take = count < 4096u - visited ? count : 4096u - visited;
+        const uint32_t take = c.count[k] < 4096u - visited ? c.count[k] : 4096u - visited;
 #pragma unroll 4
-        for (uint32_t i = 0u; i < take; i++) sum = sum + photon_term(M, last - i, q, m, r, r2);
+        for (uint32_t i = 0u; i < take; i++) sum = sum + photon_term(M, c.last[k] - i, q, m, r, r2);
         visited += take;
     }
     return sum / (PI * r2);
@@ -504,15 +525,33 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
             go = gather_bounce<TEXTURES>(C, g, stack, 0u, o, d, throughput, q) == GatherStep::Bounce;
     }
     if ((uint32_t)__popcll(__ballot(go)) >= GATHER_DENSE_LANES) {
-        if (go && gather_path<TEXTURES>(C, g, stack, 1u, o, d, throughput, q))
-            gathered[index] = gathered_light(q.throughput, gather_lane(P, M, q, load_mat(S, q.material)));
-        else if (inside)
+        const bool found = go && gather_path<TEXTURES>(C, g, stack, 1u, o, d, throughput, q);
+        PointCells cells;
+        cells.total = 0u;
+        if (found) cells = point_cells(P, M, q.p);
+        // A lane a point while every lane's walk is short; else to the queue.
+        if (__ballot(cells.total > GATHER_LANE_PHOTONS) == 0ull) {
+            if (found) gathered[index] = gathered_light(q.throughput, gather_lane(P, M, q, cells, load_mat(S, q.material)));
+            else if (inside) gathered[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return;
+        }
+        const uint32_t at = wave_slot(&Q.count[0], found);
+        if (found) {
+            float4* slot = Q.slots + (size_t)at * GATHER_SLOT_VEC4S;
+            slot[0] = make_float4(q.p.x, q.p.y, q.p.z, __uint_as_float(index));
+            slot[1] = make_float4(q.n.x, q.n.y, q.n.z, __uint_as_float(q.material));
+            slot[2] = make_float4(q.ng.x, q.ng.y, q.ng.z, 0.0f);
+            slot[3] = make_float4(q.base.x, q.base.y, q.base.z, 0.0f);
+            slot[4] = make_float4(q.wo.x, q.wo.y, q.wo.z, 0.0f);
+            slot[5] = make_float4(q.throughput.x, q.throughput.y, q.throughput.z, 0.0f);
+        } else if (inside) {
             gathered[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         return;
     }
-    const uint32_t at = wave_slot(Q.count, go);
+    const uint32_t at = wave_slot(&Q.count[1], go);
     if (go) {
-        float4* slot = Q.slots + (size_t)at * GATHER_SLOT_VEC4S;
+        float4* slot = Q.slots + (size_t)(Q.px - 1u - at) * GATHER_SLOT_VEC4S;
         slot[0] = make_float4(o.x, o.y, o.z, __uint_as_float(index));
         slot[1] = make_float4(d.x, d.y, d.z, __uint_as_float(g.state));
         slot[2] = make_float4(throughput.x, throughput.y, throughput.z, __uint_as_float(1u));
@@ -529,11 +568,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
                                                                 uint32_t slots) {
     __shared__ uint2 lds[GATHER_LDS_DEPTH * BLOCK_THREADS];
     const Ctx C{S, &P};
-    const uint32_t queued = *Q.count;
+    const uint32_t queued = Q.count[1];
     const uint32_t lane = blockIdx.x * BLOCK_THREADS + threadIdx.x;
     auto stack = BlockStack<GATHER_LDS_DEPTH, TREE>::make(lds, spill, lane, slots);
     for (uint32_t j = lane; j < queued; j += gridDim.x * BLOCK_THREADS) {
-        float4* slot = Q.slots + (size_t)j * GATHER_SLOT_VEC4S;
+        float4* slot = Q.slots + (size_t)(Q.px - 1u - j) * GATHER_SLOT_VEC4S;
         const float4 a = slot[0], b = slot[1], c = slot[2];
         const uint32_t pixel = __float_as_uint(a.w);
         Rng g;
@@ -563,7 +602,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
 // the order its atomics land in.
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_walk(DevScene S, Params P, PhotonMap M, GatherQueue Q,
                                                                float4* __restrict__ gathered) {
-    const uint32_t points = *Q.count;
+    const uint32_t front = Q.count[0], points = front + Q.count[1];
     const uint32_t sub = __lane_id() & 7u;
     const uint32_t stride = gridDim.x * BLOCK_THREADS / 8u;
     const float r = P.photon_radius, r2 = r * r;
@@ -572,7 +611,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
         GatherPoint q = {};
         uint32_t pixel = GATHER_DEAD;
         if (v < points) {
-            const float4* slot = Q.slots + (size_t)v * GATHER_SLOT_VEC4S;
+            const float4* slot = Q.slots + (size_t)(v < front ? v : Q.px - 1u - (v - front)) * GATHER_SLOT_VEC4S;
             const float4 a = slot[0];
             pixel = __float_as_uint(a.w);
             live = pixel != GATHER_DEAD;
@@ -683,7 +722,7 @@ void launch_photon_gather(const DevScene& S, const Params& P, const PhotonMap& M
     path_blocks = path_blocks < 2048u ? path_blocks : 2048u;
     uint32_t walk_blocks = (px + BLOCK_THREADS / 8 - 1) / (BLOCK_THREADS / 8);
     walk_blocks = walk_blocks < 2048u ? walk_blocks : 2048u;
-    launch_fill_u32(Q.count, 0u, 1, stream);
+    launch_fill_u32(Q.count, 0u, 2, stream);
     auto go = [&](auto begin, auto paths) {
         hipLaunchKernelGGL(begin, tiles, dim3(BLOCK_THREADS), 0, stream, S, P, M, Q, gathered, spill, slots);
         hipLaunchKernelGGL(paths, dim3(path_blocks), dim3(BLOCK_THREADS), 0, stream, S, P, Q, gathered, spill,
