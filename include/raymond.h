@@ -87,9 +87,10 @@ extern "C" {
 #define RM_SCHEDULE_PERSISTENT 2       /* persistent lanes that regenerate finished paths from a tile queue */
 #define RM_SCHEDULE_WAVEFRONT 3        /* one kernel per stage (extend, shade, shadow, finish) over queues of
                                           live paths, the paths' state in device memory between stages */
-#define RM_SCHEDULE_SORTED 4           /* as RM_SCHEDULE_COMPACT, and between bounces a block's live paths are
-                                          sorted by ray-direction octant and specular-or-not, so waves trace and
-                                          shade alike paths */
+#define RM_SCHEDULE_SORTED 4           /* as RM_SCHEDULE_COMPACT, and twice a bounce a block's live paths are
+                                          sorted: by the kind of material hit after tracing, by ray-direction
+                                          octant and specular-or-not after shading, so waves shade and trace
+                                          alike paths */
 
 typedef struct rm_context rm_context;
 typedef struct rm_scene rm_scene;
