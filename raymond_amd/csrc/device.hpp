@@ -332,13 +332,29 @@ struct Stack {
     uint2* lds;      // &shared[0][threadIdx.x]
     uint2* spill;    // this thread's column of an [entry][slot] spill array
     uint32_t stride; // slots in the spill array
+    // The LDS entries go through an LDS-space pointer and the spilled ones
+    // through a volatile one: otherwise the compiler merges the two branches'
+    // stores (and loads) into one flat access, which waits on both the
+    // vector-memory and the LDS counters, so that every traversal step waited
+    // for all loads in flight.
+    using Local = __attribute__((address_space(3))) uint64_t;
+    RM_D Local* local() const { return (Local*)lds; }
     RM_D void put(uint32_t sp, uint2 v) {
-        if (LDS_DEPTH >= (int)STACK_LIMIT || sp < (uint32_t)LDS_DEPTH) lds[sp * BLOCK] = v;
-        else spill[(size_t)(sp - LDS_DEPTH) * stride] = v;
+        if (LDS_DEPTH >= (int)STACK_LIMIT || sp < (uint32_t)LDS_DEPTH) {
+            local()[sp * BLOCK] = (uint64_t)v.y << 32 | v.x;
+        } else {
+            volatile uint2* at = spill + (size_t)(sp - LDS_DEPTH) * stride;
+            at->x = v.x;
+            at->y = v.y;
+        }
     }
     RM_D uint2 get(uint32_t sp) const {
-        if (LDS_DEPTH >= (int)STACK_LIMIT || sp < (uint32_t)LDS_DEPTH) return lds[sp * BLOCK];
-        return spill[(size_t)(sp - LDS_DEPTH) * stride];
+        if (LDS_DEPTH >= (int)STACK_LIMIT || sp < (uint32_t)LDS_DEPTH) {
+            const uint64_t e = local()[sp * BLOCK];
+            return make_uint2((uint32_t)e, (uint32_t)(e >> 32));
+        }
+        const volatile uint2* at = spill + (size_t)(sp - LDS_DEPTH) * stride;
+        return make_uint2(at->x, at->y);
     }
 };
 
