@@ -305,8 +305,10 @@ inline AtrousGrid atrous_grid(uint32_t width, uint32_t height, uint32_t step) {
     return g;
 }
 
+// PG is the step itself (1, 2 or 4): a workgroup takes all s x-phases of one
+// y-phase, so the step is a compile-time constant of each instance.
 template <class G, int PG>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_staged(Params P, int step, AtrousGrid grid,
+__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_staged(Params P, AtrousGrid grid,
                                                                  const typename G::T* __restrict__ guides,
                                                                  const float4* __restrict__ light_in,
                                                                  float4* __restrict__ light_out) {
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_staged(Params P, int s
     __shared__ float4 tile_guide[N];
     const uint32_t logical = (blockIdx.x % XCDS) * (grid.launched / XCDS) + blockIdx.x / XCDS;
     if (logical >= grid.blocks) return;
-    const int s = step;
+    constexpr int s = PG;
     const uint32_t group = logical % grid.groups, t = logical / grid.groups;
     const int xgroups = s / PG;
     const int gx = (int)(group % (uint32_t)xgroups) * PG, py = (int)(group / (uint32_t)xgroups);
@@ -342,10 +344,12 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_staged(Params P, int s
         q = tile_light[c + PG * (dx + SU * dy)];
         g = tile_guide[c + PG * (dx + SU * dy)];
     };
-    // The variance neighbourhood is in the tile only at step 1.
+    // The variance neighbourhood is in the tile only at step 1 (decided at
+    // compile time: a run-time choice between the tile and the buffer became
+    // one flat load, which waits on both memory counters).
     auto var = [&](int dx, int dy) {
-        return s == 1 ? tile_light[c + dx + SU * dy].w
-                      : light_in[(uint32_t)(y + dy) * P.width + (uint32_t)(x + dx)].w;
+        if constexpr (s == 1) return tile_light[c + dx + SU * dy].w;
+        else return light_in[(uint32_t)(y + dy) * P.width + (uint32_t)(x + dx)].w;
     };
     light_out[(uint32_t)y * P.width + (uint32_t)x] = atrous_pixel(x, y, W, H, s, tile_light[c], tile_guide[c], tap, var);
 }
@@ -471,32 +475,29 @@ void launch_prepare(const Params& P, const float4* accum, const float4* galb, co
 }
 
 template <int PG>
-static void launch_atrous_pg(const Params& P, int step, const void* guides, bool fp16, const float4* in,
-                                 float4* out, hipStream_t stream) {
-    const AtrousGrid g = atrous_grid<PG>(P.width, P.height, (uint32_t)step);
+static void launch_atrous_pg(const Params& P, const void* guides, bool fp16, const float4* in, float4* out,
+                             hipStream_t stream) {
+    const AtrousGrid g = atrous_grid<PG>(P.width, P.height, (uint32_t)PG);
     if (fp16)
-        hipLaunchKernelGGL((k_atrous_staged<GuidesF16, PG>), dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, step,
-                           g, static_cast<const ushort4*>(guides), in, out);
+        hipLaunchKernelGGL((k_atrous_staged<GuidesF16, PG>), dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, g,
+                           static_cast<const ushort4*>(guides), in, out);
     else
-        hipLaunchKernelGGL((k_atrous_staged<GuidesF32, PG>), dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, step,
-                           g, static_cast<const float4*>(guides), in, out);
+        hipLaunchKernelGGL((k_atrous_staged<GuidesF32, PG>), dim3(g.launched), dim3(BLOCK_THREADS), 0, stream, P, g,
+                           static_cast<const float4*>(guides), in, out);
 }
 
 void launch_atrous(const Params& P, int step, const void* guides, bool fp16, const float4* in, float4* out,
                    hipStream_t stream) {
-    if (step > 4) {
-        dim3 grid(stencil_blocks(P.width, P.height));
-        if (fp16)
-            hipLaunchKernelGGL(k_atrous<GuidesF16>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
-                               static_cast<const ushort4*>(guides), in, out);
-        else
-            hipLaunchKernelGGL(k_atrous<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
-                               static_cast<const float4*>(guides), in, out);
-        return;
-    }
-    if (step % 4 == 0) return launch_atrous_pg<4>(P, step, guides, fp16, in, out, stream);
-    if (step % 2 == 0) return launch_atrous_pg<2>(P, step, guides, fp16, in, out, stream);
-    launch_atrous_pg<1>(P, step, guides, fp16, in, out, stream);
+    if (step == 4) return launch_atrous_pg<4>(P, guides, fp16, in, out, stream);
+    if (step == 2) return launch_atrous_pg<2>(P, guides, fp16, in, out, stream);
+    if (step == 1) return launch_atrous_pg<1>(P, guides, fp16, in, out, stream);
+    dim3 grid(stencil_blocks(P.width, P.height));
+    if (fp16)
+        hipLaunchKernelGGL(k_atrous<GuidesF16>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
+                           static_cast<const ushort4*>(guides), in, out);
+    else
+        hipLaunchKernelGGL(k_atrous<GuidesF32>, grid, dim3(BLOCK_THREADS), 0, stream, P, step,
+                           static_cast<const float4*>(guides), in, out);
 }
 
 void launch_display(const Params& P, const float4* accum, const float4* denoised, uchar4* out, hipStream_t stream) {
