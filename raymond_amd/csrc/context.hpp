@@ -130,6 +130,10 @@ struct rm_context {
     // that last read its pair (resolved_ev), not for the path pass before it.
     rmi::DevBuf splats[2], gathered[2];
     hipEvent_t resolved_ev[2] = {nullptr, nullptr};
+    // The denoiser's end for the last frame that used each pair (recorded
+    // when that frame denoised): a frame's light tracing also waits for it.
+    hipEvent_t denoised_pair_ev[2] = {nullptr, nullptr};
+    bool denoised_pair_live[2] = {false, false};
     uint32_t pair = 0;   // the pair the next frame with light tracing uses
     // The photon map (rm::PhotonMap): cell counts, cell starts, and the
     // photons' deposits with their sorted copy (96 bytes a photon slot).
