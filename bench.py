@@ -60,7 +60,9 @@ WORKLOADS = {
         "opts": {"scene": {"seed": 2, "count": 200, "width": 1920, "height": 1080}, "maxBounces": 8,
                  "samplesPerFrame": 1, "fireflyClamp": 0.0, "denoise": True, "caustics": True},
         "data": "synthetic: seeded 200-sphere scene generated and packed by the JS host modules",
-        "traffic": ROOT / "profiles" / "k_main_spheres200_1080p_pmc.md",
+        # The path pass's counter summaries, by schedule.
+        "traffic": {"split": ROOT / "profiles" / "k_split_spheres200_1080p_pmc.md",
+                    "per-pixel": ROOT / "profiles" / "k_main_spheres200_1080p_pmc.md"},
     },
     "triangles2m-1920x1080": {
         "config": "triangles", "bounces": 12, "stride": 509,
@@ -246,6 +248,7 @@ def bench_frames(args, name: str) -> dict:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULES[args.schedule])
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, args.persistent_blocks)
         ctx.set_option(capi.OPTION_WAVES, args.waves)
+        ctx.set_option(capi.OPTION_SPLIT_DEPTH, args.split_depth)
     ctx.set_option(capi.OPTION_PRECISION, capi.PRECISION_QUANTIZED_BVH if args.quantized_bvh else 0)
     ctx.set_option(capi.OPTION_LIGHT_PATHS, wl.get("light_paths", 0))
     scene = ctx.upload_scene(p)
@@ -276,6 +279,12 @@ def bench_frames(args, name: str) -> dict:
 
     samples = w * h * spp * args.steps
     value = samples / elapsed / 1e6
+    schedule = tuning["best"][0] if tuned else args.schedule
+    # The committed counter summaries were taken at four waves, split depth 2, on the FP32 tree.
+    waves = tuning["best"][2] if tuned else args.waves
+    split_depth = (tuning["best"][5] if tuned else args.split_depth) or 2
+    as_profiled = waves == 4 and split_depth == 2 and not args.quantized_bvh
+    summary = (wl["traffic"] or {}).get(schedule) if as_profiled else None
     trace_ms, trace_n = stages["trace"]
     trace_avg_s = trace_ms / max(trace_n, 1) / 1e3
 
@@ -301,14 +310,15 @@ def bench_frames(args, name: str) -> dict:
                    "primitives": p.primitive_count, "device": device, "compute_units": cus,
                    "light_paths_per_frame": wl.get("light_paths") or min(w * h * spp, 524288) if caustics else 0,
                    "photon_radius": wl.get("photon_radius", "shrinking as frame^(-1/6) from a 3-pixel footprint"),
-                   "path": "one job, megakernel, " + ("8-bit quantized BVH" if args.quantized_bvh else "FP32 BVH"),
-                   "schedule": args.schedule, "waves_per_simd": tuning["best"][2] if tuned else args.waves,
+                   "path": "one job, " + ("8-bit quantized BVH" if args.quantized_bvh else "FP32 BVH"),
+                   "schedule": args.schedule, "waves_per_simd": waves,
+                   "split_depth": split_depth if schedule == "split" else None,
                    "tuning": tuning},
-        "roofline": dict(roofline(trace_avg_s, bytes_per_launch, wl["traffic"]),
+        "roofline": dict(roofline(trace_avg_s, bytes_per_launch, summary),
                          kernel={"compact": "k_main_compact", "sorted": "k_main_compact (sorting)",
-                                 "per-pixel": "k_main",
-                                 "persistent": "k_main_persistent", "wavefront": "the wavefront stages (k_wf_*)"}[
-                             tuning["best"][0] if tuned else args.schedule],
+                                 "per-pixel": "k_main", "persistent": "k_main_persistent",
+                                 "wavefront": "the wavefront stages (k_wf_*)",
+                                 "split": "k_split_head + k_split_tail"}[schedule],
                          launch_ms=round(trace_avg_s * 1e3, 4),
                          bytes_per_sample=round(alg["per_sample"], 1),
                          rays_per_sample=round(alg["rays_per_sample"], 3)),
@@ -612,10 +622,14 @@ def main() -> int:
     ap.add_argument("--served-precision", default="quantized-bvh,fp16-guides,compensated",
                     help="serving: precision variants of the batched variant run, comma-separated "
                          "(quantized-bvh, fp16-guides, compensated)")
-    ap.add_argument("--schedule", choices=["per-pixel", "compact", "sorted", "persistent", "wavefront", "tuned"],
-                    default="per-pixel",
+    ap.add_argument("--schedule", choices=["per-pixel", "compact", "sorted", "persistent", "wavefront", "split",
+                                           "tuned"],
+                    default="split",
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
-                         "or lanes regenerating paths from a tile queue")
+                         "lanes regenerating paths from a tile queue, or a kernel a pixel for the first bounces "
+                         "and persistent waves over the paths still alive")
+    ap.add_argument("--split-depth", type=int, default=0,
+                    help="bounces the split schedule's first kernel runs (RM_OPTION_SPLIT_DEPTH; 0: 2)")
     ap.add_argument("--waves", type=int, choices=[4, 5], default=4,
                     help="waves per SIMD of the one-thread-per-pixel path pass (RM_OPTION_WAVES)")
     ap.add_argument("--quantized-bvh", action="store_true",

@@ -57,8 +57,9 @@ extern "C" {
                                           and waves for each new scene and image size; 0 (default): no tuning */
 #define RM_OPTION_BATCH 4              /* job servers: 1 gives the frames of the jobs running at once one
                                           path-pass launch (rm_frame_batch); 0 (default): a launch per job */
-#define RM_OPTION_WAVES 5              /* waves per SIMD that RM_SCHEDULE_PIXEL's path pass is compiled for:
-                                          4 (or 0, the default) or 5; the same results bit for bit */
+#define RM_OPTION_WAVES 5              /* waves per SIMD that the RM_SCHEDULE_SPLIT and RM_SCHEDULE_PIXEL path
+                                          passes are compiled for: 4 (or 0, the default) or 5; the same results
+                                          bit for bit */
 #define RM_OPTION_LIGHT_PATHS 6        /* light paths (and photon slots) a frame with light tracing; 0 (default):
                                           the reference's min(W·H·spp, 524288). A budget past that cap changes
                                           the image's light-traced part (more photons, less noise) */
@@ -67,7 +68,9 @@ extern "C" {
 #define RM_OPTION_BLOCK_SHAPE 8        /* RM_SCHEDULE_PIXEL at four waves: RM_BLOCK_SQUARES (0, the default;
                                           16×16 tiles of 8×8 wave squares) or RM_BLOCK_ROWS (64×4 tiles, a wave
                                           a 64-pixel row); the same results bit for bit */
-#define RM_OPTION_COUNT 9
+#define RM_OPTION_SPLIT_DEPTH 9        /* RM_SCHEDULE_SPLIT: bounces the head kernel runs before it queues the
+                                          paths still alive, 1 to 8 (0, the default: 2) */
+#define RM_OPTION_COUNT 10
 
 #define RM_BLOCK_SQUARES 0
 #define RM_BLOCK_ROWS 1
@@ -82,7 +85,7 @@ extern "C" {
                                           compensated FP32: each add's rounding error is kept and folded
                                           back for the denoiser and for rm_read / rm_display */
 
-#define RM_SCHEDULE_PIXEL 0            /* default: one thread per pixel, each path run to its end */
+#define RM_SCHEDULE_PIXEL 0            /* one thread per pixel, each path run to its end */
 #define RM_SCHEDULE_COMPACT 1          /* a block's live paths are packed into its first waves between bounces */
 #define RM_SCHEDULE_PERSISTENT 2       /* persistent lanes that regenerate finished paths from a tile queue */
 #define RM_SCHEDULE_WAVEFRONT 3        /* one kernel per stage (extend, shade, shadow, finish) over queues of
@@ -91,6 +94,10 @@ extern "C" {
                                           sorted: by the kind of material hit after tracing, by ray-direction
                                           octant and specular-or-not after shading, so waves shade and trace
                                           alike paths */
+#define RM_SCHEDULE_SPLIT 5            /* the schedule a context starts with: a thread a pixel for the first
+                                          RM_OPTION_SPLIT_DEPTH bounces, then persistent waves over a queue of
+                                          the paths still alive (frames of one sample per pixel in the colour
+                                          view; RM_SCHEDULE_PIXEL runs the others) */
 
 typedef struct rm_context rm_context;
 typedef struct rm_scene rm_scene;
@@ -236,30 +243,33 @@ int rm_stage_times(rm_context* ctx, double* ms, uint32_t* launches, uint32_t n);
 int rm_reset_stage_times(rm_context* ctx);
 
 /* Sets an RM_OPTION_* for later frames. No counterpart in the reference:
- * these choose among equivalent kernel schedules (and let a tuner pick). */
+ * these choose among equivalent kernel schedules (and let a tuner pick). A
+ * context starts with RM_SCHEDULE_SPLIT and every other option at 0. */
 int rm_set_option(rm_context* ctx, int option, int64_t value);
 
 /* The autotuner. Renders `frames` whole frames after one warm-up with the
  * caller's parameters — light tracing (given its emitters) and the denoiser
- * included when they turn them on — under each candidate: one thread per
- * pixel at four and at five waves per SIMD, at four with 16 stack entries in
- * LDS and with row-shaped blocks; compacted; sorted; persistent at two grid
- * sizes; wavefront. It sets the fastest as the context's RM_OPTION_SCHEDULE,
- * RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES, RM_OPTION_STACK_DEPTH and
- * RM_OPTION_BLOCK_SHAPE. Whole frames, because the path pass shares the GPU
+ * included when they turn them on — under each candidate: split after two
+ * bounces at four and at five waves per SIMD, and after one at five; one
+ * thread per pixel at four and at five waves, at four with 16 stack entries
+ * in LDS and with row-shaped blocks; compacted; sorted; persistent at two
+ * grid sizes; wavefront. It sets the fastest as the context's
+ * RM_OPTION_SCHEDULE, RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES,
+ * RM_OPTION_STACK_DEPTH, RM_OPTION_BLOCK_SHAPE and RM_OPTION_SPLIT_DEPTH. Whole frames, because the path pass shares the GPU
  * with light tracing and the previous frame's denoiser: a path pass that
  * holds more waves can be faster alone and still make the frame slower. The
  * candidates give the same sums bit for bit, so tuning changes speed only.
  * The frames render into the context's buffers: begin the image again
  * (sample offset 0) afterwards. On failure the context keeps the options it
  * had. No counterpart in the reference. */
-#define RM_TUNE_CANDIDATES 9
+#define RM_TUNE_CANDIDATES 12
 typedef struct rm_tune_result {
     int32_t schedule[RM_TUNE_CANDIDATES];            /* each candidate's RM_SCHEDULE_* */
     uint32_t persistent_blocks[RM_TUNE_CANDIDATES];  /* and its persistent grid (0: fill the GPU) */
     uint32_t waves[RM_TUNE_CANDIDATES];              /* and its RM_OPTION_WAVES */
     uint32_t stack_depth[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_STACK_DEPTH */
     uint32_t block_shape[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_BLOCK_SHAPE */
+    uint32_t split_depth[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_SPLIT_DEPTH */
     double ms[RM_TUNE_CANDIDATES];                   /* mean ms per whole frame */
     uint32_t count;                                  /* candidates tried */
     uint32_t best;                                   /* the one now set */

@@ -218,7 +218,8 @@ static napi_value js_frame(napi_env env, napi_callback_info info) {
 /* tune(ctx, scene, params, emitters, frames): rm_tune, which renders whole
  * frames under each candidate and keeps the fastest on the context. Blocks
  * until the frames are done. Returns {best, candidates: [{schedule,
- * persistentBlocks, waves, ms}]}, best an index into candidates. */
+ * persistentBlocks, waves, stackDepth, blockShape, splitDepth, ms}]}, best
+ * an index into candidates. */
 static napi_value candidate_object(napi_env env, const rm_tune_result* res, uint32_t k) {
     napi_value o, v;
     CHECK(napi_create_object(env, &o));
@@ -232,6 +233,8 @@ static napi_value candidate_object(napi_env env, const rm_tune_result* res, uint
     CHECK(napi_set_named_property(env, o, "stackDepth", v));
     CHECK(napi_create_uint32(env, res->block_shape[k], &v));
     CHECK(napi_set_named_property(env, o, "blockShape", v));
+    CHECK(napi_create_uint32(env, res->split_depth[k], &v));
+    CHECK(napi_set_named_property(env, o, "splitDepth", v));
     CHECK(napi_create_double(env, res->ms[k], &v));
     CHECK(napi_set_named_property(env, o, "ms", v));
     return o;

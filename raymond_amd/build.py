@@ -40,8 +40,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-fno-slp-vectorize",
     f"-I{ROOT / 'include'}", "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["main.hip", "main5.hip", "main_shapes.hip", "batch.hip", "schedules.hip", "light.hip", "kernels.hip",
-           "api.hip", "server.hip"]
+SOURCES = ["main.hip", "main5.hip", "main_shapes.hip", "split.hip", "split5.hip", "batch.hip", "schedules.hip",
+           "light.hip", "kernels.hip", "api.hip", "server.hip"]
 # Host-only C++ (the BVH builder): double precision as written, no FMA contraction.
 CXX_SOURCES = ["bvh_build.cpp"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",

@@ -55,23 +55,25 @@ SIGNATURES = {
     "rm_server_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
 (OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE, OPTION_BATCH, OPTION_WAVES,
- OPTION_LIGHT_PATHS, OPTION_STACK_DEPTH, OPTION_BLOCK_SHAPE) = range(9)
+ OPTION_LIGHT_PATHS, OPTION_STACK_DEPTH, OPTION_BLOCK_SHAPE, OPTION_SPLIT_DEPTH) = range(10)
 BLOCK_SQUARES, BLOCK_ROWS = 0, 1
 PRECISION_QUANTIZED_BVH, PRECISION_FP16_GUIDES, PRECISION_COMPENSATED = 1, 2, 4
-SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT, SCHEDULE_WAVEFRONT, SCHEDULE_SORTED = 0, 1, 2, 3, 4
+SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT, SCHEDULE_WAVEFRONT, SCHEDULE_SORTED, SCHEDULE_SPLIT = range(6)
 SCHEDULES = {"per-pixel": SCHEDULE_PIXEL, "compact": SCHEDULE_COMPACT, "persistent": SCHEDULE_PERSISTENT,
-             "wavefront": SCHEDULE_WAVEFRONT, "sorted": SCHEDULE_SORTED}
+             "wavefront": SCHEDULE_WAVEFRONT, "sorted": SCHEDULE_SORTED, "split": SCHEDULE_SPLIT}
+SCHEDULE_DEFAULT = SCHEDULE_SPLIT   # the schedule a context (and a server's jobs) start with
 
 _lib = None
 
 
-TUNE_CANDIDATES = 9
+TUNE_CANDIDATES = 12
 
 
 class _TuneResult(C.Structure):
     _fields_ = [("schedule", C.c_int32 * TUNE_CANDIDATES), ("persistent_blocks", C.c_uint32 * TUNE_CANDIDATES),
                 ("waves", C.c_uint32 * TUNE_CANDIDATES), ("stack_depth", C.c_uint32 * TUNE_CANDIDATES),
-                ("block_shape", C.c_uint32 * TUNE_CANDIDATES), ("ms", C.c_double * TUNE_CANDIDATES),
+                ("block_shape", C.c_uint32 * TUNE_CANDIDATES), ("split_depth", C.c_uint32 * TUNE_CANDIDATES),
+                ("ms", C.c_double * TUNE_CANDIDATES),
                 ("count", C.c_uint32), ("best", C.c_uint32)]
 
 
@@ -271,19 +273,20 @@ class Context:
 
     def tune(self, scene: Scene, params: np.ndarray, frames: int = 2, emitters: np.ndarray | None = None) -> dict:
         """rm_tune: times whole frames under each candidate (schedule,
-        persistent blocks, waves, LDS stack depth, block shape) and keeps the
-        fastest; returns {"best": (schedule name, blocks, waves, depth,
-        shape), "candidates": [(schedule name, blocks, waves, depth, shape,
-        ms per frame), ...]}. emitters: as for frame(), when the parameters
-        turn light tracing on."""
+        persistent blocks, waves, LDS stack depth, block shape, split depth)
+        and keeps the fastest; returns {"best": (schedule name, blocks, waves,
+        depth, shape, split depth), "candidates": [(schedule name, blocks,
+        waves, depth, shape, split depth, ms per frame), ...]}. emitters: as
+        for frame(), when the parameters turn light tracing on."""
         res = _TuneResult()
         em = None if emitters is None else np.ascontiguousarray(emitters, dtype=np.float32)
         _check(load().rm_tune(self.handle, scene.handle, _ptr(np.ascontiguousarray(params)), _ptr(em),
                               0 if em is None else em.size // 4, frames, C.byref(res)), "rm_tune")
         names = {v: k for k, v in SCHEDULES.items()}
         cands = [(names[res.schedule[i]], res.persistent_blocks[i], res.waves[i], res.stack_depth[i],
-                  "rows" if res.block_shape[i] == BLOCK_ROWS else "squares", res.ms[i]) for i in range(res.count)]
-        return {"best": cands[res.best][:5], "candidates": cands}
+                  "rows" if res.block_shape[i] == BLOCK_ROWS else "squares", res.split_depth[i], res.ms[i])
+                 for i in range(res.count)]
+        return {"best": cands[res.best][:6], "candidates": cands}
 
     def set_option(self, option: int, value: int) -> None:
         _check(load().rm_set_option(self.handle, option, value), "rm_set_option")

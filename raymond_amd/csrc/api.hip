@@ -319,7 +319,7 @@ std::vector<DevBuf*> working_buffers(rm_context* c) {
             &c->guides,    &c->albedo_frame, &c->splats[0],  &c->splats[1],   &c->gathered[0], &c->gathered[1],
             &c->photon_counts, &c->photon_starts, &c->photons, &c->gather_queue, &c->emitters, &c->spill, &c->side_spill, &c->rays,
             &c->hits,      &c->counters,    &c->staging,     &c->display,     &c->comp,      &c->folded,
-            &c->wavefront};
+            &c->wavefront, &c->split};
 }
 }  // namespace
 
@@ -444,9 +444,11 @@ int rm_open(int device, rm_context** out) {
 int rm_set_option(rm_context* c, int option, int64_t value) {
     if (!c) return fail(RM_ERR_INVALID, "rm_set_option needs a context.");
     if (option < 0 || option >= RM_OPTION_COUNT) return fail(RM_ERR_INVALID, "Unknown option " + std::to_string(option) + ".");
-    if (option == RM_OPTION_SCHEDULE && (value < RM_SCHEDULE_PIXEL || value > RM_SCHEDULE_SORTED))
-        return fail(RM_ERR_INVALID,
-                    "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT, _PERSISTENT, _WAVEFRONT or _SORTED.");
+    if (option == RM_OPTION_SCHEDULE && (value < RM_SCHEDULE_PIXEL || value > RM_SCHEDULE_SPLIT))
+        return fail(RM_ERR_INVALID, "RM_OPTION_SCHEDULE is one of RM_SCHEDULE_PIXEL, _COMPACT, _PERSISTENT, "
+                                    "_WAVEFRONT, _SORTED or _SPLIT.");
+    if (option == RM_OPTION_SPLIT_DEPTH && (value < 0 || value > 8))
+        return fail(RM_ERR_INVALID, "RM_OPTION_SPLIT_DEPTH is 1 to 8 bounces (0: the default, 2).");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
     if (option == RM_OPTION_WAVES && value != 0 && value != 4 && value != 5)
@@ -481,7 +483,7 @@ int rm_close(rm_context* c) {
     for (DevBuf* b : {&c->accum, &c->galb, &c->gnorm, &c->light_a, &c->light_b, &c->denoised, &c->splats[0], &c->splats[1],
                       &c->guides, &c->albedo_frame, &c->photon_counts, &c->photon_starts, &c->photons, &c->gather_queue, &c->emitters, &c->spill, &c->side_spill,
                       &c->gathered[0], &c->gathered[1], &c->env, &c->env_table, &c->rays, &c->hits, &c->counters,
-                      &c->work, &c->staging, &c->display, &c->comp, &c->folded, &c->wavefront})
+                      &c->work, &c->staging, &c->display, &c->comp, &c->folded, &c->wavefront, &c->split})
         b->release();
     // The streams are idle: the blocks just given back need no events, which
     // must not outlive the streams they were recorded on.
@@ -773,6 +775,8 @@ struct Frame {
     uint32_t slots = 0, persistent = 0;
     float4* staging = nullptr;
     bool wavefront = false, sorted = false;
+    uint32_t* split = nullptr;
+    uint32_t split_blocks = 0;
 };
 
 // The photon gather's queue for px pixels: a 96-byte slot each, then its length.
@@ -882,6 +886,8 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
     F.staging = nullptr;
     F.wavefront = false;
     F.sorted = false;
+    F.split = nullptr;
+    F.split_blocks = 0;
     const int64_t schedule = path_buffers ? c->options[RM_OPTION_SCHEDULE] : RM_SCHEDULE_PIXEL;
     if (schedule == RM_SCHEDULE_PERSISTENT && P.view == 0u) {
         int64_t blocks = c->options[RM_OPTION_PERSISTENT_BLOCKS];
@@ -908,6 +914,19 @@ static int frame_alloc(rm_context* c, const void* emitters, uint32_t emitter_vec
         RM_HIP(c->wavefront.ensure(rm::wavefront_bytes(px)), "allocating the wavefront path state");
         F.wavefront = true;
         F.slots = threads;
+    } else if (schedule == RM_SCHEDULE_SPLIT && P.view == 0u && P.samples_per_frame == 1u) {
+        // The tail's grid, with no more blocks than the queue could fill. A
+        // lane's stack spill is its pixel's in the head and its grid slot in
+        // the tail.
+        F.split_blocks = std::min(rm::split_tail_blocks(c->compute_units, c->options[RM_OPTION_WAVES] == 5 ? 5 : 4),
+                                  (px + rm::BLOCK_THREADS - 1) / rm::BLOCK_THREADS);
+        const uint32_t threads = F.split_blocks * rm::BLOCK_THREADS;
+        const uint32_t lanes = threads > px ? threads : px;
+        int r = ensure_spill(c, lanes);
+        if (r != RM_OK) return r;
+        RM_HIP(c->split.ensure(rm::split_bytes(px)), "allocating the split schedule's path queue");
+        F.split = c->split.as<uint32_t>();
+        F.slots = lanes;
     }
     return RM_OK;
 }
@@ -1028,7 +1047,9 @@ static int frame_path(rm_context* c, Frame& F) {
         rm::LaunchMain L{S, P, c->accum.as<float4>(), c->galb.as<float4>(), c->gnorm.as<float4>(), F.comp,
                          c->spill.as<uint32_t>(), F.slots, F.persistent, c->work.as<uint32_t>(), F.staging, f.tree,
                          c->options[RM_OPTION_WAVES] == 5 ? 5 : 4, F.sorted,
-                         c->options[RM_OPTION_STACK_DEPTH] == 16 ? 16 : 8, (int)c->options[RM_OPTION_BLOCK_SHAPE]};
+                         c->options[RM_OPTION_STACK_DEPTH] == 16 ? 16 : 8, (int)c->options[RM_OPTION_BLOCK_SHAPE],
+                         F.split, c->options[RM_OPTION_SPLIT_DEPTH] > 0 ? (uint32_t)c->options[RM_OPTION_SPLIT_DEPTH] : 2u,
+                         F.split_blocks};
         rm::launch_main(L, f, c->stream);
     }
     RM_HIP(hipGetLastError(), "launching the trace");
@@ -1420,20 +1441,25 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
     const uint32_t spp = P.samples_per_frame > 0 ? P.samples_per_frame : 1u;
     const uint32_t cus = (uint32_t)(c->compute_units > 0 ? c->compute_units : 1);
     // The candidates: schedule, persistent blocks, waves, LDS stack depth, block shape.
-    const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_PIXEL,   RM_SCHEDULE_PIXEL,      RM_SCHEDULE_PIXEL,
-                                                  RM_SCHEDULE_PIXEL,   RM_SCHEDULE_COMPACT,    RM_SCHEDULE_SORTED,
+    // The candidates: schedule, persistent blocks, waves, LDS stack depth,
+    // block shape, split depth.
+    const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_SPLIT,   RM_SCHEDULE_SPLIT,   RM_SCHEDULE_SPLIT,
+                                                  RM_SCHEDULE_PIXEL,   RM_SCHEDULE_PIXEL,   RM_SCHEDULE_PIXEL,
+                                                  RM_SCHEDULE_PIXEL,   RM_SCHEDULE_COMPACT, RM_SCHEDULE_SORTED,
                                                   RM_SCHEDULE_PERSISTENT, RM_SCHEDULE_PERSISTENT, RM_SCHEDULE_WAVEFRONT};
-    const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 2u * cus, 0u};
-    const uint32_t waves[RM_TUNE_CANDIDATES] = {4u, 5u, 4u, 4u, 4u, 4u, 4u, 4u, 4u};
-    const uint32_t depths[RM_TUNE_CANDIDATES] = {8u, 8u, 16u, 8u, 8u, 8u, 8u, 8u, 8u};
-    const uint32_t shapes[RM_TUNE_CANDIDATES] = {RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_ROWS,
-                                                RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES,
-                                                RM_BLOCK_SQUARES, RM_BLOCK_SQUARES};
+    const uint32_t blocks[RM_TUNE_CANDIDATES] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 2u * cus, 0u};
+    const uint32_t waves[RM_TUNE_CANDIDATES] = {4u, 5u, 5u, 4u, 5u, 4u, 4u, 4u, 4u, 4u, 4u, 4u};
+    const uint32_t depths[RM_TUNE_CANDIDATES] = {8u, 8u, 8u, 8u, 8u, 16u, 8u, 8u, 8u, 8u, 8u, 8u};
+    const uint32_t shapes[RM_TUNE_CANDIDATES] = {RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES,
+                                                RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_ROWS,    RM_BLOCK_SQUARES,
+                                                RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES, RM_BLOCK_SQUARES};
+    const uint32_t splits[RM_TUNE_CANDIDATES] = {2u, 2u, 1u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
     rm_tune_result res = {};
-    const int tuned[5] = {RM_OPTION_SCHEDULE, RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES, RM_OPTION_STACK_DEPTH,
-                          RM_OPTION_BLOCK_SHAPE};
-    int64_t before[5];
-    for (int k = 0; k < 5; ++k) before[k] = c->options[tuned[k]];
+    constexpr int TUNED = 6;
+    const int tuned[TUNED] = {RM_OPTION_SCHEDULE,    RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES,
+                              RM_OPTION_STACK_DEPTH, RM_OPTION_BLOCK_SHAPE,       RM_OPTION_SPLIT_DEPTH};
+    int64_t before[TUNED];
+    for (int k = 0; k < TUNED; ++k) before[k] = c->options[tuned[k]];
     auto finish_all = [&]() -> int {
         RM_HIP(hipStreamSynchronize(c->stream), "tuning");
         RM_HIP(hipStreamSynchronize(c->side), "tuning");
@@ -1448,6 +1474,7 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         c->options[RM_OPTION_WAVES] = waves[k];
         c->options[RM_OPTION_STACK_DEPTH] = depths[k];
         c->options[RM_OPTION_BLOCK_SHAPE] = shapes[k];
+        c->options[RM_OPTION_SPLIT_DEPTH] = splits[k];
         std::chrono::steady_clock::time_point t0;
         for (uint32_t f = 0; f <= frames && r == RM_OK; ++f) {
             if (f == 1) {
@@ -1466,6 +1493,7 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         res.waves[k] = waves[k];
         res.stack_depth[k] = depths[k];
         res.block_shape[k] = shapes[k];
+        res.split_depth[k] = splits[k];
         res.ms[k] = ms / frames;
         res.count = k + 1;
         if (k == 0 || res.ms[k] < best_ms) {
@@ -1474,9 +1502,9 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         }
     }
     const uint32_t pick = res.best;
-    const int64_t picked[5] = {res.schedule[pick], res.persistent_blocks[pick], res.waves[pick], res.stack_depth[pick],
-                               res.block_shape[pick]};
-    for (int k = 0; k < 5; ++k) c->options[tuned[k]] = r == RM_OK ? picked[k] : before[k];
+    const int64_t picked[TUNED] = {res.schedule[pick],     res.persistent_blocks[pick], res.waves[pick],
+                                   res.stack_depth[pick], res.block_shape[pick],       res.split_depth[pick]};
+    for (int k = 0; k < TUNED; ++k) c->options[tuned[k]] = r == RM_OK ? picked[k] : before[k];
     if (r != RM_OK) return r;
     if (out) *out = res;
     return RM_OK;

@@ -107,6 +107,8 @@ struct rm_scene {
     }
 };
 
+static_assert(RM_OPTION_SCHEDULE == 0, "the schedule is the first option");
+
 struct rm_context {
     int device = 0;
     rmi::Owner owner;   // the device and streams its buffers are used on
@@ -148,6 +150,7 @@ struct rm_context {
     // last frame kept it up to date; folded holds sum + error for readers.
     rmi::DevBuf comp, folded;
     rmi::DevBuf wavefront;   // RM_SCHEDULE_WAVEFRONT's path state and queues
+    rmi::DevBuf split;       // RM_SCHEDULE_SPLIT's queue of paths and its counters
     bool comp_live = false;
     rm::Params last_params = {};   // the last frame's, for rm_display
     bool rendered = false;
@@ -155,7 +158,9 @@ struct rm_context {
     // This device's copy of the static raycast scratch (rm_raycast_static),
     // looked up by rm_open.
     uint2* static_scratch = nullptr;
-    int64_t options[RM_OPTION_COUNT] = {};
+    // A context starts on the split schedule (the first option); every other
+    // option starts at 0, its default.
+    int64_t options[RM_OPTION_COUNT] = {RM_SCHEDULE_SPLIT};
     bool timing = false;
     rmi::StageTimer timer;
     std::vector<rm_scene*> scenes;

@@ -43,7 +43,17 @@ struct LaunchMain {
     bool sorted = false;         // with staging: k_main_compact sorting live paths by key between bounces
     int stack_depth = 8;         // k_main at four waves: traversal stack entries in LDS, 8 or 16
     int block_shape = 0;         // k_main at four waves: BLOCK_SQUARES (16×16) or BLOCK_ROWS (64×4)
+    uint32_t* split = nullptr;   // non-null: the split schedule's queue of W·H slots, then its two counters
+    uint32_t split_depth = 2;    // bounces the split schedule's head kernel runs before queueing a path
+    uint32_t split_blocks = 0;   // the tail kernel's persistent grid
 };
+// The split schedule's buffer for px pixels: the queue (SLOT_WORDS words a
+// path, the path slots' layout), then two counters at this word offset.
+__host__ __device__ inline size_t split_count_offset(size_t px) { return (px * 27 + 63) / 64 * 64; }
+inline size_t split_bytes(size_t px) { return (split_count_offset(px) + 64) * 4; }
+// The tail's persistent grid on a device of `cus` compute units: as many
+// 256-thread blocks a CU as the kernels' waves per SIMD (four SIMDs a CU).
+inline uint32_t split_tail_blocks(int cus, int waves) { return (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)waves; }
 
 void launch_main(const LaunchMain& L, const Features& f, hipStream_t stream);
 // The one-thread-per-pixel pass compiled for five waves per SIMD (main5.hip).
@@ -54,6 +64,10 @@ void launch_main_rows(const LaunchMain& L, const Features& f, hipStream_t stream
 // The persistent and compacted schedules' launches (launch_main picks them).
 void launch_main_persistent(const LaunchMain& L, const Features& f, hipStream_t stream);
 void launch_main_compact(const LaunchMain& L, const Features& f, hipStream_t stream);
+// The split schedule: a head kernel over pixels, then a persistent tail over
+// the paths still alive after split_depth bounces (split.hip).
+void launch_main_split(const LaunchMain& L, const Features& f, hipStream_t stream);
+void launch_main_split5(const LaunchMain& L, const Features& f, hipStream_t stream);
 
 // Several jobs' path passes in one launch of the per-pixel kernel, the job
 // table passed by value as the kernel's argument.

@@ -20,83 +20,6 @@ constexpr uint32_t CLAIM_ITEMS = 256u;
 // bounce ties a block's waves to its slowest one, so with short paths (1 spp
 // per frame on the sphere scene) k_main is faster.
 constexpr int COMPACT_LDS_DEPTH = 4;
-constexpr int SLOT_WORDS = 27;
-
-struct SlotStore {
-    uint32_t* w;      // [SLOT_WORDS][stride]
-    uint32_t stride;  // slots: BLOCK_THREADS in LDS, the path count in global memory
-    RM_D void put(uint32_t slot, int k, float v) { w[(size_t)k * stride + slot] = __float_as_uint(v); }
-    RM_D void putu(uint32_t slot, int k, uint32_t v) { w[(size_t)k * stride + slot] = v; }
-    RM_D float get(uint32_t slot, int k) const { return __uint_as_float(w[(size_t)k * stride + slot]); }
-    RM_D uint32_t getu(uint32_t slot, int k) const { return w[(size_t)k * stride + slot]; }
-    RM_D void put3(uint32_t slot, int k, V3 v) { put(slot, k, v.x); put(slot, k + 1, v.y); put(slot, k + 2, v.z); }
-    RM_D V3 get3(uint32_t slot, int k) const { return {get(slot, k), get(slot, k + 1), get(slot, k + 2)}; }
-};
-
-// A live path's state between bounces. The RNG's seed, sample index and
-// spectrum offset are functions of the pixel and sample, and its dimension is
-// reset at every bounce, so they are recomputed rather than moved.
-RM_D void slot_save(SlotStore& st, uint32_t slot, const PathState& ps, const Guides& gd, uint32_t pix) {
-    st.put3(slot, 0, ps.o);
-    st.put3(slot, 3, ps.d);
-    st.put3(slot, 6, ps.throughput);
-    st.put3(slot, 9, ps.result);
-    st.put3(slot, 12, ps.last_p);
-    st.put(slot, 15, ps.last_pdf);
-    st.put(slot, 16, ps.travelled);
-    st.put(slot, 17, ps.wavelength);
-    st.putu(slot, 18, ps.depth | (ps.last_delta ? 1u << 24 : 0u) | (ps.guided ? 1u << 25 : 0u) |
-                          (ps.caustic ? 1u << 26 : 0u) | (ps.diffuse_seen ? 1u << 27 : 0u));
-    st.put3(slot, 19, gd.albedo);
-    st.put(slot, 22, gd.normal.x);
-    st.put(slot, 23, gd.normal.y);
-    st.put(slot, 24, gd.normal.z);
-    st.put(slot, 25, gd.normal.w);
-    st.putu(slot, 26, pix);
-}
-// The RNG and spectrum offset of pixel `index`'s sample, as path_begin seeds
-// them, with the dimension back at 0 for the next bounce.
-RM_D void reseed(Rng& g, PathState& ps, uint32_t index, uint32_t sample) {
-    g.state = pcg(index ^ pcg(sample));
-    g.qmc = true;
-    g.seed = pcg(index + 0x9e3779b9u);
-    g.index = sample;
-    g.dim = 0u;
-    ps.spectrum_u = (float)((pcg(index * 7919u + 1u) + sample * 2654435769u) >> 8u) * (1.0f / 16777216.0f);
-}
-// The fields slot_save wrote.
-RM_D void slot_fields(const SlotStore& st, uint32_t slot, PathState& ps, Guides& gd, uint32_t& pix) {
-    ps.o = st.get3(slot, 0);
-    ps.d = st.get3(slot, 3);
-    ps.throughput = st.get3(slot, 6);
-    ps.result = st.get3(slot, 9);
-    ps.last_p = st.get3(slot, 12);
-    ps.last_pdf = st.get(slot, 15);
-    ps.travelled = st.get(slot, 16);
-    ps.wavelength = st.get(slot, 17);
-    uint32_t bits = st.getu(slot, 18);
-    ps.depth = bits & 0xffffffu;
-    ps.last_delta = (bits >> 24) & 1u;
-    ps.guided = (bits >> 25) & 1u;
-    ps.caustic = (bits >> 26) & 1u;
-    ps.diffuse_seen = (bits >> 27) & 1u;
-    gd.albedo = st.get3(slot, 19);
-    gd.normal = make_float4(st.get(slot, 22), st.get(slot, 23), st.get(slot, 24), st.get(slot, 25));
-    pix = st.getu(slot, 26);
-}
-RM_D void slot_load(const SlotStore& st, uint32_t slot, const Params& P, uint32_t sample, PathState& ps, Guides& gd,
-                    Rng& g, uint32_t& pix) {
-    slot_fields(st, slot, ps, gd, pix);
-    uint32_t x, y;
-    tile_pixel_of(pix, x, y);
-    reseed(g, ps, y * P.width + x, sample);
-}
-// A wavefront slot: its word 26 is the pixel's index itself.
-RM_D void slot_load_wf(const SlotStore& st, uint32_t slot, uint32_t sample, PathState& ps, Guides& gd, Rng& g,
-                       uint32_t& pix) {
-    slot_fields(st, slot, ps, gd, pix);
-    reseed(g, ps, pix, sample);
-}
 
 // Sorting (SORT, RM_SCHEDULE_SORTED): each bounce is split in two at its
 // closest hit, and while more than one wave's worth of paths is alive the

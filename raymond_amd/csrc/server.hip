@@ -80,7 +80,7 @@ std::string cancel_message(int why) {
 
 // Device memory a job's context needs, by the buffers rm_frame (and rm_tune)
 // size for it. options: the job's RM_OPTION_* values.
-uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options, uint64_t env_bytes) {
+uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options, uint64_t env_bytes, int cus) {
     const uint64_t px = (uint64_t)w * h;
     const bool compensated = (options[RM_OPTION_PRECISION] & RM_PRECISION_COMPENSATED) != 0;
     const int64_t schedule = options[RM_OPTION_SCHEDULE];
@@ -94,6 +94,11 @@ uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options
     const uint64_t tiled = (uint64_t)((w + 15) / 16) * ((h + 15) / 16) * rm::BLOCK_THREADS;
     if (tuned || schedule == RM_SCHEDULE_COMPACT || schedule == RM_SCHEDULE_SORTED) b += tiled * 48 + (tiled > px ? tiled - px : 0) * rm::SPILL_ENTRIES * 8;
     if (tuned || schedule == RM_SCHEDULE_WAVEFRONT) b += rm::wavefront_bytes(px);
+    if (tuned || schedule == RM_SCHEDULE_SPLIT) {
+        // The split schedule's queue, and its tail's spill lanes past the pixels'.
+        const uint64_t tail = (uint64_t)rm::split_tail_blocks(cus, 5) * rm::BLOCK_THREADS;
+        b += rm::split_bytes(px) + (tail > px ? tail - px : 0) * rm::SPILL_ENTRIES * 8;
+    }
     if (caustics) {
         const uint64_t budget = (uint64_t)options[RM_OPTION_LIGHT_PATHS];
         const uint64_t photons = budget > rm::PHOTON_CAPACITY ? budget : rm::PHOTON_CAPACITY;
@@ -111,11 +116,11 @@ uint64_t job_bytes(uint32_t w, uint32_t h, bool caustics, const int64_t* options
 struct rm_server {
     int device = 0;
     uint64_t budget = 0, used = 0;
-    int64_t options[RM_OPTION_COUNT] = {};   // given to jobs submitted from now on
+    int64_t options[RM_OPTION_COUNT] = {RM_SCHEDULE_SPLIT};   // given to jobs submitted from now on
     // The schedule rm_tune picked for each (scene, width, height, samples per
     // frame), with RM_OPTION_AUTOTUNE; used by the scheduler thread only.
-    // The options rm_tune set: schedule, persistent blocks, waves, stack depth, block shape.
-    std::map<std::tuple<const rm_scene*, uint32_t, uint32_t, uint32_t>, std::array<int64_t, 5>> tuned;
+    // The options rm_tune set: schedule, persistent blocks, waves, stack depth, block shape, split depth.
+    std::map<std::tuple<const rm_scene*, uint32_t, uint32_t, uint32_t>, std::array<int64_t, 6>> tuned;
     rm_context* home = nullptr;
     std::vector<Slot> slots;
     std::mutex m;
@@ -250,15 +255,17 @@ bool rm_server::admit() {
                             j->caustics ? (uint32_t)(j->emitters.size() / 4) : 0u,
                             (uint32_t)j->options[RM_OPTION_AUTOTUNE], &res);
                 if (r == RM_OK)
-                    it = tuned.emplace(key, std::array<int64_t, 5>{res.schedule[res.best], res.persistent_blocks[res.best],
+                    it = tuned.emplace(key, std::array<int64_t, 6>{res.schedule[res.best], res.persistent_blocks[res.best],
                                                                    res.waves[res.best], res.stack_depth[res.best],
-                                                                   res.block_shape[res.best]}).first;
+                                                                   res.block_shape[res.best], res.split_depth[res.best]})
+                             .first;
             }
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_SCHEDULE, it->second[0]);
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_PERSISTENT_BLOCKS, it->second[1]);
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_WAVES, it->second[2]);
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_STACK_DEPTH, it->second[3]);
             if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_BLOCK_SHAPE, it->second[4]);
+            if (r == RM_OK) r = rm_set_option(s.ctx, RM_OPTION_SPLIT_DEPTH, it->second[5]);
         }
         if (r == RM_OK) {
             // The job is charged what its slot now holds, if that is more
@@ -534,7 +541,7 @@ int rm_server_submit(rm_server* sv, const rm_job* job, uint64_t* id) {
     const uint64_t env_bytes = j->env_rgba.empty()
                                    ? 0
                                    : (uint64_t)j->env_w * j->env_h * 8 + (uint64_t)j->table_w * (j->table_h + 1) * 4;
-    j->bytes = job_bytes(j->width, j->height, j->caustics, j->options, env_bytes);
+    j->bytes = job_bytes(j->width, j->height, j->caustics, j->options, env_bytes, sv->home->compute_units);
     // Scene uploads on the home context must have landed before any slot reads them.
     RM_HIP(hipSetDevice(sv->device), "selecting the device");
     RM_HIP(hipStreamSynchronize(sv->home->stream), "waiting for scene uploads");

@@ -42,4 +42,4 @@ export const OPTION_LIGHT_PATHS = 6;
 
 /** RM_OPTION_SCHEDULE and its values. */
 export const OPTION_SCHEDULE = 0;
-export const SCHEDULES = Object.freeze({ "per-pixel": 0, compact: 1, persistent: 2, wavefront: 3, sorted: 4 });
+export const SCHEDULES = Object.freeze({ "per-pixel": 0, compact: 1, persistent: 2, wavefront: 3, sorted: 4, split: 5 });

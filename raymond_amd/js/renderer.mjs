@@ -49,7 +49,7 @@ export class Renderer {
     #addon; #ctx; #canvas;
     #scene = null; #camera = null;
     #maxBounces = 8; #samplesPerFrame = 1; #exposure = 1.0; #fireflyClamp = 20.0;
-    #view = "color"; #denoise = true; #caustics = true; #precision = {}; #schedule = "per-pixel"; #waves = 4;
+    #view = "color"; #denoise = true; #caustics = true; #precision = {}; #schedule = "split"; #waves = 4;
     #causticsActive = false;
     #samples = 0;
     #params = new ArrayBuffer(PARAMS_BYTES);
@@ -231,8 +231,8 @@ export class Renderer {
      * Renders a few whole frames under each candidate schedule and occupancy
      * (rm_tune) and keeps the fastest for the frames from now on; the image
      * starts again. Blocks until the frames are done. Returns { schedule,
-     * waves, persistentBlocks, ms, candidates: [{ schedule, waves,
-     * persistentBlocks, ms }] }. Not in the reference.
+     * waves, persistentBlocks, stackDepth, blockShape, splitDepth, ms,
+     * candidates: [the same for each] }. Not in the reference.
      */
     tune({ frames = 2 } = {}) {
         if (!this.#scene || !this.#camera)
@@ -245,7 +245,7 @@ export class Renderer {
         const candidates = result.candidates.map((c) => ({
             schedule: names.find((k) => SCHEDULES[k] == c.schedule), waves: c.waves,
             persistentBlocks: c.persistentBlocks, stackDepth: c.stackDepth,
-            blockShape: c.blockShape == 1 ? "rows" : "squares", ms: c.ms,
+            blockShape: c.blockShape == 1 ? "rows" : "squares", splitDepth: c.splitDepth, ms: c.ms,
         }));
         const best = candidates[result.best];
         this.#schedule = best.schedule;

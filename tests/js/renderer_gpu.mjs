@@ -49,7 +49,7 @@ async function main() {
     if (!varied.every(Number.isFinite)) throw new Error("the varied frames should be finite");
     // The tuner renders whole frames under each candidate and keeps the fastest.
     const tuned = renderer.tune({ frames: 1 });
-    if (tuned.candidates.length != 9) throw new Error("the tuner should try nine candidates");
+    if (tuned.candidates.length != 12) throw new Error("the tuner should try twelve candidates");
     if (tuned.ms != Math.min(...tuned.candidates.map((c) => c.ms))) throw new Error("the tuner should keep the fastest");
     if (renderer.schedule != tuned.schedule || renderer.waves != tuned.waves || renderer.samples != 0)
         throw new Error("the renderer should take the tuned schedule and start the image again");

@@ -125,9 +125,9 @@ def test_python_and_js_constants_match_the_header(tmp_path):
     defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define (RM_\w+) \(?(-?\d+)\)?", header)}
     py = {f"RM_OPTION_{n}": getattr(capi, f"OPTION_{n}")
           for n in ("SCHEDULE", "PERSISTENT_BLOCKS", "PRECISION", "AUTOTUNE", "BATCH", "WAVES", "LIGHT_PATHS",
-                    "STACK_DEPTH", "BLOCK_SHAPE")}
+                    "STACK_DEPTH", "BLOCK_SHAPE", "SPLIT_DEPTH")}
     py.update({f"RM_SCHEDULE_{n}": getattr(capi, f"SCHEDULE_{n}") for n in ("PIXEL", "COMPACT", "PERSISTENT",
-                                                                            "WAVEFRONT", "SORTED")})
+                                                                            "WAVEFRONT", "SORTED", "SPLIT")})
     py.update({f"RM_BLOCK_{n}": getattr(capi, f"BLOCK_{n}") for n in ("SQUARES", "ROWS")})
     py["RM_ERR_CANCELLED"] = capi.ERR_CANCELLED
     py.update({f"RM_PRECISION_{n}": getattr(capi, f"PRECISION_{n}") for n in ("QUANTIZED_BVH", "FP16_GUIDES",
@@ -143,6 +143,9 @@ def test_python_and_js_constants_match_the_header(tmp_path):
     js = (root / "raymond_amd" / "js" / "addon.mjs").read_text()
     for name in ("OPTION_PRECISION", "OPTION_SCHEDULE", "OPTION_WAVES", "OPTION_LIGHT_PATHS"):
         assert int(re.search(rf"export const {name} = (\d+);", js).group(1)) == defs["RM_" + name]
+    js_schedules = re.search(r"export const SCHEDULES = Object.freeze\(\{(.*?)\}\);", js).group(1)
+    named = {k.strip().strip('"'): int(v) for k, v in (e.split(":") for e in js_schedules.split(","))}
+    assert named == capi.SCHEDULES
     if not shutil.which("gcc"):
         pytest.skip("no C compiler")
     src = tmp_path / "layout.c"

@@ -89,33 +89,39 @@ def test_firefly_clamp(ctx, pack):
                                               (CASES[5][0], CASES[5][1], 2)],
                          ids=["spheres", "spheres-3spp", "triangles-fog-env", "dispersive", "textured"])
 def test_schedules_are_bit_identical(ctx, pack, config, opts, spp):
-    """The compacting, lane-regenerating and wavefront schedules, and the
-    one-thread-per-pixel pass compiled for five waves per SIMD, compute the
-    same samples, each pixel's in the same order, as one thread per pixel at
-    four: accumulation and guides equal bit for bit."""
+    """The compacting, lane-regenerating, wavefront and split schedules, and
+    the one-thread-per-pixel pass compiled for five waves per SIMD, compute
+    the same samples, each pixel's in the same order, as one thread per pixel
+    at four: accumulation and guides equal bit for bit. The split schedule
+    runs at three split depths and at five waves (at 3 spp a frame, k_main
+    renders for it)."""
     from raymond_amd import capi
     p = pack(config, **opts)
     params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0, samples_per_frame=spp)
     try:
         ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
         plain = gpu_render(ctx, p, 3, params)
-        runs = [(capi.SCHEDULE_PIXEL, 0, 5, 8, 0), (capi.SCHEDULE_PIXEL, 0, 4, 16, 0), (capi.SCHEDULE_PIXEL, 0, 4, 8, 1),
-                (capi.SCHEDULE_COMPACT, 0, 4, 8, 0), (capi.SCHEDULE_SORTED, 0, 4, 8, 0),
-                (capi.SCHEDULE_PERSISTENT, 0, 4, 8, 0), (capi.SCHEDULE_PERSISTENT, 7, 4, 8, 0),
-                (capi.SCHEDULE_WAVEFRONT, 0, 4, 8, 0)]
-        for schedule, blocks, waves, depth, shape in runs:
+        runs = [(capi.SCHEDULE_PIXEL, 0, 5, 8, 0, 0), (capi.SCHEDULE_PIXEL, 0, 4, 16, 0, 0),
+                (capi.SCHEDULE_PIXEL, 0, 4, 8, 1, 0), (capi.SCHEDULE_COMPACT, 0, 4, 8, 0, 0),
+                (capi.SCHEDULE_SORTED, 0, 4, 8, 0, 0), (capi.SCHEDULE_PERSISTENT, 0, 4, 8, 0, 0),
+                (capi.SCHEDULE_PERSISTENT, 7, 4, 8, 0, 0), (capi.SCHEDULE_WAVEFRONT, 0, 4, 8, 0, 0),
+                (capi.SCHEDULE_SPLIT, 0, 4, 8, 0, 1), (capi.SCHEDULE_SPLIT, 0, 4, 8, 0, 2),
+                (capi.SCHEDULE_SPLIT, 0, 4, 8, 0, 3), (capi.SCHEDULE_SPLIT, 0, 5, 8, 0, 2)]
+        for schedule, blocks, waves, depth, shape, split in runs:
             ctx.set_option(capi.OPTION_SCHEDULE, schedule)
             ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, blocks)
             ctx.set_option(capi.OPTION_WAVES, waves)
             ctx.set_option(capi.OPTION_STACK_DEPTH, depth)
             ctx.set_option(capi.OPTION_BLOCK_SHAPE, shape)
+            ctx.set_option(capi.OPTION_SPLIT_DEPTH, split)
             other = gpu_render(ctx, p, 3, params)
             for name, a, b in zip(["accum", "albedo", "normal"], plain, other):
                 differ = int(np.any(a.view(np.uint32) != b.view(np.uint32), axis=-1).sum())
                 assert differ == 0, (f"schedule {schedule} (blocks {blocks}, waves {waves}, stack {depth}, "
-                                     f"shape {shape}): {differ} pixels of {name} differ")
+                                     f"shape {shape}, split depth {split}): {differ} pixels of {name} differ")
     finally:
-        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+        ctx.set_option(capi.OPTION_SPLIT_DEPTH, 0)
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_DEFAULT)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
         ctx.set_option(capi.OPTION_WAVES, 0)
         ctx.set_option(capi.OPTION_STACK_DEPTH, 0)

@@ -111,13 +111,14 @@ def test_compensated_schedules_agree_and_start_mid_run(ctx, pack):
     p = pack("mixed", width=64, height=64)
     params = p.with_params(caustics=0, denoise=1, firefly_clamp=0.0)
     outs = []
-    for schedule in (capi.SCHEDULE_PIXEL, capi.SCHEDULE_COMPACT, capi.SCHEDULE_PERSISTENT, capi.SCHEDULE_WAVEFRONT):
+    for schedule in (capi.SCHEDULE_PIXEL, capi.SCHEDULE_COMPACT, capi.SCHEDULE_PERSISTENT, capi.SCHEDULE_WAVEFRONT,
+                     capi.SCHEDULE_SPLIT):
         ctx.set_option(capi.OPTION_SCHEDULE, schedule)
         try:
             with precision(ctx, capi.PRECISION_COMPENSATED):
                 outs.append(gpu_render(ctx, p, 3, params, denoised=True))
         finally:
-            ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+            ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_DEFAULT)
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             np.testing.assert_array_equal(a, b)

@@ -100,7 +100,7 @@ def test_frames_match_the_fp32_tree(ctx, pack, config, opts):
         try:
             qw = gpu_render(ctx, p, 2, params)
         finally:
-            ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+            ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_DEFAULT)
     for a, b, what in zip(q, plain, ("accum", "guide albedo", "guide normal")):
         differ = int(np.any(a.view(np.uint32) != b.view(np.uint32), axis=-1).sum())
         assert differ == 0, f"{what}: {differ} pixels differ"

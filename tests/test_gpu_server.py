@@ -167,21 +167,25 @@ def test_tune_picks_a_schedule_and_keeps_the_image(ctx, pack):
         result = ctx.tune(scene, params, frames=2)
     finally:
         scene.release()
-    names = [(c[0], c[2], c[3], c[4]) for c in result["candidates"]]
-    assert names == [("per-pixel", 4, 8, "squares"), ("per-pixel", 5, 8, "squares"), ("per-pixel", 4, 16, "squares"),
-                     ("per-pixel", 4, 8, "rows"), ("compact", 4, 8, "squares"), ("sorted", 4, 8, "squares"),
-                     ("persistent", 4, 8, "squares"), ("persistent", 4, 8, "squares"), ("wavefront", 4, 8, "squares")]
+    names = [(c[0], c[2], c[3], c[4], c[5]) for c in result["candidates"]]
+    assert names == [("split", 4, 8, "squares", 2), ("split", 5, 8, "squares", 2), ("split", 5, 8, "squares", 1),
+                     ("per-pixel", 4, 8, "squares", 0), ("per-pixel", 5, 8, "squares", 0),
+                     ("per-pixel", 4, 16, "squares", 0), ("per-pixel", 4, 8, "rows", 0),
+                     ("compact", 4, 8, "squares", 0), ("sorted", 4, 8, "squares", 0),
+                     ("persistent", 4, 8, "squares", 0), ("persistent", 4, 8, "squares", 0),
+                     ("wavefront", 4, 8, "squares", 0)]
     assert all(ms > 0 for *_, ms in result["candidates"])
-    best = min(result["candidates"], key=lambda c: c[5])
-    assert result["best"] == best[:5]
+    best = min(result["candidates"], key=lambda c: c[6])
+    assert result["best"] == best[:6]
     try:
         tuned = alone(ctx, p, params, 3)
     finally:
-        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_DEFAULT)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
         ctx.set_option(capi.OPTION_WAVES, 0)
         ctx.set_option(capi.OPTION_STACK_DEPTH, 0)
         ctx.set_option(capi.OPTION_BLOCK_SHAPE, 0)
+        ctx.set_option(capi.OPTION_SPLIT_DEPTH, 0)
     assert np.array_equal(tuned.view(np.uint32), plain.view(np.uint32))
 
 
@@ -201,16 +205,17 @@ def test_tune_times_whole_frames_with_light_tracing(ctx, pack):
         result = ctx.tune(scene, params, frames=1, emitters=p.emitters)
         assert len(result["candidates"]) == capi.TUNE_CANDIDATES
         assert all(ms > 0 for *_, ms in result["candidates"])
-        assert result["best"] == min(result["candidates"], key=lambda c: c[5])[:5]
+        assert result["best"] == min(result["candidates"], key=lambda c: c[6])[:6]
         ctx.frame(scene, params, p.emitters)
         assert np.isfinite(ctx.read(capi.BUFFER_ACCUM)).all()
     finally:
         scene.release()
-        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_PIXEL)
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_DEFAULT)
         ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
         ctx.set_option(capi.OPTION_WAVES, 0)
         ctx.set_option(capi.OPTION_STACK_DEPTH, 0)
         ctx.set_option(capi.OPTION_BLOCK_SHAPE, 0)
+        ctx.set_option(capi.OPTION_SPLIT_DEPTH, 0)
 
 
 def test_waves_option_is_checked(ctx):
