@@ -256,7 +256,8 @@ def bench_frames(args, name: str) -> dict:
                     p.meta["envTableHeight"])
     ctx.resize(w, h)
     emitters = p.emitters if caustics else None
-    tuning = ctx.tune(scene, frame_params(0), frames=2, emitters=emitters) if tuned else None
+    # Six synchronised frames a candidate: with two, the sphere frame's first candidates time within noise.
+    tuning = ctx.tune(scene, frame_params(0), frames=6, emitters=emitters) if tuned else None
 
     k = 0
     for _ in range(args.warmup):

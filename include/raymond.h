@@ -247,18 +247,20 @@ int rm_reset_stage_times(rm_context* ctx);
  * context starts with RM_SCHEDULE_SPLIT and every other option at 0. */
 int rm_set_option(rm_context* ctx, int option, int64_t value);
 
-/* The autotuner. Renders `frames` whole frames after one warm-up with the
- * caller's parameters — light tracing (given its emitters) and the denoiser
+/* The autotuner. Renders `frames` whole frames after one warm-up, twice (the
+ * candidates in turn, each keeping its faster time), with the caller's
+ * parameters — light tracing (given its emitters) and the denoiser
  * included when they turn them on — under each candidate: split after two
  * bounces at four and at five waves per SIMD, and after one at five; one
  * thread per pixel at four and at five waves, at four with 16 stack entries
  * in LDS and with row-shaped blocks; compacted; sorted; persistent at two
  * grid sizes; wavefront. It sets the fastest as the context's
  * RM_OPTION_SCHEDULE, RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES,
- * RM_OPTION_STACK_DEPTH, RM_OPTION_BLOCK_SHAPE and RM_OPTION_SPLIT_DEPTH. Whole frames, because the path pass shares the GPU
- * with light tracing and the previous frame's denoiser: a path pass that
- * holds more waves can be faster alone and still make the frame slower. The
- * candidates give the same sums bit for bit, so tuning changes speed only.
+ * RM_OPTION_STACK_DEPTH, RM_OPTION_BLOCK_SHAPE and RM_OPTION_SPLIT_DEPTH.
+ * Whole frames, because the path pass shares the GPU with light tracing and
+ * the previous frame's denoiser: a path pass that holds more waves can be
+ * faster alone and still make the frame slower. The candidates give the same
+ * sums bit for bit, so tuning changes speed only.
  * The frames render into the context's buffers: begin the image again
  * (sample offset 0) afterwards. On failure the context keeps the options it
  * had. No counterpart in the reference. */

@@ -1468,7 +1468,11 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
     };
     double best_ms = 0.0;
     int r = RM_OK;
-    for (uint32_t k = 0; k < RM_TUNE_CANDIDATES && r == RM_OK; ++k) {
+    // Two rounds over the candidates, each candidate keeping its faster
+    // round: the first candidate of the first round otherwise pays for the
+    // GPU coming out of idle.
+    for (uint32_t n = 0; n < 2 * RM_TUNE_CANDIDATES && r == RM_OK; ++n) {
+        const uint32_t k = n % RM_TUNE_CANDIDATES;
         c->options[RM_OPTION_SCHEDULE] = schedules[k];
         c->options[RM_OPTION_PERSISTENT_BLOCKS] = blocks[k];
         c->options[RM_OPTION_WAVES] = waves[k];
@@ -1494,13 +1498,14 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         res.stack_depth[k] = depths[k];
         res.block_shape[k] = shapes[k];
         res.split_depth[k] = splits[k];
-        res.ms[k] = ms / frames;
-        res.count = k + 1;
+        res.ms[k] = n < RM_TUNE_CANDIDATES ? ms / frames : std::min(res.ms[k], ms / frames);
+        res.count = std::max(res.count, k + 1);
+    }
+    for (uint32_t k = 0; k < res.count; ++k)
         if (k == 0 || res.ms[k] < best_ms) {
             best_ms = res.ms[k];
             res.best = k;
         }
-    }
     const uint32_t pick = res.best;
     const int64_t picked[TUNED] = {res.schedule[pick],     res.persistent_blocks[pick], res.waves[pick],
                                    res.stack_depth[pick], res.block_shape[pick],       res.split_depth[pick]};
