@@ -397,6 +397,7 @@ int rm_open(int device, rm_context** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->prepared_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->denoised_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side_ev, hipEventDisableTiming);
     c->owner.device = device;
     c->owner.streams[0] = c->stream;
     c->owner.streams[1] = c->side;
@@ -404,6 +405,7 @@ int rm_open(int device, rm_context** out) {
     for (DevBuf* b : working_buffers(c)) b->owner = &c->owner;
     for (DevBuf* b : {&c->env, &c->env_table, &c->work}) b->owner = &c->owner;
     if (e != hipSuccess) {
+        if (c->side_ev) (void)hipEventDestroy(c->side_ev);
         if (c->batch_ev) (void)hipEventDestroy(c->batch_ev);
         if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
         if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
@@ -494,6 +496,7 @@ int rm_close(rm_context* c) {
             (void)hipEventDestroy(p.second);
         }
     for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+    if (c->side_ev) (void)hipEventDestroy(c->side_ev);
     if (c->batch_ev) (void)hipEventDestroy(c->batch_ev);
     if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
     if (c->prepared_ev) (void)hipEventDestroy(c->prepared_ev);
@@ -944,9 +947,35 @@ static rm::PhotonMap photon_map(rm_context* c, uint32_t capacity) {
     return M;
 }
 
+// A frame's light tracing as one job of a batched launch (rm_frame_batch).
+static rm::LightJob light_job(rm_context* c, const Frame& F) {
+    return rm::LightJob{F.S, F.P, c->emitters.as<const float4>(), c->splats[F.pair].as<uint32_t>(),
+                        photon_map(c, F.capacity), c->side_spill.as<uint32_t>(), F.side_slots, F.photons};
+}
+
+// After light tracing, on the side stream: the photon map's sort and the
+// gather, then the join the path stream's resolve waits for.
+static int light_end(rm_context* c, const Frame& F) {
+    const rm::PhotonMap M = photon_map(c, F.capacity);
+    const rm::GatherQueue Q{c->gather_queue.as<float4>(),
+                            reinterpret_cast<uint32_t*>(c->gather_queue.as<char>() + gather_queue_bytes(F.px) - 256),
+                            F.px};
+    {
+        Timed t(c, RM_STAGE_GATHER, c->side);
+        rm::launch_photon_sort(M, c->side);
+        rm::launch_photon_gather(F.S, F.P, M, Q, c->gathered[F.pair].as<float4>(), c->side_spill.as<uint32_t>(),
+                                 F.side_slots, F.f, c->side);
+    }
+    RM_HIP(hipEventRecord(c->join, c->side), "joining the side stream");
+    RM_HIP(hipGetLastError(), "launching light tracing");
+    return RM_OK;
+}
+
 // Everything before the path pass: light tracing and the photon gather on
-// the side stream, the compensation terms.
-static int frame_begin(rm_context* c, Frame& F) {
+// the side stream, the compensation terms. batched_light: stop after the
+// cell counts' clear; rm_frame_batch traces the batch's light paths in one
+// launch, then runs light_end for each frame.
+static int frame_begin(rm_context* c, Frame& F, bool batched_light = false) {
     const rm::Params& P = F.P;
     const rm::DevScene& S = F.S;
     const rm::Features& f = F.f;
@@ -969,10 +998,6 @@ static int frame_begin(rm_context* c, Frame& F) {
         const uint32_t pair = F.pair;
         c->pair = pair ^ 1u;
         const rm::PhotonMap M = photon_map(c, F.capacity);
-        const rm::GatherQueue Q{c->gather_queue.as<float4>(),
-                                reinterpret_cast<uint32_t*>(c->gather_queue.as<char>() + gather_queue_bytes(F.px) - 256),
-                                F.px};
-        float4* gathered = c->gathered[pair].as<float4>();
         RM_HIP(hipStreamWaitEvent(c->side, c->resolved_ev[pair], 0), "ordering light tracing");
         if (c->denoised_pair_live[pair])
             RM_HIP(hipStreamWaitEvent(c->side, c->denoised_pair_ev[pair], 0), "ordering light tracing");
@@ -980,18 +1005,15 @@ static int frame_begin(rm_context* c, Frame& F) {
             Timed t(c, RM_STAGE_CLEAR, c->side);
             rm::launch_fill_u32(M.counts, 0u, rm::PHOTON_TABLE + 1, c->side);
         }
-        {
-            Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
-            rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(), M,
-                                   c->side_spill.as<uint32_t>(), F.side_slots, F.photons, f, c->side);
+        if (!batched_light) {
+            {
+                Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
+                rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(), M,
+                                       c->side_spill.as<uint32_t>(), F.side_slots, F.photons, f, c->side);
+            }
+            int r = light_end(c, F);
+            if (r != RM_OK) return r;
         }
-        {
-            Timed t(c, RM_STAGE_GATHER, c->side);
-            rm::launch_photon_sort(M, c->side);
-            rm::launch_photon_gather(S, P, M, Q, gathered, c->side_spill.as<uint32_t>(), F.side_slots, f, c->side);
-        }
-        RM_HIP(hipEventRecord(c->join, c->side), "joining the side stream");
-        RM_HIP(hipGetLastError(), "launching light tracing");
     }
 
     // Compensated sums: the error terms start at zero with the image (the
@@ -1056,8 +1078,29 @@ static int frame_path(rm_context* c, Frame& F) {
     return RM_OK;
 }
 
-// Everything after the path pass: the splat and gather resolve, the denoiser.
-static int frame_end(rm_context* c, Frame& F) {
+// The denoiser's passes are done on the post stream (they were enqueued
+// there, or the post stream waits for a batch that ran them): what later
+// frames and reads wait for.
+static int denoise_recorded(rm_context* c, const Frame& F) {
+    RM_HIP(hipEventRecord(c->denoised_ev, c->post), "ordering the denoiser");
+    if (F.caustics) {
+        RM_HIP(hipEventRecord(c->denoised_pair_ev[F.pair], c->post), "ordering light tracing");
+        c->denoised_pair_live[F.pair] = true;
+    }
+    c->denoise_pending = true;
+    return RM_OK;
+}
+
+// A frame's denoiser passes as one job of a batched launch (rm_frame_batch).
+static rm::DenoiseJob denoise_job(rm_context* c, const Frame& F) {
+    return rm::DenoiseJob{F.P, c->guides.ptr, c->light_a.as<float4>(), c->light_b.as<float4>(),
+                          c->albedo_frame.as<const float4>(), c->denoised.as<float4>()};
+}
+
+// Everything after the path pass: the splat and gather resolve, the
+// denoiser. batched_denoise: stop once prepare is enqueued; rm_frame_batch
+// runs the batch's à-trous passes and compose, then denoise_recorded.
+static int frame_end(rm_context* c, Frame& F, bool batched_denoise = false) {
     const rm::Params& P = F.P;
     const bool caustics = F.caustics;
     float4* comp = F.comp;
@@ -1098,25 +1141,23 @@ static int frame_end(rm_context* c, Frame& F) {
             rm::launch_prepare(P, sums[0], sums[1], c->guides.ptr, fp16, a, c->albedo_frame.as<float4>(), c->stream);
         }
         RM_HIP(hipEventRecord(c->prepared_ev, c->stream), "ordering the denoiser");
-        RM_HIP(hipStreamWaitEvent(c->post, c->prepared_ev, 0), "ordering the denoiser");
-        const int steps[5] = {1, 2, 4, 8, 16};
-        float4* in = a;
-        float4* out = b;
-        for (int k = 0; k < 5; ++k) {
-            Timed t(c, RM_STAGE_ATROUS, c->post);
-            rm::launch_atrous(P, steps[k], c->guides.ptr, fp16, in, out, c->post);
-            std::swap(in, out);
+        if (!batched_denoise) {
+            RM_HIP(hipStreamWaitEvent(c->post, c->prepared_ev, 0), "ordering the denoiser");
+            const int steps[5] = {1, 2, 4, 8, 16};
+            float4* in = a;
+            float4* out = b;
+            for (int k = 0; k < 5; ++k) {
+                Timed t(c, RM_STAGE_ATROUS, c->post);
+                rm::launch_atrous(P, steps[k], c->guides.ptr, fp16, in, out, c->post);
+                std::swap(in, out);
+            }
+            {
+                Timed t(c, RM_STAGE_COMPOSE, c->post);
+                rm::launch_compose(P, c->albedo_frame.as<const float4>(), in, c->denoised.as<float4>(), c->post);
+            }
+            int r = denoise_recorded(c, F);
+            if (r != RM_OK) return r;
         }
-        {
-            Timed t(c, RM_STAGE_COMPOSE, c->post);
-            rm::launch_compose(P, c->albedo_frame.as<const float4>(), in, c->denoised.as<float4>(), c->post);
-        }
-        RM_HIP(hipEventRecord(c->denoised_ev, c->post), "ordering the denoiser");
-        if (caustics) {
-            RM_HIP(hipEventRecord(c->denoised_pair_ev[F.pair], c->post), "ordering light tracing");
-            c->denoised_pair_live[F.pair] = true;
-        }
-        c->denoise_pending = true;
         RM_HIP(hipGetLastError(), "launching the denoiser");
     }
     c->last_params = P;
@@ -1259,9 +1300,54 @@ int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const
                             wavefront);
         if (r != RM_OK) return r;
     }
+    // Light tracing: each frame with it enqueues its side stream's waits and
+    // the clear of its photon counts; then one launch traces every such
+    // frame's light paths, on the side stream of the first once the others'
+    // side streams have reached it, and each frame's sort and gather follow
+    // on its own side stream.
     for (uint32_t i = 0; i < n; ++i) {
-        int r = frame_begin(ctxs[i], F[i]);
+        int r = frame_begin(ctxs[i], F[i], true);
         if (r != RM_OK) return r;
+    }
+    {
+        rm::LightTable L;
+        std::memset(&L, 0, sizeof L);
+        rm::Features lu{};
+        rm_context* lead = nullptr;
+        uint32_t blocks = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!F[i].caustics) continue;
+            if (!lead) {
+                lead = ctxs[i];
+                lu = F[i].f;
+            }
+            lu.textures |= F[i].f.textures;
+            lu.dispersion |= F[i].f.dispersion;
+            L.job[L.n] = light_job(ctxs[i], F[i]);
+            L.begin[L.n++] = blocks;
+            blocks += rm::light_blocks(F[i].photons);
+        }
+        if (lead) {
+            L.begin[L.n] = blocks;
+            for (uint32_t i = 0; i < n; ++i)
+                if (F[i].caustics && ctxs[i] != lead) {
+                    RM_HIP(hipEventRecord(ctxs[i]->side_ev, ctxs[i]->side), "ordering a batch's light tracing");
+                    RM_HIP(hipStreamWaitEvent(lead->side, ctxs[i]->side_ev, 0), "ordering a batch's light tracing");
+                }
+            {
+                Timed t(lead, RM_STAGE_LIGHT_TRACE, lead->side);
+                rm::launch_light_trace_batch(L, lu, lead->side);
+            }
+            RM_HIP(hipGetLastError(), "launching the batch's light tracing");
+            RM_HIP(hipEventRecord(lead->side_ev, lead->side), "ordering a batch's light tracing");
+            for (uint32_t i = 0; i < n; ++i) {
+                if (!F[i].caustics) continue;
+                if (ctxs[i] != lead)
+                    RM_HIP(hipStreamWaitEvent(ctxs[i]->side, lead->side_ev, 0), "ordering a batch's light tracing");
+                int r = light_end(ctxs[i], F[i]);
+                if (r != RM_OK) return r;
+            }
+        }
     }
     // One launch for all the path passes, on the first context's stream once
     // every context has queued what comes before; each context's stream then
@@ -1301,9 +1387,55 @@ int rm_frame_batch(rm_context* const* ctxs, const rm_scene* const* scenes, const
     RM_HIP(hipEventRecord(run->batch_ev, run->stream), "ordering a batch");
     for (uint32_t i = 1; i < n; ++i)
         RM_HIP(hipStreamWaitEvent(ctxs[i]->stream, run->batch_ev, 0), "ordering a batch");
+    // The denoiser: with two or more frames denoising (their guides of one
+    // precision), each enqueues its resolve and prepare on its own stream,
+    // then one launch per à-trous pass and one for compose run all of them on
+    // the first one's post stream once every prepare is done; the others'
+    // post streams wait for that before they record their frame denoised.
+    uint32_t denoising = 0;
+    bool fp16 = false, mixed = false;
     for (uint32_t i = 0; i < n; ++i) {
-        int r = frame_end(ctxs[i], F[i]);
+        if (!F[i].P.denoise) continue;
+        const bool h = (ctxs[i]->options[RM_OPTION_PRECISION] & RM_PRECISION_FP16_GUIDES) != 0;
+        mixed = mixed || (denoising > 0 && h != fp16);
+        fp16 = h;
+        ++denoising;
+    }
+    const bool batched_denoise = denoising >= 2 && !mixed;
+    for (uint32_t i = 0; i < n; ++i) {
+        int r = frame_end(ctxs[i], F[i], batched_denoise);
         if (r != RM_OK) return r;
+    }
+    if (batched_denoise) {
+        rm::DenoiseTable D;
+        std::memset(&D, 0, sizeof D);
+        rm_context* lead = nullptr;
+        uint32_t lead_frame = 0, blocks = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!F[i].P.denoise) continue;
+            if (!lead) {
+                lead = ctxs[i];
+                lead_frame = i;
+            }
+            RM_HIP(hipStreamWaitEvent(lead->post, ctxs[i]->prepared_ev, 0), "ordering a batch's denoiser");
+            D.job[D.n] = denoise_job(ctxs[i], F[i]);
+            D.begin[D.n++] = blocks;
+            blocks += rm::stencil_blocks(F[i].P.width, F[i].P.height);
+        }
+        D.begin[D.n] = blocks;
+        {
+            Timed t(lead, RM_STAGE_ATROUS, lead->post);
+            rm::launch_denoise_batch(D, fp16, lead->post);
+        }
+        RM_HIP(hipGetLastError(), "launching the batch's denoiser");
+        int r = denoise_recorded(lead, F[lead_frame]);
+        if (r != RM_OK) return r;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!F[i].P.denoise || ctxs[i] == lead) continue;
+            RM_HIP(hipStreamWaitEvent(ctxs[i]->post, lead->denoised_ev, 0), "ordering a batch's denoiser");
+            r = denoise_recorded(ctxs[i], F[i]);
+            if (r != RM_OK) return r;
+        }
     }
     return RM_OK;
 }

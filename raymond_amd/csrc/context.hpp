@@ -123,6 +123,7 @@ struct rm_context {
     hipStream_t post = nullptr;
     hipEvent_t prepared_ev = nullptr, denoised_ev = nullptr;
     hipEvent_t batch_ev = nullptr;   // orders a batched path pass (rm_frame_batch) with this context's stream
+    hipEvent_t side_ev = nullptr;    // orders batched light tracing (rm_frame_batch) with this context's side stream
     bool denoise_pending = false;
     uint32_t width = 0, height = 0, capacity = 0;
     rmi::DevBuf accum, galb, gnorm, light_a, light_b, denoised, guides, albedo_frame;

@@ -249,11 +249,9 @@ RM_D float4 atrous_pixel(int x, int y, int W, int H, int step, float4 centre, fl
 // and 16: a wave's tap is one coalesced load, and the neighbouring
 // workgroups of an XCD read the same lines.
 template <class G>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const typename G::T* __restrict__ guides,
-                                                          const float4* __restrict__ light_in,
-                                                          float4* __restrict__ light_out) {
-    uint32_t ux, uy;
-    if (!stencil_pixel(P.width, P.height, ux, uy)) return;
+RM_D void atrous_direct(const Params& P, int step, const typename G::T* __restrict__ guides,
+                        const float4* __restrict__ light_in, float4* __restrict__ light_out, uint32_t ux,
+                        uint32_t uy) {
     const int x = (int)ux, y = (int)uy;
     const uint32_t W = P.width;
     const uint32_t i = uy * W + ux;
@@ -264,6 +262,39 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, co
     };
     auto var = [&](int dx, int dy) { return light_in[(uint32_t)(y + dy) * W + (uint32_t)(x + dx)].w; };
     light_out[i] = atrous_pixel(x, y, (int)P.width, (int)P.height, step, light_in[i], G::get(guides, i), tap, var);
+}
+template <class G>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous(Params P, int step, const typename G::T* __restrict__ guides,
+                                                          const float4* __restrict__ light_in,
+                                                          float4* __restrict__ light_out) {
+    uint32_t ux, uy;
+    if (stencil_pixel(P.width, P.height, ux, uy)) atrous_direct<G>(P, step, guides, light_in, light_out, ux, uy);
+}
+
+// Several jobs' à-trous pass at `step` in one launch (rm_frame_batch): the
+// grid is each job's stencil grid, one job after another, and each block
+// runs k_atrous's block of the same index for its job, reading its taps
+// directly rather than from LDS. STEP is the step as a compile-time
+// constant (0: the `step` argument), as in the single-job kernels: the
+// staged ones (steps 1, 2 and 4) know their step at compile time and
+// k_atrous (8 and 16) doesn't, and the multiply-adds the compiler fuses in
+// the weights differ between the two, so matching them keeps every pixel
+// the same bit for bit. A job reads its a and writes its b when `parity` is
+// 0, the other way round when 1. The job table is the kernel argument, read
+// by scalar loads.
+RM_D uint32_t denoise_job(const DenoiseTable& T) {
+    uint32_t j = 0;
+    while (j + 1u < T.n && blockIdx.x >= T.begin[j + 1u]) ++j;
+    return j;
+}
+template <class G, int STEP>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_batch(DenoiseTable T, int step, int parity) {
+    const uint32_t j = denoise_job(T);
+    const DenoiseJob& J = T.job[j];
+    uint32_t ux, uy;
+    if (!stencil_pixel_at(J.P.width, J.P.height, blockIdx.x - T.begin[j], ux, uy)) return;
+    atrous_direct<G>(J.P, STEP > 0 ? STEP : step, static_cast<const typename G::T*>(J.guides), parity ? J.b : J.a,
+                     parity ? J.a : J.b, ux, uy);
 }
 
 // The staged à-trous pass, for steps 1, 2 and 4. At step s the taps of
@@ -355,15 +386,27 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_atrous_staged(Params P, Atrou
 }
 
 // src/shaders.mjs:2088-2098, with the albedo k_prepare recorded.
-__global__ __launch_bounds__(BLOCK_THREADS) void k_compose(Params P, const float4* __restrict__ albedo,
-                                                           const float4* __restrict__ light_in, float4* out) {
-    uint32_t x, y;
-    tile_pixel(x, y);
-    if (x >= P.width || y >= P.height) return;
+RM_D void compose_pixel(const Params& P, const float4* __restrict__ albedo, const float4* __restrict__ light_in,
+                        float4* out, uint32_t x, uint32_t y) {
     const uint32_t i = y * P.width + x;
     float4 l = light_in[i], a = albedo[i];
     V3 r = v3(l.x, l.y, l.z) * v3(a.x, a.y, a.z);
     out[i] = make_float4(r.x, r.y, r.z, 0.0f);
+}
+__global__ __launch_bounds__(BLOCK_THREADS) void k_compose(Params P, const float4* __restrict__ albedo,
+                                                           const float4* __restrict__ light_in, float4* out) {
+    uint32_t x, y;
+    tile_pixel(x, y);
+    if (x < P.width && y < P.height) compose_pixel(P, albedo, light_in, out, x, y);
+}
+// The jobs' compose in one launch, over the à-trous batch's grid; each job's
+// light is in its b (the fifth pass's output).
+__global__ __launch_bounds__(BLOCK_THREADS) void k_compose_batch(DenoiseTable T) {
+    const uint32_t j = denoise_job(T);
+    const DenoiseJob& J = T.job[j];
+    uint32_t x, y;
+    if (stencil_pixel_at(J.P.width, J.P.height, blockIdx.x - T.begin[j], x, y))
+        compose_pixel(J.P, J.albedo, J.b, J.out, x, y);
 }
 
 // DISPLAY_SHADER's fragment (src/shaders.mjs:2100-2152) for every pixel at
@@ -509,6 +552,23 @@ void launch_display(const Params& P, const float4* accum, const float4* denoised
 void launch_compose(const Params& P, const float4* albedo, const float4* in, float4* out, hipStream_t stream) {
     dim3 grid((P.width + 15) / 16, (P.height + 15) / 16);
     hipLaunchKernelGGL(k_compose, grid, dim3(BLOCK_THREADS), 0, stream, P, albedo, in, out);
+}
+
+template <class G>
+static void launch_atrous_batch(const DenoiseTable& D, dim3 grid, hipStream_t stream) {
+    hipLaunchKernelGGL((k_atrous_batch<G, 1>), grid, dim3(BLOCK_THREADS), 0, stream, D, 1, 0);
+    hipLaunchKernelGGL((k_atrous_batch<G, 2>), grid, dim3(BLOCK_THREADS), 0, stream, D, 2, 1);
+    hipLaunchKernelGGL((k_atrous_batch<G, 4>), grid, dim3(BLOCK_THREADS), 0, stream, D, 4, 0);
+    hipLaunchKernelGGL((k_atrous_batch<G, 0>), grid, dim3(BLOCK_THREADS), 0, stream, D, 8, 1);
+    hipLaunchKernelGGL((k_atrous_batch<G, 0>), grid, dim3(BLOCK_THREADS), 0, stream, D, 16, 0);
+}
+
+void launch_denoise_batch(const DenoiseTable& D, bool fp16, hipStream_t stream) {
+    const dim3 grid(D.begin[D.n]);
+    if (grid.x == 0) return;
+    if (fp16) launch_atrous_batch<GuidesF16>(D, grid, stream);
+    else launch_atrous_batch<GuidesF32>(D, grid, stream);
+    hipLaunchKernelGGL(k_compose_batch, grid, dim3(BLOCK_THREADS), 0, stream, D);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream) {

@@ -47,6 +47,14 @@ struct LaunchMain {
     uint32_t split_depth = 2;    // bounces the split schedule's head kernel runs before queueing a path
     uint32_t split_blocks = 0;   // the tail kernel's persistent grid
 };
+// Workgroups go to the XCDS XCDs round-robin by linear block id. A stencil
+// pass's grid (stencil_pixel) has its 16×16 tiles rounded up to a multiple
+// of XCDS blocks.
+constexpr uint32_t XCDS = 8u;
+__host__ __device__ inline uint32_t stencil_blocks(uint32_t width, uint32_t height) {
+    uint32_t tiles = ((width + 15u) / 16u) * ((height + 15u) / 16u);
+    return (tiles + XCDS - 1u) / XCDS * XCDS;
+}
 // The split schedule's buffer for px pixels: the queue (SLOT_WORDS words a
 // path, the path slots' layout), then two counters at this word offset.
 __host__ __device__ inline size_t split_count_offset(size_t px) { return (px * 27 + 63) / 64 * 64; }
@@ -200,6 +208,32 @@ constexpr uint32_t LIGHT_LANES = 1u << 20;
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
                         const PhotonMap& M, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
                         hipStream_t stream);
+// Blocks of light tracing's grid for `count` light paths: one path a thread
+// up to LIGHT_LANES, then each thread takes several.
+inline uint32_t light_blocks(uint32_t count) {
+    const uint32_t lanes = count < LIGHT_LANES ? count : LIGHT_LANES;
+    return (lanes + BLOCK_THREADS - 1) / BLOCK_THREADS;
+}
+// Several jobs' light tracing in one launch (rm_frame_batch): job j's blocks
+// are [begin[j], begin[j + 1]) of the grid, light_blocks(count) of them.
+struct LightJob {
+    DevScene S;
+    Params P;
+    const float4* E;      // the emitters
+    uint32_t* splats;
+    PhotonMap M;          // its counts cleared
+    uint32_t* spill;
+    uint32_t slots;
+    uint32_t count;       // light paths
+};
+struct LightTable {
+    LightJob job[MAX_BATCH];
+    uint32_t begin[MAX_BATCH + 1];
+    uint32_t n;
+};
+static_assert(sizeof(LightTable) <= 4096, "the job table travels as the kernel argument");
+// f: the union of the jobs' features (the tree the same for all).
+void launch_light_trace_batch(const LightTable& L, const Features& f, hipStream_t stream);
 // Counting sort of the deposited photons by cell: an exclusive scan of the
 // cell counts, then each photon copied to its cell's run at its rank.
 void launch_photon_sort(const PhotonMap& M, hipStream_t stream);
@@ -234,6 +268,25 @@ void launch_prepare(const Params& P, const float4* accum, const float4* galb, co
 void launch_atrous(const Params& P, int step, const void* guides, bool fp16, const float4* in, float4* out,
                    hipStream_t stream);
 void launch_compose(const Params& P, const float4* albedo, const float4* in, float4* out, hipStream_t stream);
+// Several jobs' denoiser passes in one launch each (rm_frame_batch): the five
+// à-trous passes and compose after each job's prepare. Job j's blocks are
+// [begin[j], begin[j + 1]) of each grid, stencil_blocks(width, height) of
+// them. The passes go a → b → a → b → a → b, so compose reads b.
+struct DenoiseJob {
+    Params P;
+    const void* guides;    // FP32 or FP16 guides, the same for every job of a table
+    float4* a;             // prepare's output
+    float4* b;
+    const float4* albedo;  // prepare's albedo
+    float4* out;           // the denoised image
+};
+struct DenoiseTable {
+    DenoiseJob job[MAX_BATCH];
+    uint32_t begin[MAX_BATCH + 1];
+    uint32_t n;
+};
+static_assert(sizeof(DenoiseTable) <= 4096, "the job table travels as the kernel argument");
+void launch_denoise_batch(const DenoiseTable& D, bool fp16, hipStream_t stream);
 void launch_display(const Params& P, const float4* accum, const float4* denoised, uchar4* out, hipStream_t stream);
 void launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 

@@ -73,17 +73,18 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
 // dropped, as the reference drops them past its fixed capacity. A deposit
 // counts its photon in its cell (the rank it gets orders the cell's photons
 // for the sort) where the reference links it into the cell's list.
-template <bool TEXTURES, bool DISPERSION, int TREE>
-__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_light_trace(DevScene S, Params P, const float4* __restrict__ E,
-                                                               uint32_t* splats, PhotonMap M, uint32_t* spill,
-                                                               uint32_t slots, uint32_t photons_n) {
-    __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+//
+// light_paths is one block's share, block `block` of a grid of `blocks`;
+// k_light_trace runs a frame's grid, k_light_trace_batch several jobs' grids
+// in one launch.
+template <bool TEXTURES, bool DISPERSION, class STACK>
+RM_D void light_paths(const DevScene& S, const Params& P, const float4* __restrict__ E, uint32_t* splats,
+                      const PhotonMap& M, STACK& stack, uint32_t photons_n, uint32_t block, uint32_t blocks) {
+    constexpr int TREE = STACK::tree;
     const Ctx C{S, &P};
-    const uint32_t lane = blockIdx.x * BLOCK_THREADS + threadIdx.x;
-    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, lane, slots);
     // `first` is the same for every thread of a block, so whole waves take
     // each pass and reach the deposit's ballot together.
-    for (uint32_t first = blockIdx.x * BLOCK_THREADS; first < photons_n; first += gridDim.x * BLOCK_THREADS) {
+    for (uint32_t first = block * BLOCK_THREADS; first < photons_n; first += blocks * BLOCK_THREADS) {
         const uint32_t id = first + threadIdx.x;
         bool pending = false;
         V3 dep_p = {0, 0, 0}, dep_d = {0, 0, 0}, dep_beta = {0, 0, 0};
@@ -248,6 +249,34 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
         M.deposits[(size_t)i * 3 + 1] = make_float4(dep_beta.x, dep_beta.y, dep_beta.z, __uint_as_float(rank));
         M.deposits[(size_t)i * 3 + 2] = make_float4(-dep_d.x, -dep_d.y, -dep_d.z, 0.0f);
     }
+}
+
+template <bool TEXTURES, bool DISPERSION, int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_light_trace(
+    DevScene S, Params P, const float4* __restrict__ E, uint32_t* splats, PhotonMap M, uint32_t* spill,
+    uint32_t slots, uint32_t photons_n) {
+    __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, blockIdx.x * BLOCK_THREADS + threadIdx.x, slots);
+    light_paths<TEXTURES, DISPERSION>(S, P, E, splats, M, stack, photons_n, blockIdx.x, gridDim.x);
+}
+
+// Several jobs' light tracing in one launch (rm_frame_batch): the grid is
+// each job's light-tracing grid, one job after another, and each block runs
+// its share of its job's light paths exactly as k_light_trace's block of the
+// same index would (the same photon ids, spill lanes and deposits). The job
+// table is the kernel argument, read by scalar loads. The kernel is built
+// for the union of the jobs' features: a job's paths come out the same in a
+// kernel with more of them.
+template <bool TEXTURES, bool DISPERSION, int TREE>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_light_trace_batch(
+    LightTable T) {
+    __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
+    uint32_t j = 0;
+    while (j + 1u < T.n && blockIdx.x >= T.begin[j + 1u]) ++j;
+    const LightJob& J = T.job[j];
+    const uint32_t block = blockIdx.x - T.begin[j], blocks = T.begin[j + 1u] - T.begin[j];
+    auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, J.spill, block * BLOCK_THREADS + threadIdx.x, J.slots);
+    light_paths<TEXTURES, DISPERSION>(J.S, J.P, J.E, J.splats, J.M, stack, J.count, block, blocks);
 }
 
 // ------------------------------------------------------------------ photon sort
@@ -677,9 +706,7 @@ static void launch_light_trace_v(const DevScene& S, const Params& P, const float
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
                         const PhotonMap& M, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
                         hipStream_t stream) {
-    // One photon a thread up to LIGHT_LANES, then each thread takes several.
-    const uint32_t lanes = count < LIGHT_LANES ? count : LIGHT_LANES;
-    dim3 grid((lanes + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    const dim3 grid(light_blocks(count));
     if (grid.x == 0) return;
     if (f.textures)
         launch_light_trace_v<true, true>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
@@ -687,6 +714,21 @@ void launch_light_trace(const DevScene& S, const Params& P, const float4* emitte
         launch_light_trace_v<false, true>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
     else
         launch_light_trace_v<false, false>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
+}
+
+template <bool T, bool D>
+static void launch_light_trace_batch_v(const LightTable& L, int tree, hipStream_t stream) {
+    with_tree(tree, [&](auto t) {
+        hipLaunchKernelGGL((k_light_trace_batch<T, D, decltype(t)::value>), dim3(L.begin[L.n]), dim3(BLOCK_THREADS), 0,
+                           stream, L);
+    });
+}
+
+void launch_light_trace_batch(const LightTable& L, const Features& f, hipStream_t stream) {
+    if (L.begin[L.n] == 0) return;
+    if (f.textures) launch_light_trace_batch_v<true, true>(L, f.tree, stream);
+    else if (f.dispersion) launch_light_trace_batch_v<false, true>(L, f.tree, stream);
+    else launch_light_trace_batch_v<false, false>(L, f.tree, stream);
 }
 
 void launch_photon_sort(const PhotonMap& M, hipStream_t stream) {

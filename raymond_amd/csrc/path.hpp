@@ -60,17 +60,14 @@ inline dim3 shaped_grid(uint32_t width, uint32_t height) {
 // STRIP_TILES wide (row-major inside a strip), and each XCD takes one
 // contiguous run of that order: its in-flight blocks then share a compact
 // footprint that fits its L2. Launch with stencil_blocks() blocks.
-constexpr uint32_t XCDS = 8u;
 constexpr uint32_t STRIP_TILES = 16u;
-__host__ __device__ inline uint32_t stencil_blocks(uint32_t width, uint32_t height) {
-    uint32_t tiles = ((width + 15u) / 16u) * ((height + 15u) / 16u);
-    return (tiles + XCDS - 1u) / XCDS * XCDS;
-}
-RM_D bool stencil_pixel(uint32_t width, uint32_t height, uint32_t& x, uint32_t& y) {
+// stencil_pixel_at takes the block's index within such a grid (a grid that
+// starts on a multiple of XCDS blocks keeps each XCD's run).
+RM_D bool stencil_pixel_at(uint32_t width, uint32_t height, uint32_t block, uint32_t& x, uint32_t& y) {
     const uint32_t tiles_x = (width + 15u) / 16u, tiles_y = (height + 15u) / 16u;
     const uint32_t tiles = tiles_x * tiles_y;
     const uint32_t per_xcd = (tiles + XCDS - 1u) / XCDS;
-    const uint32_t t = (blockIdx.x % XCDS) * per_xcd + blockIdx.x / XCDS;
+    const uint32_t t = (block % XCDS) * per_xcd + block / XCDS;
     if (t >= tiles) return false;
     const uint32_t strip = t / (STRIP_TILES * tiles_y);
     const uint32_t r = t - strip * STRIP_TILES * tiles_y;
@@ -80,6 +77,9 @@ RM_D bool stencil_pixel(uint32_t width, uint32_t height, uint32_t& x, uint32_t& 
     x = tx * 16u + (wave & 1u) * 8u + (l & 7u);
     y = ty * 16u + (wave >> 1) * 8u + (l >> 3);
     return x < width && y < height;
+}
+RM_D bool stencil_pixel(uint32_t width, uint32_t height, uint32_t& x, uint32_t& y) {
+    return stencil_pixel_at(width, height, blockIdx.x, x, y);
 }
 
 template <int LDS_DEPTH, int TREE = TREE_FP32>

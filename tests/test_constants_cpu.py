@@ -76,11 +76,11 @@ ORACLE_NAMES = {"aces": ["aces1"], "to_srgb": ["srgb1"], "atrous": ["oracle_atro
                 "photon_gather": ["photon_gather_pixel"], "resolve_splats": ["oracle_resolve_splats"],
                 "qmc_next": ["next"]}
 KERNEL_NAMES = {"aces": ["aces1"], "to_srgb": ["srgb1"], "atrous": ["atrous_pixel"], "prepare": ["k_prepare"],
-                "compose": ["k_compose"], "main": ["main_pixel", "path_begin"], "light_trace": ["k_light_trace"],
+                "compose": ["compose_pixel"], "main": ["main_pixel", "path_begin"], "light_trace": ["light_paths"],
                 "photon_gather": ["k_gather_begin", "gather_bounce"], "gather": ["gather_lane", "gather_cell", "photon_term"],
                 "resolve_splats": ["resolve_add"], "qmc_next": ["next"],
                 "finite": ["finite3"], "radiance": ["path_step", "path_shade", "path_begin"],
-                "deposit": ["k_light_trace"], "dispersed": ["path_shade", "dispersed_ior"],
+                "deposit": ["light_paths"], "dispersed": ["path_shade", "dispersed_ior"],
                 "env_choice": ["light_choices", "env_choice"], "sun_choice": ["light_choices", "sun_choice"],
                 "occluded": ["occluded", "shadow"]}
 

@@ -2,11 +2,15 @@
 RM_OPTION_BATCH): several contexts' path passes in one launch of the
 per-pixel kernel, or, with every context on the wavefront schedule, each
 wavefront stage one launch for all the jobs, the launch sequence replayed
-from a hipGraph. Each job's frame is computed by the same code on the same
-inputs as alone, so its sums are the same bit for bit, even where the batch's
-kernel carries features (volumes, environment, textures) the job lacks; the
-wavefront batch is checked after every frame, so every replay of its graph is
-compared with frames rendered alone, with the program's own launches and
+from a hipGraph; the light tracing of the frames that have it in one launch;
+the denoiser's à-trous passes and compose of the frames that denoise in one
+launch each. Each job's frame is computed by the same code on the same
+inputs as alone, so its sums and denoised image are the same bit for bit,
+even where the batch's kernel carries features (volumes, environment,
+textures) the job lacks; the light-traced jobs agree to the light tests'
+tolerance (photons land in a cell in the order their atomics do). The
+wavefront batch is checked after every frame, so every replay of its graph
+is compared with frames rendered alone, with the program's own launches and
 read-backs between the replays."""
 import numpy as np
 import pytest
@@ -23,6 +27,7 @@ JOBS = [
     ("cornell", {"width": 64, "height": 64}, dict(caustics=0, denoise=1, samples_per_frame=2)),
     ("fixture", {"scene": {"name": "textured", "size": 48}}, dict(caustics=0, denoise=0)),
     ("fixture", {"scene": {"name": "causticLamp", "size": 48}}, dict(caustics=1, denoise=0)),
+    ("fixture", {"scene": {"name": "causticLamp", "size": 40}}, dict(caustics=1, denoise=1)),
 ]
 FRAMES = 3
 
