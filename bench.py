@@ -256,8 +256,7 @@ def bench_frames(args, name: str) -> dict:
                     p.meta["envTableHeight"])
     ctx.resize(w, h)
     emitters = p.emitters if caustics else None
-    # Six synchronised frames a candidate: with two, the sphere frame's first candidates time within noise.
-    tuning = ctx.tune(scene, frame_params(0), frames=6, emitters=emitters) if tuned else None
+    tuning = ctx.tune(scene, frame_params(0), frames=args.tune_frames, emitters=emitters) if tuned else None
 
     k = 0
     for _ in range(args.warmup):
@@ -417,16 +416,18 @@ def precision_flags(names: str) -> int:
 
 
 def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotune: int = 0, batch: bool = False,
-              schedule: int = 0) -> dict:
+              schedule: int | None = None) -> dict:
     """Submits the trace's jobs at their arrival times to a server with `slots`
-    slots and RM_OPTION_PRECISION flags `precision`; returns throughput and
-    latency percentiles."""
+    slots, RM_OPTION_PRECISION flags `precision` and the schedule given (None:
+    the server's default, the one the plain path's context starts with);
+    returns throughput and latency percentiles."""
     from raymond_amd import capi, scenefile
     with capi.Server(slots=slots, memory_budget=budget) as server:
         server.set_option(capi.OPTION_PRECISION, precision)
         server.set_option(capi.OPTION_AUTOTUNE, autotune)
         server.set_option(capi.OPTION_BATCH, int(batch))
-        server.set_option(capi.OPTION_SCHEDULE, schedule)
+        if schedule is not None:
+            server.set_option(capi.OPTION_SCHEDULE, schedule)
         scenes = [server.context.upload_scene(p) for p in packs]
         jobs = []
         for arrival, k, (w, h), spp, bounces in trace:
@@ -629,6 +630,8 @@ def main() -> int:
                     help="path-pass schedule: one thread per pixel, live paths packed between bounces, "
                          "lanes regenerating paths from a tile queue, or a kernel a pixel for the first bounces "
                          "and persistent waves over the paths still alive")
+    ap.add_argument("--tune-frames", type=int, default=16,
+                    help="--schedule tuned: synchronised frames a candidate (each candidate is timed twice)")
     ap.add_argument("--split-depth", type=int, default=0,
                     help="bounces the split schedule's first kernel runs (RM_OPTION_SPLIT_DEPTH; 0: 2)")
     ap.add_argument("--waves", type=int, choices=[4, 5], default=4,
