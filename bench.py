@@ -416,15 +416,17 @@ def precision_flags(names: str) -> int:
 
 
 def run_trace(packs, trace, slots: int, budget: int, precision: int = 0, autotune: int = 0, batch: bool = False,
-              schedule: int | None = None) -> dict:
+              schedule: int | None = None, tune_mask: int = 0) -> dict:
     """Submits the trace's jobs at their arrival times to a server with `slots`
     slots, RM_OPTION_PRECISION flags `precision` and the schedule given (None:
-    the server's default, the one the plain path's context starts with);
-    returns throughput and latency percentiles."""
+    the server's default, the one the plain path's context starts with), and
+    with `autotune` frames a tuner candidate over the candidates `tune_mask`
+    names (0: all); returns throughput and latency percentiles."""
     from raymond_amd import capi, scenefile
     with capi.Server(slots=slots, memory_budget=budget) as server:
         server.set_option(capi.OPTION_PRECISION, precision)
         server.set_option(capi.OPTION_AUTOTUNE, autotune)
+        server.set_option(capi.OPTION_TUNE_CANDIDATES, tune_mask)
         server.set_option(capi.OPTION_BATCH, int(batch))
         if schedule is not None:
             server.set_option(capi.OPTION_SCHEDULE, schedule)
@@ -539,7 +541,7 @@ def served_vs_plain(args) -> dict:
                                                    max(busy_plain["Msamples_per_s"], 1e-9), 3)}}
 
 
-SERVING_VARIANTS = ("served", "served_batched", "served_batched_wavefront", "served_tuned",
+SERVING_VARIANTS = ("served", "served_batched", "served_batched_wavefront", "served_tuned", "served_tuned_split",
                     "served_batched_with_precision_variants", "served_one_slot", "plain")
 
 
@@ -561,6 +563,8 @@ def bench_serving(args) -> dict:
         "served_batched_wavefront": lambda: run_trace(packs, trace, args.slots, budget, batch=True,
                                                       schedule=capi.SCHEDULE_WAVEFRONT),
         "served_tuned": lambda: run_trace(packs, trace, args.slots, budget, autotune=1),
+        # The tuner's three split candidates only (four and five waves after two bounces, five after one).
+        "served_tuned_split": lambda: run_trace(packs, trace, args.slots, budget, autotune=1, tune_mask=0b111),
         "served_batched_with_precision_variants": lambda: run_trace(packs, trace, args.slots, budget, flags,
                                                                     batch=True),
         "served_one_slot": lambda: run_trace(packs, trace, 1, budget),

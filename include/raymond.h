@@ -70,7 +70,9 @@ extern "C" {
                                           a 64-pixel row); the same results bit for bit */
 #define RM_OPTION_SPLIT_DEPTH 9        /* RM_SCHEDULE_SPLIT: bounces the head kernel runs before it queues the
                                           paths still alive, 1 to 8 (0, the default: 2) */
-#define RM_OPTION_COUNT 10
+#define RM_OPTION_TUNE_CANDIDATES 10   /* rm_tune: bit k set tries candidate k (the order rm_tune lists them);
+                                          0 (the default): all of them */
+#define RM_OPTION_COUNT 11
 
 #define RM_BLOCK_SQUARES 0
 #define RM_BLOCK_ROWS 1
@@ -272,8 +274,9 @@ typedef struct rm_tune_result {
     uint32_t stack_depth[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_STACK_DEPTH */
     uint32_t block_shape[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_BLOCK_SHAPE */
     uint32_t split_depth[RM_TUNE_CANDIDATES];        /* and its RM_OPTION_SPLIT_DEPTH */
-    double ms[RM_TUNE_CANDIDATES];                   /* mean ms per whole frame */
-    uint32_t count;                                  /* candidates tried */
+    double ms[RM_TUNE_CANDIDATES];                   /* mean ms per whole frame; 0: not tried (the
+                                                        RM_OPTION_TUNE_CANDIDATES mask left it out) */
+    uint32_t count;                                  /* candidates listed (RM_TUNE_CANDIDATES) */
     uint32_t best;                                   /* the one now set */
 } rm_tune_result;
 int rm_tune(rm_context* ctx, const rm_scene* scene, const void* params, const float* emitters,

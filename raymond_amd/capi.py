@@ -55,7 +55,7 @@ SIGNATURES = {
     "rm_server_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
 }
 (OPTION_SCHEDULE, OPTION_PERSISTENT_BLOCKS, OPTION_PRECISION, OPTION_AUTOTUNE, OPTION_BATCH, OPTION_WAVES,
- OPTION_LIGHT_PATHS, OPTION_STACK_DEPTH, OPTION_BLOCK_SHAPE, OPTION_SPLIT_DEPTH) = range(10)
+ OPTION_LIGHT_PATHS, OPTION_STACK_DEPTH, OPTION_BLOCK_SHAPE, OPTION_SPLIT_DEPTH, OPTION_TUNE_CANDIDATES) = range(11)
 BLOCK_SQUARES, BLOCK_ROWS = 0, 1
 PRECISION_QUANTIZED_BVH, PRECISION_FP16_GUIDES, PRECISION_COMPENSATED = 1, 2, 4
 SCHEDULE_PIXEL, SCHEDULE_COMPACT, SCHEDULE_PERSISTENT, SCHEDULE_WAVEFRONT, SCHEDULE_SORTED, SCHEDULE_SPLIT = range(6)

@@ -451,6 +451,9 @@ int rm_set_option(rm_context* c, int option, int64_t value) {
                                     "_WAVEFRONT, _SORTED or _SPLIT.");
     if (option == RM_OPTION_SPLIT_DEPTH && (value < 0 || value > 8))
         return fail(RM_ERR_INVALID, "RM_OPTION_SPLIT_DEPTH is 1 to 8 bounces (0: the default, 2).");
+    if (option == RM_OPTION_TUNE_CANDIDATES && (value < 0 || value >= (int64_t)1 << RM_TUNE_CANDIDATES))
+        return fail(RM_ERR_INVALID, "RM_OPTION_TUNE_CANDIDATES is a mask of the " +
+                                        std::to_string(RM_TUNE_CANDIDATES) + " candidates (0: all).");
     if (option == RM_OPTION_PERSISTENT_BLOCKS && (value < 0 || value > (1 << 20)))
         return fail(RM_ERR_INVALID, "RM_OPTION_PERSISTENT_BLOCKS is 0 (automatic) to 2^20.");
     if (option == RM_OPTION_WAVES && value != 0 && value != 4 && value != 5)
@@ -1600,11 +1603,24 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
     };
     double best_ms = 0.0;
     int r = RM_OK;
-    // Two rounds over the candidates, each candidate keeping its faster
+    const uint32_t mask = c->options[RM_OPTION_TUNE_CANDIDATES] != 0
+                              ? (uint32_t)c->options[RM_OPTION_TUNE_CANDIDATES]
+                              : (1u << RM_TUNE_CANDIDATES) - 1u;
+    for (uint32_t k = 0; k < RM_TUNE_CANDIDATES; ++k) {
+        res.schedule[k] = schedules[k];
+        res.persistent_blocks[k] = blocks[k];
+        res.waves[k] = waves[k];
+        res.stack_depth[k] = depths[k];
+        res.block_shape[k] = shapes[k];
+        res.split_depth[k] = splits[k];
+    }
+    res.count = RM_TUNE_CANDIDATES;
+    // Two rounds over the candidates the mask names, each keeping its faster
     // round: the first candidate of the first round otherwise pays for the
     // GPU coming out of idle.
     for (uint32_t n = 0; n < 2 * RM_TUNE_CANDIDATES && r == RM_OK; ++n) {
         const uint32_t k = n % RM_TUNE_CANDIDATES;
+        if (!(mask >> k & 1u)) continue;
         c->options[RM_OPTION_SCHEDULE] = schedules[k];
         c->options[RM_OPTION_PERSISTENT_BLOCKS] = blocks[k];
         c->options[RM_OPTION_WAVES] = waves[k];
@@ -1624,19 +1640,14 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         if (r == RM_OK) r = finish_all();
         if (r != RM_OK) break;
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        res.schedule[k] = schedules[k];
-        res.persistent_blocks[k] = blocks[k];
-        res.waves[k] = waves[k];
-        res.stack_depth[k] = depths[k];
-        res.block_shape[k] = shapes[k];
-        res.split_depth[k] = splits[k];
         res.ms[k] = n < RM_TUNE_CANDIDATES ? ms / frames : std::min(res.ms[k], ms / frames);
-        res.count = std::max(res.count, k + 1);
     }
-    for (uint32_t k = 0; k < res.count; ++k)
-        if (k == 0 || res.ms[k] < best_ms) {
+    bool any = false;
+    for (uint32_t k = 0; k < RM_TUNE_CANDIDATES; ++k)
+        if ((mask >> k & 1u) && (!any || res.ms[k] < best_ms)) {
             best_ms = res.ms[k];
             res.best = k;
+            any = true;
         }
     const uint32_t pick = res.best;
     const int64_t picked[TUNED] = {res.schedule[pick],     res.persistent_blocks[pick], res.waves[pick],

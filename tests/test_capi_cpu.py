@@ -125,7 +125,7 @@ def test_python_and_js_constants_match_the_header(tmp_path):
     defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define (RM_\w+) \(?(-?\d+)\)?", header)}
     py = {f"RM_OPTION_{n}": getattr(capi, f"OPTION_{n}")
           for n in ("SCHEDULE", "PERSISTENT_BLOCKS", "PRECISION", "AUTOTUNE", "BATCH", "WAVES", "LIGHT_PATHS",
-                    "STACK_DEPTH", "BLOCK_SHAPE", "SPLIT_DEPTH")}
+                    "STACK_DEPTH", "BLOCK_SHAPE", "SPLIT_DEPTH", "TUNE_CANDIDATES")}
     py.update({f"RM_SCHEDULE_{n}": getattr(capi, f"SCHEDULE_{n}") for n in ("PIXEL", "COMPACT", "PERSISTENT",
                                                                             "WAVEFRONT", "SORTED", "SPLIT")})
     py.update({f"RM_BLOCK_{n}": getattr(capi, f"BLOCK_{n}") for n in ("SQUARES", "ROWS")})

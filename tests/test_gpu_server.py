@@ -189,6 +189,35 @@ def test_tune_picks_a_schedule_and_keeps_the_image(ctx, pack):
     assert np.array_equal(tuned.view(np.uint32), plain.view(np.uint32))
 
 
+def test_tune_tries_only_the_masked_candidates(ctx, pack):
+    """RM_OPTION_TUNE_CANDIDATES: the candidates whose bits are set are
+    timed, the others list 0 ms, and the pick is among the timed ones."""
+    p = pack("spheres", scene={"width": 64, "height": 48}, width=64, height=48)
+    params = p.with_params(caustics=0, denoise=0, firefly_clamp=0.0)
+    with pytest.raises(capi.RaymondError, match="mask"):
+        ctx.set_option(capi.OPTION_TUNE_CANDIDATES, 1 << capi.TUNE_CANDIDATES)
+    scene = ctx.upload_scene(p)
+    mask = 0b10000000101   # split at four waves, split after one bounce, wavefront
+    try:
+        ctx.upload_environment(p)
+        ctx.resize(p.width, p.height)
+        ctx.set_option(capi.OPTION_TUNE_CANDIDATES, mask)
+        result = ctx.tune(scene, params, frames=1)
+    finally:
+        scene.release()
+        ctx.set_option(capi.OPTION_TUNE_CANDIDATES, 0)
+        ctx.set_option(capi.OPTION_SCHEDULE, capi.SCHEDULE_DEFAULT)
+        ctx.set_option(capi.OPTION_PERSISTENT_BLOCKS, 0)
+        ctx.set_option(capi.OPTION_WAVES, 0)
+        ctx.set_option(capi.OPTION_STACK_DEPTH, 0)
+        ctx.set_option(capi.OPTION_BLOCK_SHAPE, 0)
+        ctx.set_option(capi.OPTION_SPLIT_DEPTH, 0)
+    timed = [k for k, c in enumerate(result["candidates"]) if c[6] > 0]
+    assert timed == [k for k in range(capi.TUNE_CANDIDATES) if mask >> k & 1]
+    best = min((result["candidates"][k] for k in timed), key=lambda c: c[6])
+    assert result["best"] == best[:6]
+
+
 def test_tune_times_whole_frames_with_light_tracing(ctx, pack):
     """With light tracing and the denoiser on, the tuner renders whole frames
     (the emitters it is given included); without the emitters those frames
