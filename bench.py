@@ -256,7 +256,10 @@ def bench_frames(args, name: str) -> dict:
                     p.meta["envTableHeight"])
     ctx.resize(w, h)
     emitters = p.emitters if caustics else None
-    tuning = ctx.tune(scene, frame_params(0), frames=args.tune_frames, emitters=emitters) if tuned else None
+    # Tuned with the first timed frame's parameters: its photon radius, not the first frame's (which gathers
+    # over about twice the area), sets the side stream's load the path pass shares the GPU with.
+    tuning = (ctx.tune(scene, frame_params(args.warmup), frames=args.tune_frames, emitters=emitters)
+              if tuned else None)
 
     k = 0
     for _ in range(args.warmup):

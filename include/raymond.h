@@ -249,9 +249,10 @@ int rm_reset_stage_times(rm_context* ctx);
  * context starts with RM_SCHEDULE_SPLIT and every other option at 0. */
 int rm_set_option(rm_context* ctx, int option, int64_t value);
 
-/* The autotuner. Renders `frames` whole frames after one warm-up, twice (the
- * candidates in turn, each keeping its faster time), with the caller's
- * parameters — light tracing (given its emitters) and the denoiser
+/* The autotuner. Renders `frames` whole frames to fill the pipeline, then
+ * times `frames` more by events on the path stream (no synchronisation in
+ * between), twice (the candidates in turn, each keeping its faster time),
+ * with the caller's parameters — light tracing (given its emitters) and the denoiser
  * included when they turn them on — under each candidate: split after two
  * bounces at four and at five waves per SIMD, and after one at five; one
  * thread per pixel at four and at five waves, at four with 16 stack entries

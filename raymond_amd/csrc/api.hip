@@ -1,7 +1,6 @@
 // The C ABI (include/raymond.h) over the kernels: device memory, the frame's
 // dispatch order (src/Renderer.mjs:441-489), readback and stage timing.
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -398,6 +397,8 @@ int rm_open(int device, rm_context** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->denoised_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&c->tune_ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&c->tune_ev[1]);
     c->owner.device = device;
     c->owner.streams[0] = c->stream;
     c->owner.streams[1] = c->side;
@@ -405,6 +406,8 @@ int rm_open(int device, rm_context** out) {
     for (DevBuf* b : working_buffers(c)) b->owner = &c->owner;
     for (DevBuf* b : {&c->env, &c->env_table, &c->work}) b->owner = &c->owner;
     if (e != hipSuccess) {
+        for (hipEvent_t ev : c->tune_ev)
+            if (ev) (void)hipEventDestroy(ev);
         if (c->side_ev) (void)hipEventDestroy(c->side_ev);
         if (c->batch_ev) (void)hipEventDestroy(c->batch_ev);
         if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
@@ -499,6 +502,8 @@ int rm_close(rm_context* c) {
             (void)hipEventDestroy(p.second);
         }
     for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->tune_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->side_ev) (void)hipEventDestroy(c->side_ev);
     if (c->batch_ev) (void)hipEventDestroy(c->batch_ev);
     if (c->denoised_ev) (void)hipEventDestroy(c->denoised_ev);
@@ -1617,7 +1622,7 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
     res.count = RM_TUNE_CANDIDATES;
     // Two rounds over the candidates the mask names, each keeping its faster
     // round: the first candidate of the first round otherwise pays for the
-    // GPU coming out of idle.
+    // GPU coming out of idle. Between candidates the streams drain.
     for (uint32_t n = 0; n < 2 * RM_TUNE_CANDIDATES && r == RM_OK; ++n) {
         const uint32_t k = n % RM_TUNE_CANDIDATES;
         if (!(mask >> k & 1u)) continue;
@@ -1627,20 +1632,24 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         c->options[RM_OPTION_STACK_DEPTH] = depths[k];
         c->options[RM_OPTION_BLOCK_SHAPE] = shapes[k];
         c->options[RM_OPTION_SPLIT_DEPTH] = splits[k];
-        std::chrono::steady_clock::time_point t0;
-        for (uint32_t f = 0; f <= frames && r == RM_OK; ++f) {
-            if (f == 1) {
-                r = finish_all();
-                t0 = std::chrono::steady_clock::now();
-            }
+        // `frames` frames to fill the pipeline (light tracing and the
+        // denoiser run a frame apart from the path pass on their own
+        // streams), then `frames` more timed by events on the path stream,
+        // with no synchronisation between: a candidate that is fast right
+        // after the streams drain can be slower once they overlap.
+        for (uint32_t f = 0; f < 2 * frames && r == RM_OK; ++f) {
+            if (f == frames) RM_HIP(hipEventRecord(c->tune_ev[0], c->stream), "tuning");
             P.sample_offset = f * spp;
             P.total_samples = (f + 1) * spp;
-            if (r == RM_OK) r = rm_frame(c, s, &P, emitters, emitter_vec4s);
+            r = rm_frame(c, s, &P, emitters, emitter_vec4s);
         }
+        if (r == RM_OK) RM_HIP(hipEventRecord(c->tune_ev[1], c->stream), "tuning");
         if (r == RM_OK) r = finish_all();
         if (r != RM_OK) break;
-        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        res.ms[k] = n < RM_TUNE_CANDIDATES ? ms / frames : std::min(res.ms[k], ms / frames);
+        float ms = 0.0f;
+        RM_HIP(hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]), "tuning");
+        const double per_frame = (double)ms / frames;
+        res.ms[k] = n < RM_TUNE_CANDIDATES ? per_frame : std::min(res.ms[k], per_frame);
     }
     bool any = false;
     for (uint32_t k = 0; k < RM_TUNE_CANDIDATES; ++k)

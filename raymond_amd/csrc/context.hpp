@@ -124,6 +124,7 @@ struct rm_context {
     hipEvent_t prepared_ev = nullptr, denoised_ev = nullptr;
     hipEvent_t batch_ev = nullptr;   // orders a batched path pass (rm_frame_batch) with this context's stream
     hipEvent_t side_ev = nullptr;    // orders batched light tracing (rm_frame_batch) with this context's side stream
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};   // rm_tune's timing events on the path stream
     bool denoise_pending = false;
     uint32_t width = 0, height = 0, capacity = 0;
     rmi::DevBuf accum, galb, gnorm, light_a, light_b, denoised, guides, albedo_frame;
