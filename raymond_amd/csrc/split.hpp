@@ -37,7 +37,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(W
     tile_pixel(x, y);
     const bool valid = x < P.width && y < P.height;
     const Ctx C{S, &P};
-    const uint32_t index = y * P.width + x;
+    const uint32_t index = valid ? y * P.width + x : 0u;
     auto stack = BlockStack<MAIN_LDS_DEPTH, TREE>::make(lds, spill, index, slots);
     Rng g;
     PathState ps;
