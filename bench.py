@@ -573,7 +573,12 @@ def bench_serving(args) -> dict:
         "served_one_slot": lambda: run_trace(packs, trace, 1, budget),
         "plain": lambda: run_plain(packs, trace),
     }
-    results = {k: runs[k]() for k in SERVING_VARIANTS if k in wanted}
+    results = {}
+    for k in SERVING_VARIANTS:
+        if k in wanted:
+            results[k] = runs[k]()
+            print(f"serving: {k} done, {results[k]['Msamples_per_s']} Msamples/s in {results[k]['wall_s']} s",
+                  file=sys.stderr, flush=True)
     served, plain = results["served"], results.get("plain")
     with capi.Context(0) as ctx:
         device, cus = ctx.device_info()

@@ -503,16 +503,24 @@ RM_D PointCells point_cells(const Params& P, const PhotonMap& M, V3 p) {
 // gather() (src/shaders.mjs:1534-1570) by one lane, in the reference's order:
 // the eight cells in turn, each newest deposit first (the reference's lists
 // are pushed at the head), at most 4096 photons in all. The run's slots are
-// known up front, so unrolled iterations load their photons together.
-RM_D V3 gather_lane(const Params& P, const PhotonMap& M, const GatherPoint& q, const PointCells& c, const Mat& m) {
+// known up front, so unrolled iterations load their photons together. The
+// cells move down one place a step, so the current one is always at index 0:
+// indexed only by constants, the arrays stay in registers.
+RM_D V3 gather_lane(const Params& P, const PhotonMap& M, const GatherPoint& q, PointCells c, const Mat& m) {
     const float r = P.photon_radius, r2 = r * r;
     V3 sum = {0, 0, 0};
     uint32_t visited = 0u;
     for (uint32_t k = 0u; k < 8u && visited < 4096u; k++) {
-        const uint32_t take = c.count[k] < 4096u - visited ? c.count[k] : 4096u - visited;
+        const uint32_t take = c.count[0] < 4096u - visited ? c.count[0] : 4096u - visited;
+        const uint32_t last = c.last[0];
 #pragma unroll 4
-        for (uint32_t i = 0u; i < take; i++) sum = sum + photon_term(M, c.last[k] - i, q, m, r, r2);
+        for (uint32_t i = 0u; i < take; i++) sum = sum + photon_term(M, last - i, q, m, r, r2);
         visited += take;
+#pragma unroll
+        for (uint32_t j = 0u; j < 7u; j++) {
+            c.count[j] = c.count[j + 1u];
+            c.last[j] = c.last[j + 1u];
+        }
     }
     return sum / (PI * r2);
 }
