@@ -249,9 +249,10 @@ int rm_reset_stage_times(rm_context* ctx);
  * context starts with RM_SCHEDULE_SPLIT and every other option at 0. */
 int rm_set_option(rm_context* ctx, int option, int64_t value);
 
-/* The autotuner. Renders `frames` whole frames to fill the pipeline, then
- * times `frames` more by events on the path stream (no synchronisation in
- * between), twice (the candidates in turn, each keeping its faster time),
+/* The autotuner. Renders `frames` whole frames to warm up, waits for the
+ * GPU, then times `frames` more by events on the path stream (from a drained
+ * start, as a caller that synchronises runs its frames), twice (the
+ * candidates in turn, each keeping its faster time),
  * with the caller's parameters — light tracing (given its emitters) and the denoiser
  * included when they turn them on — under each candidate: split after two
  * bounces at four and at five waves per SIMD, and after one at five; one

@@ -1580,7 +1580,6 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
     std::memcpy(&P, params, sizeof P);
     const uint32_t spp = P.samples_per_frame > 0 ? P.samples_per_frame : 1u;
     const uint32_t cus = (uint32_t)(c->compute_units > 0 ? c->compute_units : 1);
-    // The candidates: schedule, persistent blocks, waves, LDS stack depth, block shape.
     // The candidates: schedule, persistent blocks, waves, LDS stack depth,
     // block shape, split depth.
     const int32_t schedules[RM_TUNE_CANDIDATES] = {RM_SCHEDULE_SPLIT,   RM_SCHEDULE_SPLIT,   RM_SCHEDULE_SPLIT,
@@ -1632,13 +1631,21 @@ int rm_tune(rm_context* c, const rm_scene* s, const void* params, const float* e
         c->options[RM_OPTION_STACK_DEPTH] = depths[k];
         c->options[RM_OPTION_BLOCK_SHAPE] = shapes[k];
         c->options[RM_OPTION_SPLIT_DEPTH] = splits[k];
-        // `frames` frames to fill the pipeline (light tracing and the
-        // denoiser run a frame apart from the path pass on their own
-        // streams), then `frames` more timed by events on the path stream,
-        // with no synchronisation between: a candidate that is fast right
-        // after the streams drain can be slower once they overlap.
+        // `frames` frames to warm up, the streams drained, then `frames`
+        // more timed by events on the path stream. Light tracing, the gather
+        // and the denoiser run beside the path pass on their own streams and
+        // settle into a phase against it that the start decides and that
+        // then holds: timed from a drained start, as a caller that
+        // synchronises runs its frames, a candidate is timed in the phase it
+        // will run in. (Timed straight on from the first frames, the sphere
+        // frame at five waves ran in another phase, 7% faster than it ever
+        // ran after a drain.)
         for (uint32_t f = 0; f < 2 * frames && r == RM_OK; ++f) {
-            if (f == frames) RM_HIP(hipEventRecord(c->tune_ev[0], c->stream), "tuning");
+            if (f == frames) {
+                r = finish_all();
+                if (r != RM_OK) break;
+                RM_HIP(hipEventRecord(c->tune_ev[0], c->stream), "tuning");
+            }
             P.sample_offset = f * spp;
             P.total_samples = (f + 1) * spp;
             r = rm_frame(c, s, &P, emitters, emitter_vec4s);
