@@ -49,7 +49,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # Non-packed VALU issue peak of MI355X: 256 CUs × 4 SIMDs × 16 lanes × 2.4 GHz
 # = 39.3 T lane-operations/s (the datasheet's 157.3 TFLOP/s FP32 counts an
-# FMA as two flops and packed-FP32 issue, which this build does not use).
+# FMA as two flops and packed-FP32 issue, which the kernels are compiled without).
 VALU_PEAK_GLANE_OPS = 256 * 64 * 2.4
 SIMDS = 1024
 METRIC = "Msamples/s at 1080p 8-bounce; served p50/p99 latency vs plain, 1 MI355X"
@@ -653,7 +653,7 @@ def main() -> int:
     ap.add_argument("--persistent-blocks", type=int, default=0, help="blocks for the persistent pass (0: fill the GPU)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).", file=sys.stderr)
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for {args.gpus}).", file=sys.stderr)
         return 2
     if args.workload == RAYCAST:
         line = bench_raycast(args)
