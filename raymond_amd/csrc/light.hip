@@ -205,9 +205,12 @@ RM_D void light_paths(const DevScene& S, const Params& P, const float4* __restri
             for (uint32_t depth = 0u; alive && depth < P.max_bounces; depth++) {
                 Hit hit = trace<false>(S, o, d, INF, stack, nc);
                 if (hit.prim == NONE) break;
+                // The hit primitive's material alone ends the path at a
+                // light, or at a surface other than a delta one before any
+                // delta bounce, so those paths skip building the surface.
+                Mat m = load_mat(S, __float_as_uint(S.prims[(size_t)hit.prim * 4].y));
+                if (m.kind == LIGHT || (bounces == 0u && !is_delta(m))) break;
                 Surface s = surface<TEXTURES>(S, o, d, hit);
-                Mat m = load_mat(S, s.material);
-                if (m.kind == LIGHT) break;
                 V3 base = albedo<TEXTURES>(S, m, s.p, s.uv);
                 if (!is_delta(m)) {
                     if (bounces > 0u) {
