@@ -175,12 +175,25 @@ RM_D void light_paths(const DevScene& S, const Params& P, const float4* __restri
                         pick -= omc;
                         j++;
                     }
+                    // How many targets' cones hold d. Exactly, d is in the
+                    // cone of a target at `to` (|to|² = d2 > r², the origin
+                    // outside) iff s = d·to >= 0 and s² >= d2 - r². A target
+                    // is skipped (the test adds 0) where that fails by a
+                    // margin far beyond the test's rounding, a few ulps of
+                    // the cosines: s < 0 with s² > 1e-8 d2 (the computed
+                    // cosine is negative, so 1 - cos > 1 >= omc), or s >= 0
+                    // with s² < 0.999 (d2 - r²) - 1e-4 d2 (cos θ < cos α -
+                    // 3e-4). Only near-misses take the divisions.
                     float within = 1.0f;
                     for (uint32_t i = 0u; i < targets; i++) {
+                        if (i == j) continue;
                         float4 aim = E[first_target + i];
                         V3 to = xyz(aim) - o;
-                        float omc = sphere_omc(aim.w, dot(to, to));
-                        within += (i != j && 1.0f - dot(d, to / length(to)) <= omc) ? 1.0f : 0.0f;
+                        const float s = dot(d, to), d2 = dot(to, to);
+                        if (s < 0.0f ? s * s > 1e-8f * d2 : s * s < 0.999f * (d2 - aim.w * aim.w) - 1e-4f * d2)
+                            continue;
+                        float omc = sphere_omc(aim.w, d2);
+                        within += (1.0f - dot(d, to / length(to)) <= omc) ? 1.0f : 0.0f;
                     }
                     float cos_out = dot(d, n);
                     if (cos_out <= 0.0f) alive = false;
