@@ -972,7 +972,7 @@ static int light_end(rm_context* c, const Frame& F) {
         Timed t(c, RM_STAGE_GATHER, c->side);
         rm::launch_photon_sort(M, c->side);
         rm::launch_photon_gather(F.S, F.P, M, Q, c->gathered[F.pair].as<float4>(), c->side_spill.as<uint32_t>(),
-                                 F.side_slots, F.f, c->side);
+                                 F.side_slots, F.f, rm::side_waves(c->options[RM_OPTION_WAVES] == 5 ? 5 : 4), c->side);
     }
     RM_HIP(hipEventRecord(c->join, c->side), "joining the side stream");
     RM_HIP(hipGetLastError(), "launching light tracing");
@@ -1017,7 +1017,8 @@ static int frame_begin(rm_context* c, Frame& F, bool batched_light = false) {
             {
                 Timed t(c, RM_STAGE_LIGHT_TRACE, c->side);
                 rm::launch_light_trace(S, P, c->emitters.as<const float4>(), c->splats[pair].as<uint32_t>(), M,
-                                       c->side_spill.as<uint32_t>(), F.side_slots, F.photons, f, c->side);
+                                       c->side_spill.as<uint32_t>(), F.side_slots, F.photons, f,
+                                       rm::side_waves(c->options[RM_OPTION_WAVES] == 5 ? 5 : 4), c->side);
             }
             int r = light_end(c, F);
             if (r != RM_OK) return r;

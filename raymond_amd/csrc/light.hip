@@ -55,11 +55,15 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
     atomicAdd(&splats[index + 2u], to_u32(fx.z));
 }
 
-// The light-tracing and gather kernels are held to 96 VGPRs (five waves per
-// SIMD): they run beside the path pass, and a wave of theirs then fits where
-// one path-pass wave at five waves per SIMD (96 VGPRs) retires. At more than
-// 96, on the caustics scene, light tracing waited for two to retire and took
-// 2.5 ms a frame beside the path pass instead of 0.64.
+// The light-tracing and gather kernels run beside the path pass and are
+// compiled for one more wave per SIMD than it (side_waves): five (96 VGPRs)
+// beside a path pass at four, six (80 VGPRs) beside one at five, so a wave of
+// theirs fits where a path-pass wave retires. At more than 96, on the
+// caustics scene, light tracing waited for two to retire and took 2.5 ms a
+// frame beside the path pass instead of 0.64; at six beside a path pass at
+// five, the 8 M-photon caustics frame ran 4% faster than at five.
+// k_light_trace_batch, beside the batched path pass (four waves), stays at
+// five.
 //
 // Light paths (src/shaders.mjs:1630-1814), photons_n of them a frame: the
 // reference's photon_count(P) by default, or a budget the caller sets. A
@@ -267,8 +271,8 @@ RM_D void light_paths(const DevScene& S, const Params& P, const float4* __restri
     }
 }
 
-template <bool TEXTURES, bool DISPERSION, int TREE>
-__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_light_trace(
+template <bool TEXTURES, bool DISPERSION, int TREE, int SW>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(SW))) void k_light_trace(
     DevScene S, Params P, const float4* __restrict__ E, uint32_t* splats, PhotonMap M, uint32_t* spill,
     uint32_t slots, uint32_t photons_n) {
     __shared__ uint2 lds[MAIN_LDS_DEPTH * BLOCK_THREADS];
@@ -549,8 +553,8 @@ RM_D float4 gathered_light(V3 throughput, V3 radiance) {
 
 // Every pixel's camera ray (the reference's gather stream) to its first hit,
 // and on (see above).
-template <bool TEXTURES, int TREE>
-__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_begin(DevScene S, Params P, PhotonMap M, GatherQueue Q,
+template <bool TEXTURES, int TREE, int SW>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(SW))) void k_gather_begin(DevScene S, Params P, PhotonMap M, GatherQueue Q,
                                                                 float4* __restrict__ gathered, uint32_t* spill,
                                                                 uint32_t slots) {
     __shared__ uint2 lds[GATHER_LDS_DEPTH * BLOCK_THREADS];
@@ -615,8 +619,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
 
 // The queued paths on to their gather points, a thread a path, each point
 // written over its path's slot.
-template <bool TEXTURES, int TREE>
-__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_paths(DevScene S, Params P, GatherQueue Q,
+template <bool TEXTURES, int TREE, int SW>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(SW))) void k_gather_paths(DevScene S, Params P, GatherQueue Q,
                                                                 float4* __restrict__ gathered, uint32_t* spill,
                                                                 uint32_t slots) {
     __shared__ uint2 lds[GATHER_LDS_DEPTH * BLOCK_THREADS];
@@ -653,7 +657,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
 // a time. The lanes' partial sums are added in a fixed tree; the photons'
 // order within a cell follows their deposits, which the reference leaves to
 // the order its atomics land in.
-__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_walk(DevScene S, Params P, PhotonMap M, GatherQueue Q,
+template <int SW>
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(SW))) void k_gather_walk(DevScene S, Params P, PhotonMap M, GatherQueue Q,
                                                                float4* __restrict__ gathered) {
     const uint32_t front = Q.count[0], points = front + Q.count[1];
     const uint32_t sub = __lane_id() & 7u;
@@ -717,27 +722,34 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(5
 }
 
 // ------------------------------------------------------------------ launchers
-template <bool T, bool D>
+template <bool T, bool D, int SW>
 static void launch_light_trace_v(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
                                  const PhotonMap& M, uint32_t* spill, uint32_t slots, dim3 grid, uint32_t count,
                                  int tree, hipStream_t stream) {
     with_tree(tree, [&](auto t) {
-        hipLaunchKernelGGL((k_light_trace<T, D, decltype(t)::value>), grid, dim3(BLOCK_THREADS), 0, stream, S, P,
+        hipLaunchKernelGGL((k_light_trace<T, D, decltype(t)::value, SW>), grid, dim3(BLOCK_THREADS), 0, stream, S, P,
                            emitters, splats, M, spill, slots, count);
     });
+}
+template <int SW>
+static void launch_light_trace_w(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
+                                 const PhotonMap& M, uint32_t* spill, uint32_t slots, dim3 grid, uint32_t count,
+                                 const Features& f, hipStream_t stream) {
+    if (f.textures)
+        launch_light_trace_v<true, true, SW>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
+    else if (f.dispersion)
+        launch_light_trace_v<false, true, SW>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
+    else
+        launch_light_trace_v<false, false, SW>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
 }
 
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
                         const PhotonMap& M, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
-                        hipStream_t stream) {
+                        int side_waves, hipStream_t stream) {
     const dim3 grid(light_blocks(count));
     if (grid.x == 0) return;
-    if (f.textures)
-        launch_light_trace_v<true, true>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
-    else if (f.dispersion)
-        launch_light_trace_v<false, true>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
-    else
-        launch_light_trace_v<false, false>(S, P, emitters, splats, M, spill, slots, grid, count, f.tree, stream);
+    if (side_waves == 6) launch_light_trace_w<6>(S, P, emitters, splats, M, spill, slots, grid, count, f, stream);
+    else launch_light_trace_w<5>(S, P, emitters, splats, M, spill, slots, grid, count, f, stream);
 }
 
 template <bool T, bool D>
@@ -778,8 +790,10 @@ void launch_fold(const Params& P, const float4* accum, const float4* galb, const
                        accum, galb, gnorm, comp, out);
 }
 
-void launch_photon_gather(const DevScene& S, const Params& P, const PhotonMap& M, const GatherQueue& Q,
-                          float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, hipStream_t stream) {
+template <int SW>
+static void launch_photon_gather_w(const DevScene& S, const Params& P, const PhotonMap& M, const GatherQueue& Q,
+                                   float4* gathered, uint32_t* spill, uint32_t slots, const Features& f,
+                                   hipStream_t stream) {
     const uint32_t px = P.width * P.height;
     const dim3 tiles((P.width + 15) / 16, (P.height + 15) / 16);
     // The queue's length is known on the device only: grids for at most
@@ -796,10 +810,16 @@ void launch_photon_gather(const DevScene& S, const Params& P, const PhotonMap& M
     };
     with_tree(f.tree, [&](auto t) {
         constexpr int TREE = decltype(t)::value;
-        if (f.textures) go(k_gather_begin<true, TREE>, k_gather_paths<true, TREE>);
-        else go(k_gather_begin<false, TREE>, k_gather_paths<false, TREE>);
+        if (f.textures) go(k_gather_begin<true, TREE, SW>, k_gather_paths<true, TREE, SW>);
+        else go(k_gather_begin<false, TREE, SW>, k_gather_paths<false, TREE, SW>);
     });
-    hipLaunchKernelGGL(k_gather_walk, dim3(walk_blocks), dim3(BLOCK_THREADS), 0, stream, S, P, M, Q, gathered);
+    hipLaunchKernelGGL(k_gather_walk<SW>, dim3(walk_blocks), dim3(BLOCK_THREADS), 0, stream, S, P, M, Q, gathered);
+}
+void launch_photon_gather(const DevScene& S, const Params& P, const PhotonMap& M, const GatherQueue& Q,
+                          float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, int side_waves,
+                          hipStream_t stream) {
+    if (side_waves == 6) launch_photon_gather_w<6>(S, P, M, Q, gathered, spill, slots, f, stream);
+    else launch_photon_gather_w<5>(S, P, M, Q, gathered, spill, slots, f, stream);
 }
 
 }  // namespace rm
