@@ -258,7 +258,8 @@ int rm_set_option(rm_context* ctx, int option, int64_t value);
  * bounces at four and at five waves per SIMD, and after one at five; one
  * thread per pixel at four and at five waves, at four with 16 stack entries
  * in LDS and with row-shaped blocks; compacted; sorted; persistent at two
- * grid sizes; wavefront. It sets the fastest as the context's
+ * grid sizes; wavefront (in that order of preference: a later candidate
+ * must be at least 1% faster than the best before it). It sets the fastest as the context's
  * RM_OPTION_SCHEDULE, RM_OPTION_PERSISTENT_BLOCKS, RM_OPTION_WAVES,
  * RM_OPTION_STACK_DEPTH, RM_OPTION_BLOCK_SHAPE and RM_OPTION_SPLIT_DEPTH.
  * Whole frames, because the path pass shares the GPU with light tracing and

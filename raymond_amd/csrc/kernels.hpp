@@ -205,15 +205,15 @@ inline uint64_t photon_map_bytes(uint64_t capacity) {
 // at most one photon into M (whose counts start at zero); the grid is at most
 // LIGHT_LANES threads, and `spill` has room for that many lanes.
 constexpr uint32_t LIGHT_LANES = 1u << 20;
-// side_waves (5 or 6): the waves per SIMD light tracing and the gather are
-// compiled for, one more than the path pass beside them runs (side_waves()).
+// side_waves (5 or 7): the waves per SIMD light tracing and the gather are
+// compiled for beside the path pass (side_waves()).
 void launch_light_trace(const DevScene& S, const Params& P, const float4* emitters, uint32_t* splats,
                         const PhotonMap& M, uint32_t* spill, uint32_t slots, uint32_t count, const Features& f,
                         int side_waves, hipStream_t stream);
 // Light tracing's and the gather's waves per SIMD beside a path pass of
-// `path_waves`: one more, so that a wave of theirs (80 VGPRs at six, 96 at
-// five) fits where a path-pass wave retires.
-inline int side_waves(int path_waves) { return path_waves == 5 ? 6 : 5; }
+// `path_waves`: more than it, so that a wave of theirs fits where a path-pass
+// wave retires — five (96 VGPRs) beside four, seven (72 VGPRs) beside five.
+inline int side_waves(int path_waves) { return path_waves == 5 ? 7 : 5; }
 // Blocks of light tracing's grid for `count` light paths: one path a thread
 // up to LIGHT_LANES, then each thread takes several.
 inline uint32_t light_blocks(uint32_t count) {

@@ -56,12 +56,13 @@ RM_D void splat(const Ctx& C, Rng& g, STACK& stack, uint32_t* splats, const Surf
 }
 
 // The light-tracing and gather kernels run beside the path pass and are
-// compiled for one more wave per SIMD than it (side_waves): five (96 VGPRs)
-// beside a path pass at four, six (80 VGPRs) beside one at five, so a wave of
-// theirs fits where a path-pass wave retires. At more than 96, on the
+// compiled for more waves per SIMD than it (side_waves): five (96 VGPRs)
+// beside a path pass at four, seven (72 VGPRs) beside one at five, so a wave
+// of theirs fits where a path-pass wave retires. At more than 96, on the
 // caustics scene, light tracing waited for two to retire and took 2.5 ms a
-// frame beside the path pass instead of 0.64; at six beside a path pass at
-// five, the 8 M-photon caustics frame ran 4% faster than at five.
+// frame beside the path pass instead of 0.64; at seven beside a path pass at
+// five, the 8 M-photon caustics frame ran 5% faster than at five, and at six
+// beside a path pass at four, the sphere frame 0.5% slower.
 // k_light_trace_batch, beside the batched path pass (four waves), stays at
 // five.
 //
@@ -748,7 +749,7 @@ void launch_light_trace(const DevScene& S, const Params& P, const float4* emitte
                         int side_waves, hipStream_t stream) {
     const dim3 grid(light_blocks(count));
     if (grid.x == 0) return;
-    if (side_waves == 6) launch_light_trace_w<6>(S, P, emitters, splats, M, spill, slots, grid, count, f, stream);
+    if (side_waves == 7) launch_light_trace_w<7>(S, P, emitters, splats, M, spill, slots, grid, count, f, stream);
     else launch_light_trace_w<5>(S, P, emitters, splats, M, spill, slots, grid, count, f, stream);
 }
 
@@ -818,7 +819,7 @@ static void launch_photon_gather_w(const DevScene& S, const Params& P, const Pho
 void launch_photon_gather(const DevScene& S, const Params& P, const PhotonMap& M, const GatherQueue& Q,
                           float4* gathered, uint32_t* spill, uint32_t slots, const Features& f, int side_waves,
                           hipStream_t stream) {
-    if (side_waves == 6) launch_photon_gather_w<6>(S, P, M, Q, gathered, spill, slots, f, stream);
+    if (side_waves == 7) launch_photon_gather_w<7>(S, P, M, Q, gathered, spill, slots, f, stream);
     else launch_photon_gather_w<5>(S, P, M, Q, gathered, spill, slots, f, stream);
 }
 
